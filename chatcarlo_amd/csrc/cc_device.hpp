@@ -1,0 +1,589 @@
+// Device-side building blocks of the gfx950 photon-transport path.
+//
+// Float32 everywhere except the FP64 tallies and collision accumulators.
+// Cancellation-prone geometry terms (|o-c|^2 - r^2, o.d) are evaluated in
+// float-float arithmetic (Dekker 1971: error-free TwoSum / FMA TwoProd), so a
+// float32 kernel reproduces the FP64 roots to ~1e-7 relative.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/chatcarlo_hip.h"
+
+namespace cc {
+
+constexpr float kHC = 12.3984193f;      // chatcarlo/physics.py:35
+constexpr float kMEC2 = 511.0f;         // chatcarlo/physics.py:34
+constexpr float kFluorCut = 5.0f;       // chatcarlo/physics.py:38
+constexpr float kTwoPi = 6.28318530717958647692f;
+constexpr int kMaxEdges = 16;
+constexpr int kFlStride = 20;
+constexpr int kNParts = 5;
+enum { P_PHOTO = 0, P_COMPT = 1, P_RAYL = 2, P_MUEN = 3, P_TOTAL = 4 };
+
+struct Prim {
+  int kind, axis, material, pad;
+  float p[8];
+};
+struct World {
+  float lo[3], hi[3];
+  int background, n_prim;
+};
+
+struct Tables {
+  int n_mat, n_e, n_el, n_q, n_h10;
+  const float* density;
+  const float* e_node;
+  const float* dl_node;
+  const float2* ss;          // [n_mat][5][n_e] (sig, slope)
+  const int* edge_idx;       // [n_mat][n_e]
+  const float* edge_e;       // [n_mat][16]
+  const float* edge_dl;      // [n_mat][16][2]
+  const float* edge_sig;     // [n_mat][16][3][2]
+  const float* edge_slope;   // [n_mat][16][3][2]
+  const int* mat_elem_off;
+  const int* mat_elem;
+  const int* sel_off;
+  const float* sel;
+  const float* sq_over_z;    // [n_el][n_q]
+  const float* q_s;
+  const float* ray_x;
+  const float* ray_a;        // [n_el][n_q]
+  const float* fl;           // [n_el][20]
+  const float* h10;          // [4][n_h10]
+};
+
+// ---------------------------------------------------------------------------
+// float-float helpers
+// ---------------------------------------------------------------------------
+struct ff { float h, l; };
+__device__ __forceinline__ ff two_sum(float a, float b) {
+  float s = a + b;
+  float bb = s - a;
+  float e = (a - (s - bb)) + (b - bb);
+  return {s, e};
+}
+__device__ __forceinline__ ff two_prod(float a, float b) {
+  float p = a * b;
+  return {p, __builtin_fmaf(a, b, -p)};
+}
+__device__ __forceinline__ ff ff_add(ff a, ff b) {
+  ff s = two_sum(a.h, b.h);
+  float l = s.l + a.l + b.l;
+  return two_sum(s.h, l);
+}
+__device__ __forceinline__ ff ff_neg(ff a) { return {-a.h, -a.l}; }
+__device__ __forceinline__ ff ff_mul(ff a, ff b) {
+  ff p = two_prod(a.h, b.h);
+  float l = p.l + (a.h * b.l + a.l * b.h);
+  return two_sum(p.h, l);
+}
+__device__ __forceinline__ ff ff_sqr(ff a) { return ff_mul(a, a); }
+__device__ __forceinline__ float ff_f(ff a) { return a.h + a.l; }
+__device__ __forceinline__ ff ff_sqrt(ff a) {
+  if (a.h <= 0.f) return {0.f, 0.f};
+  float s = sqrtf(a.h);
+  // one Newton step: s' = s + (a - s^2) / (2 s)
+  ff s2 = two_prod(s, s);
+  float r = ((a.h - s2.h) - s2.l) + a.l;
+  return two_sum(s, r / (2.f * s));
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+    k.x += 0x9E3779B9u;
+    k.y += 0xBB67AE85u;
+  }
+  return c;
+}
+__device__ __forceinline__ float u01(uint32_t x) {   // (0, 1], float32-exact
+  return (float)((x >> 8) + 1u) * 5.9604644775390625e-8f;
+}
+
+// Per-history stream: counter (history lo, history hi, event counter, lane tag), key = seed.
+struct PhiloxRng {
+  uint2 key;
+  uint32_t h_lo, h_hi, tag, ctr;
+  float buf[4];
+  int pos;
+  __device__ PhiloxRng(uint64_t seed, uint64_t hist, uint32_t tag_, uint32_t ctr_)
+      : key(make_uint2((uint32_t)seed, (uint32_t)(seed >> 32))), h_lo((uint32_t)hist),
+        h_hi((uint32_t)(hist >> 32)), tag(tag_), ctr(ctr_), pos(4) {}
+  __device__ __forceinline__ float next() {
+    if (pos == 4) {
+      uint4 r = philox4x32_10(make_uint4(h_lo, h_hi, ctr, tag), key);
+      ++ctr;
+      buf[0] = u01(r.x); buf[1] = u01(r.y); buf[2] = u01(r.z); buf[3] = u01(r.w);
+      pos = 0;
+    }
+    return buf[pos++];
+  }
+};
+
+// Caller-supplied uniforms (sub-kernel parity tests).
+struct BufRng {
+  const float* u;
+  int n, pos;
+  __device__ BufRng(const float* u_, int n_) : u(u_), n(n_), pos(0) {}
+  __device__ __forceinline__ float next() { return pos < n ? u[pos++] : 1.0f; }
+  __device__ __forceinline__ float at(int slot) const { return slot < n ? u[slot] : 1.0f; }
+};
+
+// ---------------------------------------------------------------------------
+// geometry (chatcarlo/geometry.py:20-231)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void slab(const float* lo, const float* hi, const float o[3],
+                                     const float d[3], float& ten, float& tex, bool& hit) {
+  ten = -INFINITY;
+  tex = INFINITY;
+  bool miss = false;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (fabsf(d[k]) < 1e-12f) {
+      miss |= (o[k] < lo[k]) || (o[k] > hi[k]);
+    } else {
+      float ta = (lo[k] - o[k]) / d[k];
+      float tb = (hi[k] - o[k]) / d[k];
+      ten = fmaxf(ten, fminf(ta, tb));
+      tex = fminf(tex, fmaxf(ta, tb));
+    }
+  }
+  hit = !miss && (ten <= tex);
+}
+
+// Stable quadratic a t^2 + 2 b t + c = 0 with b, c, disc in float-float.
+__device__ __forceinline__ bool quad_roots(float a, ff b, ff c, float& t1, float& t2) {
+  ff disc = ff_add(ff_mul(b, b), ff_neg(ff_mul({a, 0.f}, c)));
+  if (disc.h < 0.f || (disc.h == 0.f && disc.l < 0.f)) return false;
+  ff sq = ff_sqrt(disc);
+  float sb = (b.h >= 0.f) ? 1.f : -1.f;
+  ff q = ff_neg(ff_add(b, {sb * sq.h, sb * sq.l}));   // q = -(b + sign(b) sqrt(disc))
+  float qf = ff_f(q);
+  float r1, r2;
+  if (qf == 0.f) {
+    r1 = r2 = 0.f;
+  } else {
+    r1 = qf / a;
+    r2 = ff_f(c) / qf;
+  }
+  t1 = fminf(r1, r2);
+  t2 = fmaxf(r1, r2);
+  return true;
+}
+
+__device__ __forceinline__ bool isect_sphere(const Prim& P, const float o[3], const float d[3],
+                                             float& t1, float& t2) {
+  ff b = {0.f, 0.f}, cc = {0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    ff oc = two_sum(o[k], -P.p[k]);
+    b = ff_add(b, ff_mul(oc, {d[k], 0.f}));
+    cc = ff_add(cc, ff_sqr(oc));
+  }
+  cc = ff_add(cc, ff_neg(two_prod(P.p[3], P.p[3])));
+  // |d| = 1 up to rounding: a = d.d
+  float a = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  (void)a;
+  return quad_roots(1.0f, b, cc, t1, t2);
+}
+
+__device__ __forceinline__ bool isect_cylinder(const Prim& P, const float o[3], const float d[3],
+                                               float& t1, float& t2) {
+  const int ax = P.axis;
+  const int i0 = (ax == 0) ? 1 : 0;
+  const int i1 = (ax == 2) ? 1 : 2;
+  ff ox = two_sum(o[i0], -P.p[i0]);
+  ff oy = two_sum(o[i1], -P.p[i1]);
+  float dx = d[i0], dy = d[i1];
+  float a = dx * dx + dy * dy;
+  ff b = ff_add(ff_mul(ox, {dx, 0.f}), ff_mul(oy, {dy, 0.f}));
+  ff cc = ff_add(ff_add(ff_sqr(ox), ff_sqr(oy)), ff_neg(two_prod(P.p[3], P.p[3])));
+  float ten_r, tex_r;
+  bool miss_r = false;
+  if (a < 1e-12f) {
+    if (cc.h < 0.f || (cc.h == 0.f && cc.l <= 0.f)) {
+      ten_r = -INFINITY;
+      tex_r = INFINITY;
+    } else {
+      miss_r = true;
+      ten_r = tex_r = 0.f;
+    }
+  } else {
+    if (!quad_roots(a, b, cc, ten_r, tex_r)) miss_r = true;
+  }
+  // caps at c_ax +- half_h, plane offsets formed exactly in float-float
+  float dz = d[ax];
+  ff lo_rel = ff_add(two_sum(P.p[ax], -P.p[4]), {-o[ax], 0.f});
+  ff hi_rel = ff_add(two_sum(P.p[ax], P.p[4]), {-o[ax], 0.f});
+  float ten_z, tex_z;
+  bool miss_z = false;
+  if (fabsf(dz) < 1e-12f) {
+    // o outside [lo, hi]  <=>  lo - o > 0 or hi - o < 0
+    miss_z = (lo_rel.h > 0.f || (lo_rel.h == 0.f && lo_rel.l > 0.f)) ||
+             (hi_rel.h < 0.f || (hi_rel.h == 0.f && hi_rel.l < 0.f));
+    ten_z = -INFINITY;
+    tex_z = INFINITY;
+  } else {
+    float ta = ff_f(lo_rel) / dz, tb = ff_f(hi_rel) / dz;
+    ten_z = fminf(ta, tb);
+    tex_z = fmaxf(ta, tb);
+  }
+  if (miss_r || miss_z) return false;
+  t1 = fmaxf(ten_r, ten_z);
+  t2 = fminf(tex_r, tex_z);
+  return !(t1 > t2);
+}
+
+__device__ __forceinline__ bool isect_prim(const Prim& P, const float o[3], const float d[3],
+                                           float& t1, float& t2) {
+  if (P.kind == CC_PRIM_BOX) {
+    bool hit;
+    slab(&P.p[0], &P.p[3], o, d, t1, t2, hit);
+    return hit;
+  }
+  if (P.kind == CC_PRIM_SPHERE) return isect_sphere(P, o, d, t1, t2);
+  return isect_cylinder(P, o, d, t1, t2);
+}
+
+// sign of (|p - c|^2 - r^2) <= 0 evaluated in float-float
+__device__ __forceinline__ bool contains(const Prim& P, const float q[3]) {
+  if (P.kind == CC_PRIM_BOX) {
+    return q[0] >= P.p[0] && q[0] <= P.p[3] && q[1] >= P.p[1] && q[1] <= P.p[4] &&
+           q[2] >= P.p[2] && q[2] <= P.p[5];
+  }
+  if (P.kind == CC_PRIM_SPHERE) {
+    ff s = {0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s = ff_add(s, ff_sqr(two_sum(q[k], -P.p[k])));
+    s = ff_add(s, ff_neg(two_prod(P.p[3], P.p[3])));
+    return s.h < 0.f || (s.h == 0.f && s.l <= 0.f);
+  }
+  const int ax = P.axis;
+  const int i0 = (ax == 0) ? 1 : 0;
+  const int i1 = (ax == 2) ? 1 : 2;
+  ff s = ff_add(ff_sqr(two_sum(q[i0], -P.p[i0])), ff_sqr(two_sum(q[i1], -P.p[i1])));
+  s = ff_add(s, ff_neg(two_prod(P.p[3], P.p[3])));
+  if (!(s.h < 0.f || (s.h == 0.f && s.l <= 0.f))) return false;
+  ff z = two_sum(q[ax], -P.p[ax]);
+  ff za = (z.h < 0.f || (z.h == 0.f && z.l < 0.f)) ? ff_neg(z) : z;
+  ff e = ff_add(za, {-P.p[4], 0.f});
+  return e.h < 0.f || (e.h == 0.f && e.l <= 0.f);
+}
+
+// Last primitive containing the point wins (chatcarlo/geometry.py:205-212).
+__device__ __forceinline__ int locate(const Prim* prims, int n_prim, int background, const float q[3]) {
+  for (int i = n_prim - 1; i >= 0; --i)
+    if (contains(prims[i], q)) return prims[i].material;
+  return background;
+}
+
+// Min positive root over all primitives vs the world exit (chatcarlo/geometry.py:214-231).
+__device__ __forceinline__ float next_boundary(const Prim* prims, int n_prim, const World& W,
+                                               const float o[3], const float d[3], float eps,
+                                               bool& escape) {
+  float t_obj = INFINITY;
+  for (int i = 0; i < n_prim; ++i) {
+    float t1, t2;
+    if (isect_prim(prims[i], o, d, t1, t2)) {
+      if (t1 > eps && t1 < t_obj) t_obj = t1;
+      if (t2 > eps && t2 < t_obj) t_obj = t2;
+    }
+  }
+  float ten, tex;
+  bool hit;
+  slab(W.lo, W.hi, o, d, ten, tex, hit);
+  if (!hit) tex = INFINITY;
+  escape = tex <= t_obj;
+  return fminf(t_obj, tex);
+}
+
+// ---------------------------------------------------------------------------
+// cross-section lookup (chatcarlo/materials.py:228-262 restated on the pack)
+// ---------------------------------------------------------------------------
+struct EPos {
+  int i;       // interval
+  float lr;    // ln(E / E_i)
+  float f;     // lr / ln(E_{i+1}/E_i)
+};
+
+__device__ __forceinline__ EPos energy_pos(const Tables& T, float E) {
+  const float inv_dln = (float)(T.n_e - 1) / 5.0106352940962555f;   // ln(150)
+  int i = (int)(logf(E) * inv_dln);
+  i = min(max(i, 0), T.n_e - 2);
+  if (E < T.e_node[i] && i > 0) --i;
+  else if (E >= T.e_node[i + 1] && i < T.n_e - 2) ++i;
+  float ei = T.e_node[i];
+  EPos p;
+  p.i = i;
+  p.lr = log1pf((E - ei) / ei);
+  p.f = p.lr / T.dl_node[i];
+  return p;
+}
+
+// part value; edge parts: P_PHOTO -> 0, P_MUEN -> 1, P_TOTAL -> 2
+__device__ __forceinline__ float xs_part(const Tables& T, int m, int part, float E, const EPos& p) {
+  float2 s = T.ss[((size_t)m * kNParts + part) * T.n_e + p.i];
+  int ep = (part == P_PHOTO) ? 0 : (part == P_MUEN) ? 1 : (part == P_TOTAL) ? 2 : -1;
+  if (ep >= 0) {
+    int k = T.edge_idx[m * T.n_e + p.i];
+    if (k >= 0) {
+      const int r = m * kMaxEdges + k;
+      float ee = T.edge_e[r];
+      if (E < ee) {
+        float fb = p.lr / T.edge_dl[r * 2 + 0];
+        return s.x * expf(fb * T.edge_slope[(r * 3 + ep) * 2 + 0]);
+      }
+      float fa = log1pf((E - ee) / ee) / T.edge_dl[r * 2 + 1];
+      return T.edge_sig[(r * 3 + ep) * 2 + 1] * expf(fa * T.edge_slope[(r * 3 + ep) * 2 + 1]);
+    }
+  }
+  return s.x * expf(p.f * s.y);
+}
+
+__device__ __forceinline__ bool energy_ok(float E) { return E >= 1.0f && E <= 150.0f; }
+
+// h*(10)/Phi [pSv cm^2], log-log with clamping (chatcarlo/dose_coefficients.py:32-42)
+__device__ __forceinline__ float h10_coeff(const float* h10, int n, float E) {
+  const float* he = h10;
+  const float* hv = h10 + n;
+  const float* hd = h10 + 2 * n;
+  const float* hs = h10 + 3 * n;
+  float ec = fminf(fmaxf(E, he[0]), he[n - 1]);
+  int j = 0;
+  while (j < n - 2 && ec >= he[j + 1]) ++j;
+  float f = logf(ec / he[j]) / hd[j];
+  return hv[j] * expf(f * hs[j]);
+}
+
+// ---------------------------------------------------------------------------
+// samplers (chatcarlo/physics.py), generic over the uniform source
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int select_elem(const Tables& T, int m, int kind, const EPos& p, float r) {
+  const int off = T.mat_elem_off[m];
+  const int ne = T.mat_elem_off[m + 1] - off;
+  const float* c0 = T.sel + T.sel_off[m] + ((size_t)kind * T.n_e + p.i) * ne;
+  const float* c1 = c0 + ne;
+  int k = 0;
+  for (int j = 0; j < ne; ++j) {
+    float c = c0[j] + p.f * (c1[j] - c0[j]);
+    k += (r > c) ? 1 : 0;
+  }
+  k = min(k, ne - 1);
+  return T.mat_elem[off + k];
+}
+
+// np.interp(x, xp, fp) on a sorted table; caller gives a starting guess j0.
+__device__ __forceinline__ float interp_guess(const float* xp, const float* fp, int n, float x, int j) {
+  if (x <= xp[0]) return fp[0];
+  if (x >= xp[n - 1]) return fp[n - 1];
+  j = min(max(j, 0), n - 2);
+  while (j > 0 && x < xp[j]) --j;
+  while (j < n - 2 && x >= xp[j + 1]) ++j;
+  float x0 = xp[j], x1 = xp[j + 1];
+  return fp[j] + (fp[j + 1] - fp[j]) / (x1 - x0) * (x - x0);
+}
+
+template <class Rng>
+__device__ __forceinline__ float compton_sample(const Tables& T, int m, float E, const EPos& p, Rng& rng,
+                                                int kmax, float& cos_t, int& el, int& trials) {
+  el = select_elem(T, m, P_COMPT, p, rng.next());
+  const float a = E / kMEC2;
+  const float emin = 1.0f / (1.0f + 2.0f * a);
+  const float env = 1.0f / emin + emin;
+  const float* sq = T.sq_over_z + (size_t)el * T.n_q;
+  const float dq = (20.0f - 1e-3f) / (float)(T.n_q - 2);
+  for (int t = 0; t < kmax; ++t) {
+    float x1 = rng.next(), x2 = rng.next(), x3 = rng.next();
+    float eps = emin + x1 * (1.0f - emin);
+    float omc = (1.0f / eps - 1.0f) / a;
+    float cs = 1.0f - omc;
+    float sin2 = omc * (2.0f - omc);
+    float g = 1.0f / eps + eps - sin2;
+    bool acc = x2 * env <= g;
+    float omcc = fminf(fmaxf(omc, 0.0f), 2.0f);
+    float q = E * sqrtf(omcc * 0.5f) / kHC;
+    int jg = (q < 1e-3f) ? 0 : 1 + (int)((q - 1e-3f) / dq);
+    float soz = interp_guess(T.q_s, sq, T.n_q, q, jg);
+    acc = acc && (x3 <= soz);
+    if (acc) {
+      cos_t = cs;
+      trials = t + 1;
+      return eps;
+    }
+  }
+  trials = 0;
+  cos_t = 1.0f;
+  return 1.0f;
+}
+
+// returns cos theta; trials = 0 when the cap is hit; qerr set if x_max beyond the table
+template <class Rng>
+__device__ __forceinline__ float rayleigh_sample(const Tables& T, int m, float E, const EPos& p, Rng& rng,
+                                                 int kmax, int& el, int& trials, bool& qerr) {
+  el = select_elem(T, m, P_RAYL, p, rng.next());
+  const int nq = T.n_q;
+  const float* xg = T.ray_x;
+  const float* ag = T.ray_a + (size_t)el * nq;
+  float qm = E / kHC;
+  float x_max = qm * qm;
+  qerr = x_max > xg[nq - 1];
+  const float dqr = 20.0f / (float)(nq - 1);
+  float a_cut = interp_guess(xg, ag, nq, x_max, (int)(qm / dqr));
+  for (int t = 0; t < kmax; ++t) {
+    float x1 = rng.next(), x2 = rng.next();
+    float target = x1 * a_cut;
+    // largest j with A[j] <= target (np.interp bracket)
+    int lo = 0, hi = nq - 1;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (ag[mid] <= target) lo = mid; else hi = mid;
+    }
+    float x;
+    if (target >= ag[nq - 1]) x = xg[nq - 1];
+    else x = xg[lo] + (xg[lo + 1] - xg[lo]) / (ag[lo + 1] - ag[lo]) * (target - ag[lo]);
+    x = fminf(x, x_max);
+    float c = fminf(fmaxf(1.0f - 2.0f * x / x_max, -1.0f), 1.0f);
+    if (x2 <= (1.0f + c * c) * 0.5f) {
+      trials = t + 1;
+      return c;
+    }
+  }
+  trials = 0;
+  return 1.0f;
+}
+
+// K fluorescence (chatcarlo/physics.py:155-212); slots u0..u3
+template <class Rng>
+__device__ __forceinline__ bool fluor_sample(const Tables& T, int m, float E, const EPos& p, Rng& rng,
+                                             int& el, float& e_line) {
+  float u0 = rng.next(), u1 = rng.next(), u2 = rng.next(), u3 = rng.next();
+  el = select_elem(T, m, P_PHOTO, p, u0);
+  const float* f = T.fl + (size_t)el * kFlStride;
+  e_line = 0.0f;
+  float edge = f[0];
+  if (!(edge > 0.0f) || !(E > edge)) return false;
+  if (!(u1 < f[2])) return false;
+  if (!(u2 < f[1])) return false;
+  int li = 0;
+  for (int l = 0; l < 8; ++l) li += (u3 > f[12 + l]) ? 1 : 0;
+  li = min(li, 7);
+  float ln = f[4 + li];
+  if (ln >= kFluorCut) {
+    e_line = ln;
+    return true;
+  }
+  return false;
+}
+
+// New direction from cos theta and phi = 2 pi u (chatcarlo/physics.py:354-367)
+__device__ __forceinline__ void rotate_dir(float d[3], float cos_t, float u_phi) {
+  float st = sqrtf(fmaxf(1.0f - cos_t * cos_t, 0.0f));
+  float sp, cp;
+  sincosf(kTwoPi * u_phi, &sp, &cp);
+  float up0, up1, up2;
+  if (fabsf(d[2]) < 0.999f) { up0 = 0.f; up1 = 0.f; up2 = 1.f; }
+  else { up0 = 1.f; up1 = 0.f; up2 = 0.f; }
+  float u0 = up1 * d[2] - up2 * d[1];
+  float u1 = up2 * d[0] - up0 * d[2];
+  float u2 = up0 * d[1] - up1 * d[0];
+  float inv = 1.0f / sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
+  u0 *= inv; u1 *= inv; u2 *= inv;
+  float v0 = d[1] * u2 - d[2] * u1;
+  float v1 = d[2] * u0 - d[0] * u2;
+  float v2 = d[0] * u1 - d[1] * u0;
+  float a = st * cp, b = st * sp;
+  float n0 = a * u0 + b * v0 + cos_t * d[0];
+  float n1 = a * u1 + b * v1 + cos_t * d[1];
+  float n2 = a * u2 + b * v2 + cos_t * d[2];
+  float inv2 = 1.0f / sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
+  d[0] = n0 * inv2; d[1] = n1 * inv2; d[2] = n2 * inv2;
+}
+
+// chatcarlo/physics.py:215-220
+__device__ __forceinline__ void isotropic_dir(float d[3], float u1, float u2) {
+  float c = -1.0f + 2.0f * u1;
+  float s = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+  float sp, cp;
+  sincosf(kTwoPi * u2, &sp, &cp);
+  d[0] = s * cp; d[1] = s * sp; d[2] = c;
+}
+
+// ---------------------------------------------------------------------------
+// track-length tally: exact 3-D DDA (Amanatides & Woo 1987) with FP64 atomics
+// ---------------------------------------------------------------------------
+struct GridDev {
+  double* dk;     // batch kerma [keV]
+  double* dh;     // batch H*(10) track integral [pSv cm^3]
+  int nx, ny, nz;
+  float org[3];
+  float h;
+  float inv_h;
+};
+
+__device__ __forceinline__ void atomic_add_f64(double* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Scores wk*len into dk and wh*len into dh for every voxel the segment
+// o + t d, t in [0, L], overlaps.  Returns the number of voxel visits.
+__device__ __forceinline__ int dda_score(const GridDev& G, const float o[3], const float d[3], float L,
+                                         double wk, double wh) {
+  const int n[3] = {G.nx, G.ny, G.nz};
+  float t0 = 0.0f, t1 = L;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float lo = G.org[k], hi = G.org[k] + (float)n[k] * G.h;
+    if (fabsf(d[k]) < 1e-30f) {
+      if (o[k] < lo || o[k] > hi) return 0;
+    } else {
+      float ta = (lo - o[k]) / d[k], tb = (hi - o[k]) / d[k];
+      t0 = fmaxf(t0, fminf(ta, tb));
+      t1 = fminf(t1, fmaxf(ta, tb));
+    }
+  }
+  if (!(t1 > t0)) return 0;
+  int idx[3], step[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    // voxel holding the clipped start point; a start that rounds onto the
+    // wrong side of a plane only moves a zero-length piece
+    int ik = (int)floorf((o[k] + d[k] * t0 - G.org[k]) * G.inv_h);
+    idx[k] = min(max(ik, 0), n[k] - 1);
+    step[k] = (d[k] > 0.f) ? 1 : ((d[k] < 0.f) ? -1 : 0);
+  }
+  float tc = t0;
+  int visits = 0;
+  const int cap = G.nx + G.ny + G.nz + 4;
+  for (int it = 0; it < cap; ++it) {
+    // parametric distance to the next plane on each axis, recomputed from the index
+    float tm[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float plane = G.org[k] + (float)(idx[k] + (step[k] > 0 ? 1 : 0)) * G.h;
+      tm[k] = step[k] != 0 ? (plane - o[k]) / d[k] : INFINITY;
+    }
+    int a = (tm[0] < tm[1]) ? ((tm[0] < tm[2]) ? 0 : 2) : ((tm[1] < tm[2]) ? 1 : 2);
+    float tn = fminf(fmaxf(tm[a], tc), t1);
+    double len = (double)tn - (double)tc;
+    if (len > 0.0) {
+      size_t v = ((size_t)idx[0] * G.ny + idx[1]) * G.nz + idx[2];
+      atomic_add_f64(G.dk + v, wk * len);
+      atomic_add_f64(G.dh + v, wh * len);
+      ++visits;
+    }
+    tc = tn;
+    if (tn >= t1) break;
+    idx[a] += step[a];
+    if (idx[a] < 0 || idx[a] >= n[a]) break;
+  }
+  return visits;
+}
+
+}  // namespace cc

@@ -1,0 +1,180 @@
+/* chatcarlo_hip.h -- C-ABI of the MI355X (gfx950) photon-transport library.
+ *
+ * The reference has no FFI: its hot path is the Python API
+ *   run_transport(scene, n_histories, seed, batch_size, dose_grid, grid_resolution_cm,
+ *                 n_workers, track_uncertainty)        chatcarlo/transport.py:358-361
+ *   transport_photons(pos, dirv, energy, geometry, rng, grid, recorder, tally_rng,
+ *                     fluorescence_enabled)             chatcarlo/transport.py:98-103
+ *   VoxelGrid / accumulate_track_length / end_batch     chatcarlo/tally.py:26-94, 248-293
+ *   Geometry.material_at / next_boundary                chatcarlo/geometry.py:205-231
+ * The Python package chatcarlo_amd re-exports those names and calls the entry
+ * points below through ctypes.  Conventions: every function returns an int
+ * status (CC_OK = 0, negative = error class); cc_last_error() gives the message
+ * of the last failure on the calling thread.  Host buffers are caller-owned
+ * (pointer + length); device buffers are owned by the opaque context.  One
+ * host thread per context; all work of a context runs on its own HIP stream.
+ */
+#ifndef CHATCARLO_HIP_H
+#define CHATCARLO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CC_OK = 0,
+  CC_ERR_ARG = -1,     /* invalid argument (sizes, ids, ranges) -> ValueError */
+  CC_ERR_HIP = -2,     /* HIP runtime failure */
+  CC_ERR_STATE = -3,   /* call order (no scene / tables / grid bound) */
+  CC_ERR_DEVICE = -4   /* a kernel raised a device error bit (see cc_run_out.error_bits) */
+};
+
+/* device error bits reported in cc_run_out.error_bits */
+enum {
+  CC_DEV_ENERGY_RANGE = 1,   /* energy outside [1,150] keV (chatcarlo/materials.py:234-237) */
+  CC_DEV_ITER_CAP = 2,       /* rejection loop hit its trial cap */
+  CC_DEV_Q_RANGE = 4,        /* Rayleigh x_max beyond the form-factor table (physics.py:333-337) */
+  CC_DEV_STEP_CAP = 8,       /* boundary-crossing cap of one flight hit */
+  CC_DEV_ROUND_CAP = 16      /* round cap of one batch hit */
+};
+
+enum { CC_PRIM_BOX = 0, CC_PRIM_SPHERE = 1, CC_PRIM_CYLINDER = 2 };
+enum { CC_MAX_PRIMS = 64, CC_MAX_MAT = 32 };
+
+/* One analytic primitive (48 bytes).  box: p[0..2] = lo, p[3..5] = hi;
+ * sphere: p[0..2] = centre, p[3] = radius; cylinder: p[0..2] = centre,
+ * p[3] = radius, p[4] = half height, axis 0/1/2 = x/y/z.
+ * Replaces the scene geometry list of chatcarlo/geometry.py:121-164. */
+typedef struct {
+  int32_t kind, axis, material, pad;
+  float p[8];
+} cc_prim;
+
+typedef struct {
+  float lo[3], hi[3];        /* world box (chatcarlo/geometry.py:160-164) */
+  int32_t background;        /* material id outside every primitive */
+  int32_t n_prim;
+} cc_world;
+
+/* Physics data pack (host pointers, float32 unless noted).  Layouts:
+ * sig/slope [n_mat][5][n_e] parts photo, compton, rayleigh, mu_en, total;
+ * edge_* [n_mat][max_edges](...); sel per material (3, n_e, n_elem) at sel_off[m];
+ * fl [n_el][20] = edge, omega, k_frac, pad, lines[8], cdf[8];
+ * h10 [4][n_h10]: energies, values, ln-ratios of energies, ln-ratios of values.
+ * Replaces chatcarlo/materials.py:202-496 and dose_coefficients.py:26-42. */
+typedef struct {
+  int32_t n_mat, n_e, n_el, n_q, max_edges, n_h10, sel_len, elem_len;
+  const float* density;
+  const float* e_node;
+  const float* dl_node;
+  const float* sig;
+  const float* slope;
+  const int32_t* edge_of_interval;
+  const float* edge_e;
+  const float* edge_dl;
+  const float* edge_sig;
+  const float* edge_slope;
+  const int32_t* mat_elem_off;   /* n_mat + 1 */
+  const int32_t* mat_elem;
+  const int32_t* sel_off;        /* n_mat */
+  const float* sel;
+  const float* sq_over_z;
+  const float* q_s;
+  const float* ray_x;
+  const float* ray_a;
+  const float* fl;
+  const float* h10;
+} cc_tables;
+
+enum { CC_FIELD_RECT = 0, CC_FIELD_CONE = 1, CC_FIELD_PARALLEL = 2 };
+
+/* Source (chatcarlo/source.py:67-210 without heel, spectrum.py:73-82). */
+typedef struct {
+  int32_t field, n_spec, n_angles, rotation;   /* rotation: 0 none, 1 continuous, 2 discrete */
+  double pos[3], dir[3], u[3], v[3];
+  double sid, width, height, cos_half;
+  double iso[3], axis[3];
+  double scan_length;
+  const double* spec_e;      /* n_spec energies keV */
+  const double* spec_cdf;    /* n_spec cumulative probabilities, last = 1 */
+} cc_source;
+
+enum {
+  CC_RUN_FLUORESCENCE = 1,     /* physics.fluorescence (chatcarlo/transport.py:388) */
+  CC_RUN_GRID = 2,             /* score the bound grid */
+  CC_RUN_TRACK_UNCERTAINTY = 4, /* batch moments (chatcarlo/tally.py:59-94) */
+  CC_RUN_TIMING = 8             /* per-kernel HIP-event timing into cc_run_out.kernel_ms */
+};
+
+typedef struct {
+  uint64_t n_histories, n_absorbed, n_escaped, scatter_sum, n_fluorescence, n_batches;
+  uint64_t n_events;            /* flights + interactions processed */
+  uint32_t error_bits, rounds_max;
+  double edep_keV[CC_MAX_MAT];  /* collision estimator per material */
+  double edep_sum2[CC_MAX_MAT]; /* sum over batches of S_b^2 / n_b */
+  double kernel_ms[8];          /* source, flight, photo, compton, rayleigh, end_batch, other, spare */
+  uint64_t kernel_launches[8];  /* launches timed per kernel kind */
+} cc_run_out;
+
+const char* cc_last_error(void);
+int cc_version(int32_t* major, int32_t* minor);
+
+int cc_ctx_create(int32_t device, void** ctx);
+int cc_ctx_destroy(void* ctx);
+int cc_upload_tables(void* ctx, const cc_tables* t);
+int cc_upload_scene(void* ctx, const cc_prim* prims, int32_t n_prim, const cc_world* world);
+
+/* Grid: origin (3), shape (nx, ny, nz), cubic voxel size; flags bit 1 = track_uncertainty.
+ * Replaces VoxelGrid.from_bbox / __post_init__ (chatcarlo/tally.py:42-57). */
+int cc_bind_grid(void* ctx, const double* origin, const int32_t* shape, double voxel_cm, int32_t flags);
+int cc_unbind_grid(void* ctx);
+int cc_reset_grid(void* ctx);
+/* Copy the totals and moments back (any pointer may be NULL). */
+int cc_read_grid(void* ctx, double* kerma_keV, double* h10_pSv_cm3, double* kerma_sum2,
+                 double* h10_sum2, int32_t* n_batches_hit, int64_t* n_batches, int64_t* n_histories);
+/* Add caller arrays into the device totals (resume / merge). */
+int cc_load_grid(void* ctx, const double* kerma_keV, const double* h10_pSv_cm3, const double* kerma_sum2,
+                 const double* h10_sum2, const int32_t* n_batches_hit, int64_t n_batches, int64_t n_histories);
+
+/* Full run: histories [first_history, first_history + n) in batches of `batch`;
+ * Philox keyed by seed.  Replaces run_transport's serial branch
+ * (chatcarlo/transport.py:257-312, 383-396). */
+int cc_run(void* ctx, const cc_source* src, uint64_t n_histories, uint64_t seed, uint64_t first_history,
+           uint32_t batch, uint32_t flags, cc_run_out* out);
+
+/* Caller-supplied photons, updated in place (chatcarlo/transport.py:98-233).
+ * pos, dir: n x 3; energy: n.  Outputs per photon: n_scatter, absorbed, escaped. */
+int cc_transport(void* ctx, int64_t n, double* pos, double* dir, double* energy, uint64_t seed,
+                 uint64_t first_history, uint32_t flags, int32_t* n_scatter, uint8_t* absorbed,
+                 uint8_t* escaped, cc_run_out* out);
+
+/* Geometry queries on the GPU (chatcarlo/geometry.py:205-231). */
+int cc_locate(void* ctx, int64_t n, const float* pts, int32_t* material);
+int cc_test_boundary(void* ctx, int64_t n, const float* o, const float* d, float eps, float* t,
+                     int32_t* escape);
+
+/* Uniform-fed sub-kernels: the production device functions fed caller uniforms. */
+/* out per photon: mu, p_photo, p_compt, photo, compton, rayleigh, mu_en, mu_total, h10 */
+int cc_test_xs(void* ctx, int64_t n, const int32_t* mat, const float* e, float* out9);
+int cc_test_compton(void* ctx, int64_t n, const int32_t* mat, const float* e, const float* u,
+                    int32_t k_trials, float* eps, float* cos_t, int32_t* elem, int32_t* trials);
+int cc_test_rayleigh(void* ctx, int64_t n, const int32_t* mat, const float* e, const float* u,
+                     int32_t k_trials, float* cos_t, int32_t* elem, int32_t* trials);
+int cc_test_fluor(void* ctx, int64_t n, const int32_t* mat, const float* e, const float* u4,
+                  int32_t* elem, int32_t* emit, float* e_line);
+int cc_test_rotate(int64_t n, const float* d, const float* cos_t, const float* u_phi, float* out);
+int cc_test_isotropic(int64_t n, const float* u1, const float* u2, float* out);
+/* Score segments (o, d, length, weights per cm) into the bound grid's batch buffers and fold them. */
+int cc_test_dda(void* ctx, int64_t n, const float* o, const float* d, const float* len,
+                const double* w_kerma, const double* w_h10);
+int cc_test_philox(int64_t n, const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4);
+/* Source sampling for histories [first, first + n): pos, dir (n x 3), energy (n). */
+int cc_test_source(void* ctx, const cc_source* src, int64_t n, uint64_t seed, uint64_t first_history,
+                   float* pos, float* dir, float* energy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

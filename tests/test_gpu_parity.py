@@ -1,0 +1,281 @@
+"""GPU sub-kernel parity: production device functions fed the oracle's inputs.
+
+Every kernel here is the one the transport path runs, called through the
+C-ABI test entry points with caller-supplied uniforms; the oracle evaluates the
+same formulas in FP64 on the same float32 tables and inputs.  Tolerances:
+1e-6 relative for real-valued outputs (absolute 1e-6 for quantities in
+[-1, 1] such as cosines and probabilities), exact for integer outputs except
+for the stated handful of accept/reject decisions that sit within float32
+rounding of their threshold.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import core
+from oracle.philox import philox4x32_10, to_uniform
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(20261203)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _lib():
+    from chatcarlo_amd import _native as N
+    return N, N.load()
+
+
+def test_philox_known_answers_and_random(engine):
+    N, lib = _lib()
+    kat_c = np.array([[0, 0, 0, 0], [0xFFFFFFFF] * 4, [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344]],
+                     dtype=np.uint32)
+    kat_k = np.array([[0, 0], [0xFFFFFFFF] * 2, [0xA4093822, 0x299F31D0]], dtype=np.uint32)
+    kat_o = np.array([[0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8],
+                      [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD],
+                      [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]], dtype=np.uint32)
+    c = np.concatenate([kat_c, RNG.integers(0, 2 ** 32, (4096, 4), dtype=np.uint32)])
+    k = np.concatenate([kat_k, RNG.integers(0, 2 ** 32, (4096, 2), dtype=np.uint32)])
+    out = np.zeros_like(c)
+    N.check(lib.cc_test_philox(len(c), N.ptr(c, C.c_uint32), N.ptr(k, C.c_uint32), N.ptr(out, C.c_uint32)))
+    np.testing.assert_array_equal(out[:3], kat_o)
+    np.testing.assert_array_equal(out, philox4x32_10(c, k))
+
+
+def _energies(n, pack):
+    e = np.exp(RNG.uniform(np.log(1.0), np.log(150.0), n))
+    # add points straddling every material's edges and the node energies
+    extra = []
+    for m in range(len(pack.materials)):
+        for k in range(pack.edge_e.shape[1]):
+            ee = pack.edge_e[m, k]
+            if ee > 0:
+                extra += [ee * (1 - 1e-6), ee, ee * (1 + 1e-6), ee * 0.999, ee * 1.001]
+    extra += list(pack.e_node[::37]) + [1.0, 150.0]
+    e = np.concatenate([e, np.clip(extra, 1.0, 150.0)])
+    return f32(e).astype(np.float64)
+
+
+def test_xs_lookup_parity(engine, dpack):
+    N, lib = _lib()
+    e = _energies(20000, dpack)
+    n = len(e)
+    mat = RNG.integers(0, len(dpack.materials), n).astype(np.int32)
+    out = np.zeros((n, 9), dtype=np.float32)
+    ef = f32(e)
+    N.check(lib.cc_test_xs(engine.ctx, n, N.ptr(mat, C.c_int32), N.ptr(ef, C.c_float), N.ptr(out, C.c_float)))
+    ref = core.xs_lookup(dpack, mat, e)
+    h10 = core.h10_coeff_dev(dpack, e)
+    for col, key in ((0, "mu"), (3, "photo"), (4, "compt"), (5, "rayl"), (6, "mu_en"), (7, "mu_total")):
+        np.testing.assert_allclose(out[:, col], ref[key], rtol=1e-6, err_msg=key)
+    np.testing.assert_allclose(out[:, 1], ref["p_photo"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(out[:, 2], ref["p_compt"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(out[:, 8], h10, rtol=1e-6)
+    # the device-form h*(10) equals the reference form log-log interpolation
+    np.testing.assert_allclose(h10, core.h10_coeff(dpack, e), rtol=1e-6)
+
+
+def test_xs_rejects_out_of_range_energy(engine):
+    N, lib = _lib()
+    mat = np.zeros(2, dtype=np.int32)
+    e = f32([0.5, 60.0])
+    out = np.zeros((2, 9), dtype=np.float32)
+    with pytest.raises(ValueError):
+        N.check(lib.cc_test_xs(engine.ctx, 2, N.ptr(mat, C.c_int32), N.ptr(e, C.c_float), N.ptr(out, C.c_float)))
+
+
+def _test_scene():
+    from chatcarlo_amd.geometry import Geometry
+    geoms = [
+        {"shape": "box", "material": "concrete", "center": [0, 0, 0], "size_cm": [240, 220, 200]},
+        {"shape": "box", "material": "air", "center": [0, 0, 0], "size_cm": [200, 180, 160]},
+        {"shape": "box", "material": "water", "center": [10, -5, 3], "size_cm": [30, 20, 50]},
+        {"shape": "cylinder", "material": "bone", "center": [10, -5, 3], "radius_cm": 1.5, "height_cm": 40,
+         "axis": "z"},
+        {"shape": "cylinder", "material": "pmma", "center": [-40, 30, 0], "radius_cm": 16, "height_cm": 15,
+         "axis": "x"},
+        {"shape": "cylinder", "material": "aluminum", "center": [-40, -50, 20], "radius_cm": 5, "height_cm": 30,
+         "axis": "y"},
+        {"shape": "sphere", "material": "lead", "center": [50, 40, -30], "radius_cm": 12},
+        {"shape": "box", "material": "lead", "center": [0, 60, 0], "size_cm": [80, 0.05, 80]},
+        {"shape": "sphere", "material": "water", "center": [50, 40, -30], "radius_cm": 6},
+    ]
+    return Geometry(geoms)
+
+
+def test_locate_bit_exact(engine, dpack):
+    g = _test_scene()
+    prims, world = g.packed(dpack.mat_id)
+    og = core.OGeom.from_packed(prims, world)
+    pts = f32(RNG.uniform(g.bbox_min, g.bbox_max, (200000, 3))).astype(np.float64)
+    got = engine.locate(g, pts)
+    np.testing.assert_array_equal(got, og.material_at(pts))
+
+
+def test_boundary_parity(engine, dpack):
+    g = _test_scene()
+    prims, world = g.packed(dpack.mat_id)
+    og = core.OGeom.from_packed(prims, world)
+    n = 200000
+    o = f32(RNG.uniform(g.bbox_min + 1, g.bbox_max - 1, (n, 3))).astype(np.float64)
+    d = RNG.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = f32(d).astype(np.float64)
+    eps = 1e-6
+    t, esc, _ = engine.boundary(g, o, d, eps)
+    t_ref, esc_ref = og.next_boundary(o, d, eps)
+    rel = np.abs(t - t_ref) / t_ref
+    # well-conditioned rays: 1e-6 relative
+    assert np.mean(rel <= 1e-6) > 0.999, np.sort(rel)[-20:]
+    # every ray: within 1e-6 relative plus a float32 position-ulp floor
+    assert np.all(np.abs(t - t_ref) <= 1e-6 * t_ref + 2e-5), np.max(np.abs(t - t_ref) - 1e-6 * t_ref)
+    agree = esc == esc_ref
+    assert np.mean(agree) > 0.9999
+
+
+def test_compton_parity(engine, dpack):
+    N, lib = _lib()
+    n, k = 20000, 24
+    e = _energies(n, dpack)[:n]
+    mat = RNG.integers(0, len(dpack.materials), n).astype(np.int32)
+    u = f32(1.0 - RNG.random((n, 1 + 3 * k)))
+    eps = np.zeros(n, np.float32); cs = np.zeros(n, np.float32)
+    el = np.zeros(n, np.int32); tr = np.zeros(n, np.int32)
+    ef = f32(e)
+    N.check(lib.cc_test_compton(engine.ctx, n, N.ptr(mat, C.c_int32), N.ptr(ef, C.c_float), N.ptr(u, C.c_float), k,
+                                N.ptr(eps, C.c_float), N.ptr(cs, C.c_float), N.ptr(el, C.c_int32),
+                                N.ptr(tr, C.c_int32)))
+    r_eps, r_cos, r_el, r_tr = core.compton_u(dpack, mat, e, u.astype(np.float64), k)
+    same = (tr == r_tr) & (el == r_el)
+    assert np.sum(~same) <= 2, np.where(~same)[0][:10]
+    ok = same & (r_tr > 0)
+    np.testing.assert_allclose(eps[ok], r_eps[ok], rtol=1e-6)
+    np.testing.assert_allclose(cs[ok], r_cos[ok], rtol=0, atol=1e-6)
+
+
+def test_rayleigh_parity(engine, dpack):
+    N, lib = _lib()
+    n, k = 20000, 16
+    e = _energies(n, dpack)[:n]
+    mat = RNG.integers(0, len(dpack.materials), n).astype(np.int32)
+    u = f32(1.0 - RNG.random((n, 1 + 2 * k)))
+    cs = np.zeros(n, np.float32); el = np.zeros(n, np.int32); tr = np.zeros(n, np.int32)
+    ef = f32(e)
+    N.check(lib.cc_test_rayleigh(engine.ctx, n, N.ptr(mat, C.c_int32), N.ptr(ef, C.c_float), N.ptr(u, C.c_float),
+                                 k, N.ptr(cs, C.c_float), N.ptr(el, C.c_int32), N.ptr(tr, C.c_int32)))
+    r_cos, r_el, r_tr = core.rayleigh_u(dpack, mat, e, u.astype(np.float64), k)
+    same = (tr == r_tr) & (el == r_el)
+    assert np.sum(~same) <= 2
+    ok = same & (r_tr > 0)
+    np.testing.assert_allclose(cs[ok], r_cos[ok], rtol=0, atol=1e-5)
+    assert np.mean(np.abs(cs[ok] - r_cos[ok]) <= 1e-6) > 0.999
+
+
+def test_fluorescence_parity(engine, dpack):
+    N, lib = _lib()
+    n = 50000
+    mids = [dpack.mat_id(x) for x in ("lead", "tungsten", "copper", "iron", "concrete", "bone", "water")]
+    mat = np.asarray(RNG.choice(mids, n), dtype=np.int32)
+    e = f32(RNG.uniform(5.0, 150.0, n)).astype(np.float64)
+    u = f32(1.0 - RNG.random((n, 4)))
+    el = np.zeros(n, np.int32); em = np.zeros(n, np.int32); ln = np.zeros(n, np.float32)
+    ef = f32(e)
+    N.check(lib.cc_test_fluor(engine.ctx, n, N.ptr(mat, C.c_int32), N.ptr(ef, C.c_float), N.ptr(u, C.c_float),
+                              N.ptr(el, C.c_int32), N.ptr(em, C.c_int32), N.ptr(ln, C.c_float)))
+    r_el, r_em, r_ln = core.fluor_u(dpack, mat, e, u.astype(np.float64))
+    assert np.sum(el != r_el) <= 2
+    assert np.sum(em.astype(bool) != r_em) <= 2
+    same = (el == r_el) & (em.astype(bool) == r_em)
+    np.testing.assert_allclose(ln[same], r_ln[same], rtol=1e-6)
+    assert r_em.sum() > 1000
+
+
+def test_rotate_and_isotropic_parity(engine):
+    N, lib = _lib()
+    n = 100000
+    d = RNG.normal(size=(n, 3))
+    d[: n // 10] = [0.0, 0.0, 1.0]      # frame switch branch |d_z| >= 0.999
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = f32(d)
+    cs = f32(RNG.uniform(-1, 1, n))
+    up = f32(RNG.random(n))
+    out = np.zeros((n, 3), np.float32)
+    N.check(lib.cc_test_rotate(n, N.ptr(d, C.c_float), N.ptr(cs, C.c_float), N.ptr(up, C.c_float),
+                               N.ptr(out, C.c_float)))
+    ref = core.rotate_u(d.astype(np.float64), cs.astype(np.float64), up.astype(np.float64))
+    np.testing.assert_allclose(out, ref, rtol=0, atol=2e-6)
+    np.testing.assert_allclose(np.sum(out * d, axis=1), cs, atol=2e-6)
+    u1, u2 = f32(RNG.random(n)), f32(RNG.random(n))
+    N.check(lib.cc_test_isotropic(n, N.ptr(u1, C.c_float), N.ptr(u2, C.c_float), N.ptr(out, C.c_float)))
+    np.testing.assert_allclose(out, core.isotropic_u(u1.astype(float), u2.astype(float)), rtol=0, atol=2e-6)
+
+
+def test_dda_matches_exact_overlap(engine):
+    N, lib = _lib()
+    origin, shape, vox = np.array([-10.0, -7.5, -5.0]), (20, 15, 10), 1.0
+    engine.bind_grid(origin, shape, vox, False)
+    n = 20000
+    o = f32(RNG.uniform(-14, 14, (n, 3)))
+    d = RNG.normal(size=(n, 3))
+    d[:200] = [1, 0, 0]                  # axis-aligned rays
+    d[200:400] = [0, -1, 0]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = f32(d)
+    L = f32(RNG.exponential(8.0, n))
+    wk = RNG.uniform(0.5, 2.0, n)
+    wh = RNG.uniform(0.5, 2.0, n)
+    N.check(lib.cc_test_dda(engine.ctx, n, N.ptr(o, C.c_float), N.ptr(d, C.c_float), N.ptr(L, C.c_float),
+                            N.ptr(wk, C.c_double), N.ptr(wh, C.c_double)))
+    g = engine.read_grid(shape, moments=False)
+    ref_k = np.zeros(shape)
+    ref_h = np.zeros(shape)
+    core.overlap_tally(ref_k, origin, vox, o.astype(float), d.astype(float), L.astype(float), wk)
+    core.overlap_tally(ref_h, origin, vox, o.astype(float), d.astype(float), L.astype(float), wh)
+    np.testing.assert_allclose(g["kerma"].sum(), ref_k.sum(), rtol=1e-6)
+    np.testing.assert_allclose(g["h10"].sum(), ref_h.sum(), rtol=1e-6)
+    scale = np.max(wk * L)
+    assert np.max(np.abs(g["kerma"] - ref_k)) <= 1e-5 * scale
+
+
+def test_source_parity(engine, dpack):
+    from chatcarlo_amd.source import SourceRecord
+    from oracle.source import source_from_uniforms
+    N, lib = _lib()
+    srcs = [
+        {"position": [0, -180, 0], "direction": [0, 1, 0], "field": {"shape": "rect", "size_cm": [35, 43],
+                                                                      "sid_cm": 180}, "kvp": 80},
+        {"position": [0, 0, 100], "direction": [0, 0, -1], "field": {"shape": "cone", "diameter_cm": 20,
+                                                                      "sid_cm": 100},
+         "spectrum": [{"energy_keV": 30, "weight": 1}, {"energy_keV": 60, "weight": 2}]},
+        {"position": [0, -60, 0], "direction": [0, 1, 0], "field": {"shape": "rect", "size_cm": [50, 4],
+                                                                     "sid_cm": 60}, "kvp": 120,
+         "rotation": {"isocenter": [0, 0, 0], "axis": "z", "scan_length_cm": 10}},
+        {"position": [0, 0, -15], "direction": [0, 0, 1], "field": {"shape": "parallel", "size_cm": [10, 10]},
+         "spectrum": [{"energy_keV": 60, "weight": 1}], "rotation": {"isocenter": [0, 0, 0], "axis": "x",
+                                                                     "n_angles": 8}},
+    ]
+    n = 4096
+    hist0 = 123456789
+    seed = 987654321987
+    for src in srcs:
+        rec = SourceRecord(src)
+        pos = np.zeros((n, 3), np.float32); dr = np.zeros((n, 3), np.float32); en = np.zeros(n, np.float32)
+        N.check(lib.cc_test_source(engine.ctx, C.byref(rec.struct), n, seed, hist0, N.ptr(pos, C.c_float),
+                                   N.ptr(dr, C.c_float), N.ptr(en, C.c_float)))
+        h = hist0 + np.arange(n, dtype=np.uint64)
+        key = np.tile(np.array([seed & 0xFFFFFFFF, seed >> 32], dtype=np.uint32), (n, 1))
+        blocks = []
+        for ctr in range(2):
+            c = np.stack([h & 0xFFFFFFFF, h >> 32, np.full(n, ctr), np.zeros(n)], axis=1).astype(np.uint32)
+            blocks.append(to_uniform(philox4x32_10(c, key)))
+        u = np.concatenate(blocks, axis=1)
+        cdf32 = rec.spec_cdf.astype(np.float32).astype(np.float64)
+        cdf32[-1] = 1.0
+        ro, rd, re = source_from_uniforms(src, u, rec.spec_e.astype(np.float32).astype(np.float64), cdf32)
+        np.testing.assert_allclose(pos, ro, rtol=0, atol=2e-4)
+        np.testing.assert_allclose(dr, rd, rtol=0, atol=2e-6)
+        assert np.mean(en == re.astype(np.float32)) > 0.9995

@@ -1,0 +1,179 @@
+"""GPU transport: the reference's physics checks and a per-voxel chi^2 against the oracle.
+
+Carries over chatcarlo's own tests (tests/test_transport.py:30-51 Beer-Lambert,
+tests/test_fluorescence.py:80-138 energy conservation and fluorescence,
+tests/test_tally.py:54-75 collision vs track-length) to the GPU path, and
+compares the GPU tallies with the FP64 oracle run on the same scene and data
+pack voxel by voxel (chi^2 with batch-moment sigmas of both runs).
+"""
+import numpy as np
+import pytest
+
+from oracle import core
+from oracle import transport as OT
+
+pytestmark = pytest.mark.gpu
+
+
+def _slab(material, thickness, size=100.0):
+    from chatcarlo_amd.geometry import Geometry
+    return Geometry([{"name": "slab", "shape": "box", "material": material, "center": [0.0, 0.0, 0.0],
+                      "size_cm": [thickness, size, size]}])
+
+
+def _pencil(n, thickness, energy):
+    pos = np.tile(np.array([-thickness / 2 - 10.0, 0.0, 0.0]), (n, 1))
+    dirv = np.tile(np.array([1.0, 0.0, 0.0]), (n, 1))
+    return pos, dirv, np.full(n, float(energy))
+
+
+@pytest.mark.parametrize("material,thickness,energy,seed", [("water", 10.0, 60.0, 1), ("lead", 0.2, 80.0, 2)])
+def test_beer_lambert(engine, material, thickness, energy, seed):
+    from chatcarlo_amd.materials import linear_mu
+    from chatcarlo_amd.transport import transport_photons
+    n = 200_000
+    g = _slab(material, thickness)
+    pos, dirv, e = _pencil(n, thickness, energy)
+    r = transport_photons(pos, dirv, e, g, np.random.default_rng(seed))
+    unc = np.sum(r.escaped & (r.n_scatter == 0)) / n
+    # air in front of and behind the slab (10 cm + world margin 50 cm)
+    mu = linear_mu(material, energy)[0]
+    mu_air = linear_mu("air", energy)[0]
+    expected = np.exp(-mu * thickness - mu_air * (10.0 + 50.0))
+    se = np.sqrt(expected * (1 - expected) / n)
+    assert abs(unc - expected) < 5 * se
+
+
+@pytest.mark.parametrize("fluor", [True, False])
+def test_energy_conservation(engine, fluor):
+    from chatcarlo_amd.transport import transport_photons
+    n = 20000
+    g = _slab("lead", 0.5, 50.0)
+    pos, dirv, e = _pencil(n, 0.5, 100.0)
+    e_in = float(e.sum())
+    r = transport_photons(pos, dirv, e, g, np.random.default_rng(42), fluorescence_enabled=fluor)
+    dep = sum(r.energy_deposited.values())
+    esc = float(np.sum(r.final_energy[r.escaped]))
+    # float32 photon energies: the balance closes to float32 rounding of each photon energy
+    assert abs(dep + esc - e_in) / e_in < 1e-6
+    assert np.all(r.absorbed | r.escaped)
+    if fluor:
+        assert r.n_fluorescence > 0
+    else:
+        assert r.n_fluorescence == 0
+
+
+def test_no_fluorescence_below_k_edge(engine):
+    from chatcarlo_amd.transport import transport_photons
+    n = 5000
+    g = _slab("lead", 0.5, 50.0)
+    pos, dirv, e = _pencil(n, 0.5, 87.0)
+    r = transport_photons(pos, dirv, e, g, np.random.default_rng(8))
+    assert r.n_fluorescence == 0
+
+
+def test_collision_vs_tracklength(engine):
+    from chatcarlo_amd.tally import VoxelGrid
+    from chatcarlo_amd.transport import transport_photons
+    n = 300_000
+    g = _slab("water", 15.0, 60.0)
+    grid = VoxelGrid.from_bbox(g.bbox_min, g.bbox_max, resolution_cm=3.0)
+    pos, dirv, e = _pencil(n, 15.0, 60.0)
+    pos[:, 0] = -15.0 / 2 - 5.0
+    r = transport_photons(pos, dirv, e, g, np.random.default_rng(7), grid=grid)
+    coll = sum(r.energy_deposited.values())
+    track = grid.kerma_keV.sum()
+    assert abs(track - coll) / coll < 0.05
+
+
+def _config1_scene():
+    return {
+        "source": {"position": [0.0, 0.0, -15.0], "direction": [0.0, 0.0, 1.0],
+                   "field": {"shape": "parallel", "size_cm": [1e-6, 1e-6]},
+                   "spectrum": [{"energy_keV": 60.0, "weight": 1.0}]},
+        "geometry": [{"name": "air", "shape": "box", "material": "air", "center": [0, 0, 0],
+                      "size_cm": [100, 100, 100]},
+                     {"name": "slab", "shape": "box", "material": "water", "center": [0, 0, 0],
+                      "size_cm": [30, 30, 10]}],
+        "physics": {"fluorescence": True},
+    }
+
+
+def test_chi2_against_oracle(engine, dpack):
+    """Per-voxel chi^2 between the GPU run and the FP64 oracle run (water slab, 60 keV)."""
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.scene import validate_scene
+    from chatcarlo_amd.source import SourceRecord
+    sc = validate_scene(_config1_scene())
+    assert sc.ok
+    geo = Geometry(sc.raw["geometry"])
+    origin, shape, vox = np.array([-15.0, -15.0, -5.0]), (30, 30, 10), 1.0
+    # GPU: 4e5 histories in 20 batches
+    engine.set_geometry(geo)
+    engine.bind_grid(origin, shape, vox, True)
+    rec = SourceRecord(sc.raw["source"])
+    n_gpu = 400_000
+    out = engine.run(rec, n_gpu, 12345, 0, 20_000,
+                     N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY)
+    g = engine.read_grid(shape)
+    # oracle: 1e5 histories in 10 batches, same pack, exact overlap tally
+    prims, world = geo.packed(dpack.mat_id)
+    og = core.OGeom.from_packed(prims, world)
+    grid = OT.Grid(origin, shape, vox)
+    n_or = 100_000
+    r = OT.run(sc.raw["source"], og, dpack, rec.spec_e, rec.spec_cdf, n_or, 7, batch_size=10_000, grid=grid)
+
+    def mean_var(T, Q, M, Nh):
+        var = core.batch_variance(T, Q, M, Nh)
+        return T / Nh, var / Nh
+
+    m1, v1 = mean_var(g["kerma"], g["kerma_sum2"], g["n_batches"], g["n_histories"])
+    m2, v2 = mean_var(grid.kerma, grid.kerma_q, grid.n_batches, grid.n_histories)
+    r1 = np.sqrt(v1) / np.where(m1 > 0, m1, np.inf)
+    r2 = np.sqrt(v2) / np.where(m2 > 0, m2, np.inf)
+    sel = (m1 > 0) & (m2 > 0) & (r1 < 0.1) & (r2 < 0.1)
+    assert sel.sum() > 500
+    z = (m1[sel] - m2[sel]) / np.sqrt(v1[sel] + v2[sel])
+    chi2 = float(np.sum(z ** 2))
+    dof = int(sel.sum())
+    # chi^2 / dof within 3 sigma of its expectation, and no voxel beyond 5 sigma
+    assert abs(chi2 / dof - 1.0) < 3.0 * np.sqrt(2.0 / dof) + 0.05, (chi2, dof)
+    assert np.max(np.abs(z)) < 5.5
+    # scalar physics summary within statistics
+    for key, k_or in (("absorbed", "n_absorbed"), ("escaped", "n_escaped")):
+        p1 = getattr(out, "n_" + key) / n_gpu
+        p2 = r[k_or] / n_or
+        se = np.sqrt(p1 * (1 - p1) / n_gpu + p2 * (1 - p2) / n_or)
+        assert abs(p1 - p2) < 5 * se, (key, p1, p2)
+
+
+def test_run_transport_api(engine):
+    from chatcarlo_amd.scene import validate_scene
+    from chatcarlo_amd.transport import run_transport
+    raw = {"source": {"kvp": 100, "position": [0, -50, 0], "direction": [0, 1, 0],
+                      "field": {"size_cm": [30, 30], "sid_cm": 100}},
+           "geometry": [{"name": "target", "shape": "box", "material": "water", "center": [0, 0, 0],
+                         "size_cm": [20, 20, 20]}]}
+    sc = validate_scene(raw)
+    assert sc.ok
+    r1 = run_transport(sc, n_histories=50_000, seed=1, dose_grid=True, grid_resolution_cm=5.0,
+                       batch_size=10_000)
+    h10 = r1.grid.h10_map_pSv()
+    assert np.all(np.isfinite(h10)) and np.all(h10 >= 0) and h10.sum() > 0
+    assert r1.n_batches == 5 and r1.grid.n_batches == 5
+    assert 0 < r1.fraction_absorbed < 1 and 0 < r1.fraction_escaped < 1
+    assert abs(r1.fraction_absorbed + r1.fraction_escaped - 1.0) < 1e-12
+    assert set(r1.energy_deposited_MeV) <= {"water", "air"}
+    rel = r1.grid.kerma_relative_error()
+    assert np.nanmin(rel) > 0
+    # totals do not depend on whether statistics are tracked
+    r2 = run_transport(sc, n_histories=50_000, seed=1, dose_grid=True, grid_resolution_cm=5.0,
+                       batch_size=10_000, track_uncertainty=False)
+    np.testing.assert_allclose(r2.grid.kerma_keV, r1.grid.kerma_keV, rtol=1e-12, atol=0)
+    assert r2.energy_deposited_MeV.keys() == r1.energy_deposited_MeV.keys()
+    # same seed, same results (up to FP64 atomic ordering)
+    r3 = run_transport(sc, n_histories=50_000, seed=1, dose_grid=True, grid_resolution_cm=5.0,
+                       batch_size=10_000)
+    np.testing.assert_allclose(r3.grid.kerma_keV, r1.grid.kerma_keV, rtol=1e-12, atol=1e-300)
+    assert r3.fraction_absorbed == r1.fraction_absorbed
