@@ -1,0 +1,163 @@
+"""Benchmark: analog photon transport histories/s on one MI355X (chest room, 80 kVp, 1e7 histories).
+
+One step = one full pass of the transport hot path over the workload's history
+count (source sampling, flights with the DDA kerma/H*(10) tally, interactions,
+batch moments), inputs resident on the device.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "histories/s; H*(10) FOM 1/(σ²t) at 2% rel err, learned vs analog"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes (roofline accounting)
+FLIGHT_STATE_B = 64        # particle state read + write per flight
+VOXEL_B = 32               # two FP64 read-modify-write tallies per voxel crossing
+EVENT_B = 128              # SoA state read + write per transport event
+SOURCE_B = 64              # source write per history
+
+
+def _cpu_sample(args):
+    """Time the oracle's reference-style NumPy transport on a bounded sample (one process)."""
+    raw, n, seed, min_seconds = args
+    from chatcarlo_amd.datapack import build_pack
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.spectrum import spectrum_table
+    from chatcarlo_amd.tally import VoxelGrid
+    from oracle import core
+    from oracle import transport as OT
+    pack = build_pack()
+    geo = Geometry(raw["geometry"])
+    prims, world = geo.packed(pack.mat_id)
+    og = core.OGeom.from_packed(prims, world)
+    vg = VoxelGrid.from_bbox(geo.bbox_min, geo.bbox_max, 5.0)
+    grid = OT.Grid(vg.origin_cm, vg.shape, vg.voxel_size_cm)
+    e, cdf = spectrum_table(raw["source"])
+    done = 0
+    t0 = time.perf_counter()
+    k = 0
+    while done < n and (time.perf_counter() - t0 < min_seconds or done == 0):
+        m = min(10_000, n - done)
+        OT.run(raw["source"], og, pack, e, cdf, m, seed + k, batch_size=m, grid=grid, exact_tally=False)
+        done += m
+        k += 1
+    return done, time.perf_counter() - t0
+
+
+def cpu_baseline(raw, all_cores):
+    import multiprocessing as mp
+    done, dt = _cpu_sample((raw, 100_000, 11, 12.0))
+    out = {"value": done / dt, "unit": "histories/s", "cores": 1, "kind": "port",
+           "sample": f"{done} histories of the same workload (oracle NumPy restatement of the "
+                     f"reference transport, stratified track-length tally), one process, {dt:.1f} s"}
+    if all_cores:
+        nproc = max(1, min(os.cpu_count() or 1, 16))
+        ctx = mp.get_context("spawn")
+        with ctx.Pool(nproc) as pool:
+            t0 = time.perf_counter()
+            res = pool.map(_cpu_sample, [(raw, done, 100 + i, 0.0) for i in range(nproc)])
+            dt2 = time.perf_counter() - t0
+        tot = sum(r[0] for r in res)
+        out["all_cores"] = {"value": tot / dt2, "cores": nproc,
+                            "sample": f"{tot} histories over {nproc} processes, {dt2:.1f} s incl. start-up"}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--histories", type=float, default=1e7)
+    ap.add_argument("--batch", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-all-cores", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(histories shard across GPUs only in the design, not in this build)", file=sys.stderr)
+        sys.exit(2)
+
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.scene import validate_scene
+    from chatcarlo_amd.source import SourceRecord
+    from chatcarlo_amd.tally import VoxelGrid
+    from chatcarlo_amd.transport import default_engine
+    from chatcarlo_amd.workloads import chest_room
+
+    raw, _ = chest_room()
+    sc = validate_scene(raw)
+    assert sc.ok, sc.errors
+    geo = Geometry(sc.raw["geometry"])
+    vg = VoxelGrid.from_bbox(geo.bbox_min, geo.bbox_max, 5.0, track_uncertainty=True)
+    eng = default_engine()
+    eng.set_geometry(geo)
+    rec = SourceRecord(sc.raw["source"])
+    n = int(args.histories)
+    flags = N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY | N.RUN_TIMING
+
+    def step(k):
+        eng.bind_grid(vg.origin_cm, vg.shape, vg.voxel_size_cm, True)
+        return eng.run(rec, n, 1000 + k, 0, args.batch, flags)
+
+    for k in range(args.warmup):
+        step(k)
+    outs = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        outs.append(step(args.warmup + k))
+    dt = time.perf_counter() - t0
+    hist = n * args.steps
+    value = hist / dt
+
+    flights = sum(o.n_flights for o in outs)
+    visits = sum(o.n_voxel_visits for o in outs)
+    events = sum(o.n_events for o in outs)
+    f_ms = sum(o.kernel_ms[1] for o in outs)
+    f_launch = sum(o.kernel_launches[1] for o in outs)
+    f_bytes = FLIGHT_STATE_B * flights + VOXEL_B * visits
+    f_avg_s = f_ms / 1000.0 / max(f_launch, 1)
+    achieved = f_bytes / max(f_launch, 1) / f_avg_s / 1e9
+    n_evt = events / hist
+    n_vox = visits / hist
+    b_hist = VOXEL_B * n_vox + EVENT_B * n_evt + SOURCE_B
+    o = outs[-1]
+    kernel_share = {name: round(sum(x.kernel_ms[i] for x in outs) / (dt * 1000.0), 4)
+                    for i, name in enumerate(["source", "flight", "photo", "compton", "rayleigh", "end_batch"])}
+    line = {
+        "metric": METRIC, "value": value, "unit": "histories/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1000.0 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (FP64 tallies)", "data": "synthetic",
+        "config": {"workload": "chest_room_80kVp_kramers_2.5mmAl_1keV_bins", "histories_per_step": n,
+                   "batch": args.batch, "grid": list(vg.shape), "voxel_cm": 5.0,
+                   "tallies": "kerma + H*(10) track-length, batch moments", "fluorescence": True},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_flight",
+                     "algorithmic_bytes_per_launch": f_bytes / max(f_launch, 1),
+                     "avg_launch_ms": f_avg_s * 1000.0, "launches": int(f_launch)},
+        "path": {"voxel_scores_per_s": value * n_vox * 2, "n_evt_per_history": n_evt,
+                 "n_vox_per_history": n_vox, "bytes_per_history": b_hist,
+                 "path_gbs": value * b_hist / 1e9, "path_frac": value * b_hist / 1e9 / HBM_PEAK_GBS,
+                 "kernel_time_share": kernel_share, "rounds_max": int(o.rounds_max)},
+        "physics": {"fraction_absorbed": o.n_absorbed / n, "fraction_escaped": o.n_escaped / n,
+                    "mean_interactions": o.scatter_sum / n, "n_fluorescence": int(o.n_fluorescence)},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(raw, args.cpu_all_cores)
+        line["cpu_baseline"]["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

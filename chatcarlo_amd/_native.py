@@ -59,7 +59,8 @@ class CcSource(C.Structure):
 class CcRunOut(C.Structure):
     _fields_ = [("n_histories", C.c_uint64), ("n_absorbed", C.c_uint64), ("n_escaped", C.c_uint64),
                 ("scatter_sum", C.c_uint64), ("n_fluorescence", C.c_uint64),
-                ("n_batches", C.c_uint64), ("n_events", C.c_uint64), ("error_bits", C.c_uint32),
+                ("n_batches", C.c_uint64), ("n_events", C.c_uint64), ("n_flights", C.c_uint64),
+                ("n_voxel_visits", C.c_uint64), ("error_bits", C.c_uint32),
                 ("rounds_max", C.c_uint32), ("edep_keV", C.c_double * CC_MAX_MAT),
                 ("edep_sum2", C.c_double * CC_MAX_MAT), ("kernel_ms", C.c_double * 8),
                 ("kernel_launches", C.c_uint64 * 8)]

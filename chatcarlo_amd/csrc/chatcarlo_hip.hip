@@ -55,7 +55,7 @@ enum { F_ABSORBED = 1, F_ESCAPED = 2, F_DEAD = 4 };
 // device counter slots (uint32)
 enum { C_ACT_A = 0, C_ACT_B = 1, C_QP = 2, C_QC = 3, C_QR = 4, C_NCNT = 8 };
 // device stats slots (u64)
-enum { S_ABS = 0, S_ESC = 1, S_SCAT = 2, S_FLUOR = 3, S_EVT = 4, S_NSTAT = 8 };
+enum { S_ABS = 0, S_ESC = 1, S_SCAT = 2, S_FLUOR = 3, S_EVT = 4, S_FLIGHT = 5, S_VOX = 6, S_NSTAT = 8 };
 
 struct PState {
   float *x, *y, *z, *dx, *dy, *dz, *e, *tau, *wt;
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     const bool valid = j < n;
     int type = -1, i = 0;
     bool escaped = false;
-    unsigned long long events = 0;
+    unsigned long long events = 0, visits = 0;
     if (valid) {
       i = act[j];
       float p[3] = {S.x[i], S.y[i], S.z[i]};
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
           const float taub = mu * tb;
           const bool inter = tau < taub;
           const float ds = inter ? tau / fmaxf(mu, 1e-30f) : tb;
-          if (L.grid_on) dda_score(L.G, p, d, ds, (double)w * (double)E * (double)muen, wh);
+          if (L.grid_on) visits += dda_score(L.G, p, d, ds, (double)w * (double)E * (double)muen, wh);
           p[0] += d[0] * ds; p[1] += d[1] * ds; p[2] += d[2] * ds;
           if (inter) {
             PhiloxRng rng(L.seed, L.hist0 + (uint64_t)i, 0u, S.ctr[i]);
@@ -340,7 +340,9 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
       }
     }
     wave_count(valid && escaped, L.stats + S_ESC);
+    wave_count(valid, L.stats + S_FLIGHT);
     wave_sum_u64(events, L.stats + S_EVT);
+    wave_sum_u64(visits, L.stats + S_VOX);
     int slot;
     slot = wave_append(type == 0, L.cnt + C_QP);
     if (slot >= 0) qp[slot] = i;
@@ -824,6 +826,8 @@ int read_batch(Ctx* c, uint32_t nb, cc_run_out* out, int* err_bits) {
   out->scatter_sum += st[S_SCAT];
   out->n_fluorescence += st[S_FLUOR];
   out->n_events += st[S_EVT];
+  out->n_flights += st[S_FLIGHT];
+  out->n_voxel_visits += st[S_VOX];
   for (int m = 0; m < CC_MAX_MAT; ++m) {
     out->edep_keV[m] += ed[m];
     if (nb > 0) out->edep_sum2[m] += ed[m] * ed[m] / (double)nb;

@@ -329,7 +329,8 @@ def run_transport(scene, n_histories: int = 100_000, seed: int | None = None,
     else:
         sem, rel = {}, {}
         n_batches = int(out.n_batches)
-    stats = dict(n_events=int(out.n_events), rounds_max=int(out.rounds_max),
+    stats = dict(n_events=int(out.n_events), n_flights=int(out.n_flights),
+                 n_voxel_visits=int(out.n_voxel_visits), rounds_max=int(out.rounds_max),
                  kernel_ms=list(out.kernel_ms), kernel_launches=list(out.kernel_launches))
     return TransportResult(
         n_histories=int(n_histories),

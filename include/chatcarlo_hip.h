@@ -110,7 +110,9 @@ enum {
 
 typedef struct {
   uint64_t n_histories, n_absorbed, n_escaped, scatter_sum, n_fluorescence, n_batches;
-  uint64_t n_events;            /* flights + interactions processed */
+  uint64_t n_events;            /* flight segments (boundary-to-boundary steps) */
+  uint64_t n_flights;           /* photons processed by the flight kernel */
+  uint64_t n_voxel_visits;      /* voxel crossings scored by the DDA */
   uint32_t error_bits, rounds_max;
   double edep_keV[CC_MAX_MAT];  /* collision estimator per material */
   double edep_sum2[CC_MAX_MAT]; /* sum over batches of S_b^2 / n_b */
