@@ -57,23 +57,29 @@ struct Tables {
 // float-float helpers
 // ---------------------------------------------------------------------------
 struct ff { float h, l; };
+// FMA contraction is switched off inside every helper: fusing a rounded
+// product into a later add would break the error-free transformations.
 __device__ __forceinline__ ff two_sum(float a, float b) {
+#pragma clang fp contract(off)
   float s = a + b;
   float bb = s - a;
   float e = (a - (s - bb)) + (b - bb);
   return {s, e};
 }
 __device__ __forceinline__ ff two_prod(float a, float b) {
+#pragma clang fp contract(off)
   float p = a * b;
   return {p, __builtin_fmaf(a, b, -p)};
 }
 __device__ __forceinline__ ff ff_add(ff a, ff b) {
+#pragma clang fp contract(off)
   ff s = two_sum(a.h, b.h);
   float l = s.l + a.l + b.l;
   return two_sum(s.h, l);
 }
 __device__ __forceinline__ ff ff_neg(ff a) { return {-a.h, -a.l}; }
 __device__ __forceinline__ ff ff_mul(ff a, ff b) {
+#pragma clang fp contract(off)
   ff p = two_prod(a.h, b.h);
   float l = p.l + (a.h * b.l + a.l * b.h);
   return two_sum(p.h, l);
@@ -81,6 +87,7 @@ __device__ __forceinline__ ff ff_mul(ff a, ff b) {
 __device__ __forceinline__ ff ff_sqr(ff a) { return ff_mul(a, a); }
 __device__ __forceinline__ float ff_f(ff a) { return a.h + a.l; }
 __device__ __forceinline__ ff ff_sqrt(ff a) {
+#pragma clang fp contract(off)
   if (a.h <= 0.f) return {0.f, 0.f};
   float s = sqrtf(a.h);
   // one Newton step: s' = s + (a - s^2) / (2 s)
@@ -158,9 +165,10 @@ __device__ __forceinline__ void slab(const float* lo, const float* hi, const flo
   hit = !miss && (ten <= tex);
 }
 
-// Stable quadratic a t^2 + 2 b t + c = 0 with b, c, disc in float-float.
-__device__ __forceinline__ bool quad_roots(float a, ff b, ff c, float& t1, float& t2) {
-  ff disc = ff_add(ff_mul(b, b), ff_neg(ff_mul({a, 0.f}, c)));
+// Stable quadratic a t^2 + 2 b t + c = 0 with a, b, c, disc in float-float.
+__device__ __forceinline__ bool quad_roots(ff aa, ff b, ff c, float& t1, float& t2) {
+  ff disc = ff_add(ff_mul(b, b), ff_neg(ff_mul(aa, c)));
+  const float a = ff_f(aa);
   if (disc.h < 0.f || (disc.h == 0.f && disc.l < 0.f)) return false;
   ff sq = ff_sqrt(disc);
   float sb = (b.h >= 0.f) ? 1.f : -1.f;
@@ -188,10 +196,8 @@ __device__ __forceinline__ bool isect_sphere(const Prim& P, const float o[3], co
     cc = ff_add(cc, ff_sqr(oc));
   }
   cc = ff_add(cc, ff_neg(two_prod(P.p[3], P.p[3])));
-  // |d| = 1 up to rounding: a = d.d
-  float a = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-  (void)a;
-  return quad_roots(1.0f, b, cc, t1, t2);
+  // unit direction: a = 1 (chatcarlo/geometry.py:45-54)
+  return quad_roots({1.0f, 0.0f}, b, cc, t1, t2);
 }
 
 __device__ __forceinline__ bool isect_cylinder(const Prim& P, const float o[3], const float d[3],
@@ -202,7 +208,8 @@ __device__ __forceinline__ bool isect_cylinder(const Prim& P, const float o[3], 
   ff ox = two_sum(o[i0], -P.p[i0]);
   ff oy = two_sum(o[i1], -P.p[i1]);
   float dx = d[i0], dy = d[i1];
-  float a = dx * dx + dy * dy;
+  ff aa = ff_add(two_prod(dx, dx), two_prod(dy, dy));
+  float a = ff_f(aa);
   ff b = ff_add(ff_mul(ox, {dx, 0.f}), ff_mul(oy, {dy, 0.f}));
   ff cc = ff_add(ff_add(ff_sqr(ox), ff_sqr(oy)), ff_neg(two_prod(P.p[3], P.p[3])));
   float ten_r, tex_r;
@@ -216,7 +223,7 @@ __device__ __forceinline__ bool isect_cylinder(const Prim& P, const float o[3], 
       ten_r = tex_r = 0.f;
     }
   } else {
-    if (!quad_roots(a, b, cc, ten_r, tex_r)) miss_r = true;
+    if (!quad_roots(aa, b, cc, ten_r, tex_r)) miss_r = true;
   }
   // caps at c_ax +- half_h, plane offsets formed exactly in float-float
   float dz = d[ax];
@@ -392,7 +399,7 @@ __device__ __forceinline__ float interp_guess(const float* xp, const float* fp, 
 
 template <class Rng>
 __device__ __forceinline__ float compton_sample(const Tables& T, int m, float E, const EPos& p, Rng& rng,
-                                                int kmax, float& cos_t, int& el, int& trials) {
+                                                int kmax, float& cos_t, float& sin_t, int& el, int& trials) {
   el = select_elem(T, m, P_COMPT, p, rng.next());
   const float a = E / kMEC2;
   const float emin = 1.0f / (1.0f + 2.0f * a);
@@ -402,7 +409,9 @@ __device__ __forceinline__ float compton_sample(const Tables& T, int m, float E,
   for (int t = 0; t < kmax; ++t) {
     float x1 = rng.next(), x2 = rng.next(), x3 = rng.next();
     float eps = emin + x1 * (1.0f - emin);
-    float omc = (1.0f / eps - 1.0f) / a;
+    // 1 - cos = (1/eps - 1)/a written without the cancellation of 1/eps - 1:
+    // 1 - eps = (1 - emin)(1 - x1) and 1 - emin = 2a/(1 + 2a)
+    float omc = 2.0f * (1.0f - x1) / ((1.0f + 2.0f * a) * eps);
     float cs = 1.0f - omc;
     float sin2 = omc * (2.0f - omc);
     float g = 1.0f / eps + eps - sin2;
@@ -414,19 +423,22 @@ __device__ __forceinline__ float compton_sample(const Tables& T, int m, float E,
     acc = acc && (x3 <= soz);
     if (acc) {
       cos_t = cs;
+      // sin^2 = (1 - cos)(1 + cos) with 1 + cos = 2 x1 / eps
+      sin_t = 2.0f * sqrtf(x1 * (1.0f - x1) / (1.0f + 2.0f * a)) / eps;
       trials = t + 1;
       return eps;
     }
   }
   trials = 0;
   cos_t = 1.0f;
+  sin_t = 0.0f;
   return 1.0f;
 }
 
 // returns cos theta; trials = 0 when the cap is hit; qerr set if x_max beyond the table
 template <class Rng>
 __device__ __forceinline__ float rayleigh_sample(const Tables& T, int m, float E, const EPos& p, Rng& rng,
-                                                 int kmax, int& el, int& trials, bool& qerr) {
+                                                 int kmax, float& sin_t, int& el, int& trials, bool& qerr) {
   el = select_elem(T, m, P_RAYL, p, rng.next());
   const int nq = T.n_q;
   const float* xg = T.ray_x;
@@ -451,11 +463,14 @@ __device__ __forceinline__ float rayleigh_sample(const Tables& T, int m, float E
     x = fminf(x, x_max);
     float c = fminf(fmaxf(1.0f - 2.0f * x / x_max, -1.0f), 1.0f);
     if (x2 <= (1.0f + c * c) * 0.5f) {
+      // sin^2 = (1 - c)(1 + c) = 4 x (x_max - x) / x_max^2
+      sin_t = 2.0f * sqrtf(fmaxf(x * (x_max - x), 0.0f)) / x_max;
       trials = t + 1;
       return c;
     }
   }
   trials = 0;
+  sin_t = 0.0f;
   return 1.0f;
 }
 
@@ -482,9 +497,11 @@ __device__ __forceinline__ bool fluor_sample(const Tables& T, int m, float E, co
   return false;
 }
 
-// New direction from cos theta and phi = 2 pi u (chatcarlo/physics.py:354-367)
-__device__ __forceinline__ void rotate_dir(float d[3], float cos_t, float u_phi) {
-  float st = sqrtf(fmaxf(1.0f - cos_t * cos_t, 0.0f));
+// New direction from (cos theta, sin theta) and phi = 2 pi u (chatcarlo/physics.py:354-367).
+// Each sampler supplies sin theta in a form free of the cancellation of
+// sqrt(1 - cos^2) near theta = 0 and pi.
+__device__ __forceinline__ void rotate_dir(float d[3], float cos_t, float sin_t, float u_phi) {
+  float st = sin_t;
   float sp, cp;
   sincosf(kTwoPi * u_phi, &sp, &cp);
   float up0, up1, up2;
@@ -506,10 +523,10 @@ __device__ __forceinline__ void rotate_dir(float d[3], float cos_t, float u_phi)
   d[0] = n0 * inv2; d[1] = n1 * inv2; d[2] = n2 * inv2;
 }
 
-// chatcarlo/physics.py:215-220
+// chatcarlo/physics.py:215-220: cos = U(-1, 1); sin = 2 sqrt(u (1 - u)) = sqrt(1 - cos^2)
 __device__ __forceinline__ void isotropic_dir(float d[3], float u1, float u2) {
   float c = -1.0f + 2.0f * u1;
-  float s = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+  float s = 2.0f * sqrtf(fmaxf(u1 * (1.0f - u1), 0.0f));
   float sp, cp;
   sincosf(kTwoPi * u2, &sp, &cp);
   d[0] = s * cp; d[1] = s * sp; d[2] = c;

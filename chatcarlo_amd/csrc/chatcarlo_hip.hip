@@ -66,7 +66,7 @@ struct PState {
 struct Src {
   int field, n_spec, n_angles, rotation;
   float pos[3], dir[3], u[3], v[3];
-  float sid, width, height, cos_half;
+  float sid, width, height, cos_half, omch;   // omch = 1 - cos_half
   float iso[3], axis[3];
   float scan;
   const float* spec_e;
@@ -183,8 +183,10 @@ __device__ __forceinline__ void sample_source(const Src& s, Rng& rng, float p[3]
     }
   }
   if (s.field == CC_FIELD_CONE) {
-    float c = s.cos_half + (1.f - s.cos_half) * u2;
-    float sn = sqrtf(fmaxf(1.f - c * c, 0.f));
+    // c = cos_half + (1 - cos_half) u; 1 - c = (1 - cos_half)(1 - u) keeps sin accurate near the axis
+    float omc = s.omch * (1.f - u2);
+    float c = 1.f - omc;
+    float sn = sqrtf(fmaxf(omc * (2.f - omc), 0.f));
     float sp, cp;
     sincosf(kTwoPi * u3, &sp, &cp);
     for (int k = 0; k < 3; ++k) d[k] = c * dd[k] + sn * cp * uu[k] + sn * sp * vv[k];
@@ -418,14 +420,14 @@ __global__ __launch_bounds__(kBlock) void k_compton(Launch L, const int* q, cons
       const float w = S.wt[i];
       PhiloxRng rng(L.seed, L.hist0 + (uint64_t)i, 0u, S.ctr[i]);
       const EPos ep = energy_pos(T, E);
-      float cs;
+      float cs, sn;
       int el, trials;
-      const float eps = compton_sample(T, m, E, ep, rng, kTrialCap, cs, el, trials);
+      const float eps = compton_sample(T, m, E, ep, rng, kTrialCap, cs, sn, el, trials);
       if (trials == 0) raise_err(L.err, CC_DEV_ITER_CAP);
       const float enew = E * eps;
       dep = (double)w * ((double)E - (double)enew);
       float d[3] = {S.dx[i], S.dy[i], S.dz[i]};
-      rotate_dir(d, cs, rng.next());
+      rotate_dir(d, cs, sn, rng.next());
       S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
       S.e[i] = enew;
       S.tau[i] = -logf(rng.next());
@@ -456,11 +458,12 @@ __global__ __launch_bounds__(kBlock) void k_rayleigh(Launch L, const int* q, con
       const EPos ep = energy_pos(T, E);
       int el, trials;
       bool qerr;
-      const float cs = rayleigh_sample(T, m, E, ep, rng, kTrialCap, el, trials, qerr);
+      float sn;
+      const float cs = rayleigh_sample(T, m, E, ep, rng, kTrialCap, sn, el, trials, qerr);
       if (trials == 0) raise_err(L.err, CC_DEV_ITER_CAP);
       if (qerr) raise_err(L.err, CC_DEV_Q_RANGE);
       float d[3] = {S.dx[i], S.dy[i], S.dz[i]};
-      rotate_dir(d, cs, rng.next());
+      rotate_dir(d, cs, sn, rng.next());
       S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
       S.tau[i] = -logf(rng.next());
       S.nsc[i] += 1;
@@ -557,9 +560,9 @@ __global__ void k_test_compton(Tables T, int n, const int* mat, const float* e, 
   if (i >= n) return;
   BufRng rng(u + (size_t)i * ku, ku);
   const EPos ep = energy_pos(T, e[i]);
-  float c;
+  float c, sn;
   int z, t;
-  eps[i] = compton_sample(T, mat[i], e[i], ep, rng, kmax, c, z, t);
+  eps[i] = compton_sample(T, mat[i], e[i], ep, rng, kmax, c, sn, z, t);
   cs[i] = c; el[i] = z; tr[i] = t;
 }
 
@@ -571,7 +574,8 @@ __global__ void k_test_rayleigh(Tables T, int n, const int* mat, const float* e,
   const EPos ep = energy_pos(T, e[i]);
   int z, t;
   bool qerr;
-  cs[i] = rayleigh_sample(T, mat[i], e[i], ep, rng, kmax, z, t, qerr);
+  float sn;
+  cs[i] = rayleigh_sample(T, mat[i], e[i], ep, rng, kmax, sn, z, t, qerr);
   if (qerr) raise_err(err, CC_DEV_Q_RANGE);
   el[i] = z; tr[i] = t;
 }
@@ -592,7 +596,8 @@ __global__ void k_test_rotate(int n, const float* d, const float* cs, const floa
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float dd[3] = {d[3 * i], d[3 * i + 1], d[3 * i + 2]};
-  rotate_dir(dd, cs[i], up[i]);
+  const float c = cs[i];
+  rotate_dir(dd, c, sqrtf(fmaxf((1.0f - c) * (1.0f + c), 0.0f)), up[i]);
   out[3 * i] = dd[0]; out[3 * i + 1] = dd[1]; out[3 * i + 2] = dd[2];
 }
 
@@ -897,7 +902,8 @@ int make_src_impl(Ctx* c, const cc_source* s, Src* d) {
     d->iso[k] = (float)s->iso[k]; d->axis[k] = (float)s->axis[k];
   }
   d->sid = (float)s->sid; d->width = (float)s->width; d->height = (float)s->height;
-  d->cos_half = (float)s->cos_half; d->scan = (float)s->scan_length;
+  d->cos_half = (float)s->cos_half; d->omch = (float)(1.0 - s->cos_half);
+  d->scan = (float)s->scan_length;
   d->spec_e = c->spec;
   d->spec_cdf = c->spec + s->n_spec;
   return CC_OK;

@@ -15,11 +15,14 @@ from .spectrum import kramers_spectrum_bins
 
 
 def water_slab_60kev():
-    """60 keV pencil beam along +z into a 30x30x10 cm water slab in a 100 cm air cube;
-    fluorescence on; 1 cm kerma grid over the slab."""
+    """60 keV pencil beam (10 um square parallel beam on the axis) along +z into a 30x30x10 cm
+    water slab in a 100 cm air cube; fluorescence on; 1 cm kerma grid over the slab.
+
+    The beam axis runs along the corner line of four voxels; its 10 um width is resolvable in
+    float32 at the grid's 15 cm offset, so GPU and FP64 oracle split photons alike."""
     raw = {
         "source": {"position": [0.0, 0.0, -15.0], "direction": [0.0, 0.0, 1.0],
-                   "field": {"shape": "parallel", "size_cm": [1e-6, 1e-6]},
+                   "field": {"shape": "parallel", "size_cm": [1e-3, 1e-3]},
                    "spectrum": [{"energy_keV": 60.0, "weight": 1.0}]},
         "geometry": [
             {"name": "room_air", "shape": "box", "material": "air", "center": [0, 0, 0], "size_cm": [100, 100, 100]},

@@ -87,16 +87,8 @@ def test_collision_vs_tracklength(engine):
 
 
 def _config1_scene():
-    return {
-        "source": {"position": [0.0, 0.0, -15.0], "direction": [0.0, 0.0, 1.0],
-                   "field": {"shape": "parallel", "size_cm": [1e-6, 1e-6]},
-                   "spectrum": [{"energy_keV": 60.0, "weight": 1.0}]},
-        "geometry": [{"name": "air", "shape": "box", "material": "air", "center": [0, 0, 0],
-                      "size_cm": [100, 100, 100]},
-                     {"name": "slab", "shape": "box", "material": "water", "center": [0, 0, 0],
-                      "size_cm": [30, 30, 10]}],
-        "physics": {"fluorescence": True},
-    }
+    from chatcarlo_amd.workloads import water_slab_60kev
+    return water_slab_60kev()[0]
 
 
 def test_chi2_against_oracle(engine, dpack):
@@ -109,20 +101,21 @@ def test_chi2_against_oracle(engine, dpack):
     assert sc.ok
     geo = Geometry(sc.raw["geometry"])
     origin, shape, vox = np.array([-15.0, -15.0, -5.0]), (30, 30, 10), 1.0
-    # GPU: 4e5 histories in 20 batches
+    # GPU: 4e5 histories in 100 batches (many batches keep the variance estimates sharp:
+    # with M batches E[1/var_hat] = (M-1)/(M-3) / var inflates chi^2 by that factor)
     engine.set_geometry(geo)
     engine.bind_grid(origin, shape, vox, True)
     rec = SourceRecord(sc.raw["source"])
     n_gpu = 400_000
-    out = engine.run(rec, n_gpu, 12345, 0, 20_000,
+    out = engine.run(rec, n_gpu, 12345, 0, 4_000,
                      N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY)
     g = engine.read_grid(shape)
-    # oracle: 1e5 histories in 10 batches, same pack, exact overlap tally
+    # oracle: 1e5 histories in 50 batches, same pack, exact overlap tally
     prims, world = geo.packed(dpack.mat_id)
     og = core.OGeom.from_packed(prims, world)
     grid = OT.Grid(origin, shape, vox)
     n_or = 100_000
-    r = OT.run(sc.raw["source"], og, dpack, rec.spec_e, rec.spec_cdf, n_or, 7, batch_size=10_000, grid=grid)
+    r = OT.run(sc.raw["source"], og, dpack, rec.spec_e, rec.spec_cdf, n_or, 7, batch_size=2_000, grid=grid)
 
     def mean_var(T, Q, M, Nh):
         var = core.batch_variance(T, Q, M, Nh)
@@ -137,7 +130,8 @@ def test_chi2_against_oracle(engine, dpack):
     z = (m1[sel] - m2[sel]) / np.sqrt(v1[sel] + v2[sel])
     chi2 = float(np.sum(z ** 2))
     dof = int(sel.sum())
-    # chi^2 / dof within 3 sigma of its expectation, and no voxel beyond 5 sigma
+    # chi^2 / dof within 3 sigma of its expectation (the variance-estimate inflation of
+    # ~(M-1)/(M-3) for M = 50 batches is inside the 0.05 allowance), no voxel beyond 5.5 sigma
     assert abs(chi2 / dof - 1.0) < 3.0 * np.sqrt(2.0 / dof) + 0.05, (chi2, dof)
     assert np.max(np.abs(z)) < 5.5
     # scalar physics summary within statistics
