@@ -27,6 +27,18 @@ EVENT_B = 128              # SoA state read + write per transport event
 SOURCE_B = 64              # source write per history
 
 
+def measured_traffic(kernel="k_flight"):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_chest_room_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"{kernel}_traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def _cpu_sample(args):
     """Time the oracle's reference-style NumPy transport on a bounded sample (one process)."""
     raw, n, seed, min_seconds = args
@@ -82,7 +94,19 @@ def main():
     ap.add_argument("--batch", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-all-cores", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        nbytes = args.calibrate_copy << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(histories shard across GPUs only in the design, not in this build)", file=sys.stderr)
@@ -143,7 +167,8 @@ def main():
                    "batch": args.batch, "grid": list(vg.shape), "voxel_cm": 5.0,
                    "tallies": "kerma + H*(10) track-length, batch moments", "fluorescence": True},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_flight",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_flight"),
+                     "kernel": "k_flight",
                      "algorithmic_bytes_per_launch": f_bytes / max(f_launch, 1),
                      "avg_launch_ms": f_avg_s * 1000.0, "launches": int(f_launch)},
         "path": {"voxel_scores_per_s": value * n_vox * 2, "n_evt_per_history": n_evt,

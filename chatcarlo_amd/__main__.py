@@ -1,0 +1,104 @@
+"""`python -m chatcarlo_amd run scene.yaml ...`: the reference's `chatcarlo run` on the GPU.
+
+Mirrors chatcarlo/__main__.py:131-253 and the `run` parser at :327-349 (flags
+-n/--n-histories, --seed, --dose-grid, --resolution, --dose-out,
+--no-uncertainty, --workers) and writes the same npz keys.  `validate` is kept
+for the scene front end; the visualisation sub-commands are not part of this
+package.
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+
+import numpy as np
+
+
+def dose_map_Gy(grid, geometry):
+    """Kerma -> Gy with the material density at each voxel centre (chatcarlo/diagnostics.py:16-26)."""
+    from .materials import density
+    centers = grid.voxel_centers()
+    mats = geometry.material_at(centers)
+    dens = {m: density(m) for m in set(mats.tolist())}
+    rho = np.array([dens[m] for m in mats]).reshape(grid.shape)
+    return grid.dose_map_Gy(rho)
+
+
+def cmd_validate(args) -> int:
+    from .scene import load_scene
+    scene = load_scene(args.scene)
+    for w in scene.warnings:
+        print(f"[warning] {w}")
+    for e in scene.errors:
+        print(f"[error] {e}", file=sys.stderr)
+    if scene.ok:
+        print("OK")
+    return 0 if scene.ok else 1
+
+
+def cmd_run(args) -> int:
+    from .geometry import Geometry
+    from .scene import load_scene
+    from .transport import run_transport
+    scene = load_scene(args.scene)
+    if not scene.ok:
+        for e in scene.errors:
+            print(f"[error] {e}", file=sys.stderr)
+        return 1
+    for w in scene.warnings:
+        print(f"[warning] {w}")
+    result = run_transport(scene, n_histories=int(args.n_histories), seed=args.seed,
+                           dose_grid=args.dose_grid, grid_resolution_cm=args.resolution,
+                           n_workers=args.workers, track_uncertainty=not args.no_uncertainty,
+                           batch_size=args.batch_size)
+    print(f"histories: {result.n_histories:,}")
+    print(f"fraction absorbed (photoelectric): {result.fraction_absorbed:.4f}")
+    print(f"fraction escaped: {result.fraction_escaped:.4f}")
+    print(f"mean interactions per photon: {result.mean_scatter_events:.4f}")
+    print(f"fluorescence emissions: {result.n_fluorescence:,}")
+    print("energy deposited per material [MeV, all histories]:")
+    for name, e in sorted(result.energy_deposited_MeV.items(), key=lambda kv: -kv[1]):
+        rel = result.energy_deposited_rel_err.get(name)
+        print(f"  {name}: {e:.6g}" + (f"  (R = {rel:.3g})" if rel is not None and np.isfinite(rel) else ""))
+    if args.dose_grid:
+        geometry = Geometry(scene.raw["geometry"])
+        grid = result.grid
+        dose = dose_map_Gy(grid, geometry)
+        h10 = grid.h10_map_pSv()
+        n = result.n_histories
+        dose_ph, h10_ph = dose / n, h10 / n
+        print(f"dose grid: shape={grid.shape}, resolution={grid.voxel_size_cm} cm")
+        print(f"  max absorbed dose [Gy/history]: {dose_ph.max():.6g}")
+        print(f"  total kerma: {grid.total_kerma_MeV():.6g} MeV ({grid.total_kerma_MeV() / n * 1000:.6g} keV/history)")
+        print(f"  max H*(10) [pSv/history]: {h10_ph.max():.6g}")
+        if args.dose_out:
+            np.savez(args.dose_out, dose_per_history_Gy=dose_ph, h10_per_history_pSv=h10_ph,
+                     kerma_keV=grid.kerma_keV, h10_track_pSv_cm3=grid.h10_track_pSv_cm3,
+                     origin_cm=grid.origin_cm, voxel_size_cm=grid.voxel_size_cm, shape=np.array(grid.shape))
+            print(f"wrote dose grid: {args.dose_out}")
+    return 0
+
+
+def main(argv=None) -> int:
+    p = argparse.ArgumentParser(prog="chatcarlo_amd")
+    sub = p.add_subparsers(dest="cmd", required=True)
+    pv = sub.add_parser("validate", help="validate a scene.yaml")
+    pv.add_argument("scene")
+    pv.set_defaults(func=cmd_validate)
+    pr = sub.add_parser("run", help="run photon transport on the GPU")
+    pr.add_argument("scene")
+    pr.add_argument("-n", "--n-histories", type=float, default=1e5)
+    pr.add_argument("--seed", type=int, default=None)
+    pr.add_argument("--dose-grid", action="store_true")
+    pr.add_argument("--resolution", type=float, default=5.0)
+    pr.add_argument("--dose-out")
+    pr.add_argument("--no-uncertainty", action="store_true")
+    pr.add_argument("--workers", type=int, default=1, help="accepted for compatibility; one GPU is used")
+    pr.add_argument("--batch-size", type=int, default=200_000)
+    pr.set_defaults(func=cmd_run)
+    args = p.parse_args(argv)
+    return args.func(args)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
