@@ -548,57 +548,110 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Sum of a double over all 64 lanes of the wave; every lane must be active.
+__device__ __forceinline__ double wave_sum_f64(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Adds (ak, ah) to voxel v for every lane with has = true.  Lanes that share a
+// voxel are combined first (up to kAggRounds groups, led by the lowest pending
+// lane), so a wave whose photons walk the same voxels -- the primary beam near
+// the focal spot -- issues one pair of atomics per voxel instead of one per
+// lane.  Every lane of the wave must call this (converged control flow).
+constexpr int kAggRounds = 4;
+__device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t v, double ak, double ah) {
+  const int lane = __lane_id();
+  unsigned long long pend = __ballot(has);
+  for (int r = 0; r < kAggRounds && pend; ++r) {
+    const int leader = __ffsll((long long)pend) - 1;
+    const uint32_t vl = __shfl(v, leader);
+    const bool mine = has && v == vl;
+    const unsigned long long m = __ballot(mine);
+    if (m == (1ull << leader)) break;          // no partner: the rest go direct
+    const double sk = wave_sum_f64(mine ? ak : 0.0);
+    const double sh = wave_sum_f64(mine ? ah : 0.0);
+    if (lane == leader) {
+      atomic_add_f64(G.dk + vl, sk);
+      atomic_add_f64(G.dh + vl, sh);
+    }
+    has = has && !mine;
+    pend &= ~m;
+  }
+  if (has) {
+    atomic_add_f64(G.dk + v, ak);
+    atomic_add_f64(G.dh + v, ah);
+  }
+}
+
 // Scores wk*len into dk and wh*len into dh for every voxel the segment
-// o + t d, t in [0, L], overlaps.  Returns the number of voxel visits.
+// o + t d, t in [0, L], overlaps, for lanes with seg = true.  Exact 3-D DDA
+// (Amanatides & Woo 1987); all lanes of the wave must call it.  Returns the
+// number of voxel visits of the calling lane.
 __device__ __forceinline__ int dda_score(const GridDev& G, const float o[3], const float d[3], float L,
-                                         double wk, double wh) {
+                                         double wk, double wh, bool seg) {
   const int n[3] = {G.nx, G.ny, G.nz};
   float t0 = 0.0f, t1 = L;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float lo = G.org[k], hi = G.org[k] + (float)n[k] * G.h;
-    if (fabsf(d[k]) < 1e-30f) {
-      if (o[k] < lo || o[k] > hi) return 0;
-    } else {
-      float ta = (lo - o[k]) / d[k], tb = (hi - o[k]) / d[k];
-      t0 = fmaxf(t0, fminf(ta, tb));
-      t1 = fminf(t1, fmaxf(ta, tb));
-    }
-  }
-  if (!(t1 > t0)) return 0;
-  int idx[3], step[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    // voxel holding the clipped start point; a start that rounds onto the
-    // wrong side of a plane only moves a zero-length piece
-    int ik = (int)floorf((o[k] + d[k] * t0 - G.org[k]) * G.inv_h);
-    idx[k] = min(max(ik, 0), n[k] - 1);
-    step[k] = (d[k] > 0.f) ? 1 : ((d[k] < 0.f) ? -1 : 0);
-  }
-  float tc = t0;
-  int visits = 0;
-  const int cap = G.nx + G.ny + G.nz + 4;
-  for (int it = 0; it < cap; ++it) {
-    // parametric distance to the next plane on each axis, recomputed from the index
-    float tm[3];
+  if (seg) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      float plane = G.org[k] + (float)(idx[k] + (step[k] > 0 ? 1 : 0)) * G.h;
-      tm[k] = step[k] != 0 ? (plane - o[k]) / d[k] : INFINITY;
+      float lo = G.org[k], hi = G.org[k] + (float)n[k] * G.h;
+      if (fabsf(d[k]) < 1e-30f) {
+        if (o[k] < lo || o[k] > hi) seg = false;
+      } else {
+        float ta = (lo - o[k]) / d[k], tb = (hi - o[k]) / d[k];
+        t0 = fmaxf(t0, fminf(ta, tb));
+        t1 = fminf(t1, fmaxf(ta, tb));
+      }
     }
-    int a = (tm[0] < tm[1]) ? ((tm[0] < tm[2]) ? 0 : 2) : ((tm[1] < tm[2]) ? 1 : 2);
-    float tn = fminf(fmaxf(tm[a], tc), t1);
-    double len = (double)tn - (double)tc;
-    if (len > 0.0) {
-      size_t v = ((size_t)idx[0] * G.ny + idx[1]) * G.nz + idx[2];
-      atomic_add_f64(G.dk + v, wk * len);
-      atomic_add_f64(G.dh + v, wh * len);
-      ++visits;
+    if (!(t1 > t0)) seg = false;
+  }
+  int idx[3] = {0, 0, 0}, step[3] = {0, 0, 0};
+  if (seg) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      // voxel holding the clipped start point; a start that rounds onto the
+      // wrong side of a plane only moves a zero-length piece
+      int ik = (int)floorf((o[k] + d[k] * t0 - G.org[k]) * G.inv_h);
+      idx[k] = min(max(ik, 0), n[k] - 1);
+      step[k] = (d[k] > 0.f) ? 1 : ((d[k] < 0.f) ? -1 : 0);
     }
-    tc = tn;
-    if (tn >= t1) break;
-    idx[a] += step[a];
-    if (idx[a] < 0 || idx[a] >= n[a]) break;
+  }
+  float tc = t0;
+  int visits = 0, it = 0;
+  const int cap = G.nx + G.ny + G.nz + 4;
+  bool act = seg;
+  while (__ballot(act)) {
+    bool has = false;
+    uint32_t v = 0xffffffffu;
+    double ak = 0.0, ah = 0.0;
+    if (act) {
+      // parametric distance to the next plane on each axis, recomputed from the index
+      float tm[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        float plane = G.org[k] + (float)(idx[k] + (step[k] > 0 ? 1 : 0)) * G.h;
+        tm[k] = step[k] != 0 ? (plane - o[k]) / d[k] : INFINITY;
+      }
+      int a = (tm[0] < tm[1]) ? ((tm[0] < tm[2]) ? 0 : 2) : ((tm[1] < tm[2]) ? 1 : 2);
+      float tn = fminf(fmaxf(tm[a], tc), t1);
+      double len = (double)tn - (double)tc;
+      if (len > 0.0) {
+        v = (uint32_t)(((size_t)idx[0] * G.ny + idx[1]) * G.nz + idx[2]);
+        ak = wk * len;
+        ah = wh * len;
+        has = true;
+        ++visits;
+      }
+      tc = tn;
+      if (tn >= t1 || ++it >= cap) {
+        act = false;
+      } else {
+        idx[a] += step[a];
+        if (idx[a] < 0 || idx[a] >= n[a]) act = false;
+      }
+    }
+    wave_tally(G, has, v, ak, ah);
   }
   return visits;
 }

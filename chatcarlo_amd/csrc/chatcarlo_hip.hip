@@ -266,80 +266,97 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
   const uint32_t n = *cnt_in;
   const Tables& T = L.T;
   PState S = L.S;
+  // Every lane of a wave stays in the segment loop until the whole wave is
+  // done (flags instead of divergent exits), so the DDA tally can combine
+  // lanes that share a voxel.
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t j = base + threadIdx.x;
     const bool valid = j < n;
     int type = -1, i = 0;
-    bool escaped = false;
+    bool escaped = false, alive = false;
     unsigned long long events = 0, visits = 0;
+    float p[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 1.f};
+    float E = 1.0f, tau = 0.f, w = 0.f, mu = 0.f, muen = 0.f;
+    double wh = 0.0;
+    int mc = -1, it = 0;
+    EPos ep{0, 0.f, 0.f};
     if (valid) {
       i = act[j];
-      float p[3] = {S.x[i], S.y[i], S.z[i]};
-      float d[3] = {S.dx[i], S.dy[i], S.dz[i]};
-      const float E = S.e[i];
-      float tau = S.tau[i];
-      const float w = S.wt[i];
+      p[0] = S.x[i]; p[1] = S.y[i]; p[2] = S.z[i];
+      d[0] = S.dx[i]; d[1] = S.dy[i]; d[2] = S.dz[i];
+      E = S.e[i];
+      tau = S.tau[i];
+      w = S.wt[i];
       if (!energy_ok(E)) {
         raise_err(L.err, CC_DEV_ENERGY_RANGE);
         S.flags[i] |= F_DEAD;
       } else {
-        const EPos ep = energy_pos(T, E);
-        const double wh = L.grid_on ? (double)w * (double)h10_coeff(T.h10, T.n_h10, E) : 0.0;
-        int mc = -1;
-        float mu = 0.f, muen = 0.f;
-        int it = 0;
-        for (; it < kMaxCrossings; ++it) {
-          ++events;
-          const float am = fmaxf(fabsf(p[0]), fmaxf(fabsf(p[1]), fabsf(p[2])));
-          const float eps = 1e-6f + 1.2e-7f * am;
-          bool esc;
-          float tb = next_boundary(sp, L.W.n_prim, L.W, p, d, eps, esc);
-          if (!(tb < INFINITY) || tb < 0.f) {   // outside the world box
-            escaped = true;
-            break;
-          }
+        alive = true;
+        ep = energy_pos(T, E);
+        if (L.grid_on) wh = (double)w * (double)h10_coeff(T.h10, T.n_h10, E);
+      }
+    }
+    while (__ballot(alive)) {
+      float ds = 0.f, taub = 0.f;
+      bool inter = false, esc = false, seg = false;
+      int m = 0;
+      if (alive) {
+        ++events;
+        const float am = fmaxf(fabsf(p[0]), fmaxf(fabsf(p[1]), fabsf(p[2])));
+        const float eps = 1e-6f + 1.2e-7f * am;
+        float tb = next_boundary(sp, L.W.n_prim, L.W, p, d, eps, esc);
+        if (!(tb < INFINITY) || tb < 0.f) {   // outside the world box
+          escaped = true;
+          alive = false;
+        } else {
           const float mid[3] = {p[0] + d[0] * 0.5f * tb, p[1] + d[1] * 0.5f * tb, p[2] + d[2] * 0.5f * tb};
-          const int m = locate(sp, L.W.n_prim, L.W.background, mid);
+          m = locate(sp, L.W.n_prim, L.W.background, mid);
           if (m != mc) {
             mc = m;
             const float rho = T.density[m];
             mu = xs_part(T, m, P_TOTAL, E, ep) * rho;
             if (L.grid_on) muen = xs_part(T, m, P_MUEN, E, ep) * rho;
           }
-          const float taub = mu * tb;
-          const bool inter = tau < taub;
-          const float ds = inter ? tau / fmaxf(mu, 1e-30f) : tb;
-          if (L.grid_on) visits += dda_score(L.G, p, d, ds, (double)w * (double)E * (double)muen, wh);
-          p[0] += d[0] * ds; p[1] += d[1] * ds; p[2] += d[2] * ds;
-          if (inter) {
-            PhiloxRng rng(L.seed, L.hist0 + (uint64_t)i, 0u, S.ctr[i]);
-            const float r = rng.next();
-            S.ctr[i] = rng.ctr;
-            const float ph = xs_part(T, m, P_PHOTO, E, ep);
-            const float co = xs_part(T, m, P_COMPT, E, ep);
-            const float ra = xs_part(T, m, P_RAYL, E, ep);
-            const float tot = ph + co + ra;
-            const float pp = tot > 0.f ? ph / tot : 0.f;
-            const float pc = tot > 0.f ? co / tot : 0.f;
-            type = (r < pp) ? 0 : ((r < pp + pc) ? 1 : 2);
-            S.mat[i] = m;
-            tau = 0.f;
-            break;
-          }
+          taub = mu * tb;
+          inter = tau < taub;
+          ds = inter ? tau / fmaxf(mu, 1e-30f) : tb;
+          seg = L.grid_on != 0;
+        }
+      }
+      if (L.grid_on) visits += dda_score(L.G, p, d, ds, (double)w * (double)E * (double)muen, wh, seg);
+      if (alive) {
+        p[0] += d[0] * ds; p[1] += d[1] * ds; p[2] += d[2] * ds;
+        if (inter) {
+          PhiloxRng rng(L.seed, L.hist0 + (uint64_t)i, 0u, S.ctr[i]);
+          const float r = rng.next();
+          S.ctr[i] = rng.ctr;
+          const float ph = xs_part(T, m, P_PHOTO, E, ep);
+          const float co = xs_part(T, m, P_COMPT, E, ep);
+          const float ra = xs_part(T, m, P_RAYL, E, ep);
+          const float tot = ph + co + ra;
+          const float pp = tot > 0.f ? ph / tot : 0.f;
+          const float pc = tot > 0.f ? co / tot : 0.f;
+          type = (r < pp) ? 0 : ((r < pp + pc) ? 1 : 2);
+          S.mat[i] = m;
+          tau = 0.f;
+          alive = false;
+        } else {
           tau -= taub;
           if (esc) {
             escaped = true;
-            break;
+            alive = false;
+          } else if (++it >= kMaxCrossings) {
+            raise_err(L.err, CC_DEV_STEP_CAP);
+            S.flags[i] |= F_DEAD;
+            alive = false;
           }
         }
-        if (it == kMaxCrossings) {
-          raise_err(L.err, CC_DEV_STEP_CAP);
-          S.flags[i] |= F_DEAD;
-        }
-        if (escaped) S.flags[i] |= F_ESCAPED;
-        S.x[i] = p[0]; S.y[i] = p[1]; S.z[i] = p[2];
-        S.tau[i] = tau;
       }
+    }
+    if (valid && energy_ok(E)) {
+      if (escaped) S.flags[i] |= F_ESCAPED;
+      S.x[i] = p[0]; S.y[i] = p[1]; S.z[i] = p[2];
+      S.tau[i] = tau;
     }
     wave_count(valid && escaped, L.stats + S_ESC);
     wave_count(valid, L.stats + S_FLIGHT);
@@ -611,11 +628,20 @@ __global__ void k_test_iso(int n, const float* u1, const float* u2, float* out) 
 
 __global__ void k_test_dda(GridDev G, int n, const float* o, const float* d, const float* len, const double* wk,
                            const double* wh) {
+  // every lane takes part in the wave-aggregated tally; lanes past n score nothing
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float oo[3] = {o[3 * i], o[3 * i + 1], o[3 * i + 2]};
-  float dd[3] = {d[3 * i], d[3 * i + 1], d[3 * i + 2]};
-  dda_score(G, oo, dd, len[i], wk[i], wh[i]);
+  const bool valid = i < n;
+  float oo[3] = {0.f, 0.f, 0.f}, dd[3] = {0.f, 0.f, 1.f};
+  float L = 0.f;
+  double a = 0.0, b = 0.0;
+  if (valid) {
+    oo[0] = o[3 * i]; oo[1] = o[3 * i + 1]; oo[2] = o[3 * i + 2];
+    dd[0] = d[3 * i]; dd[1] = d[3 * i + 1]; dd[2] = d[3 * i + 2];
+    L = len[i];
+    a = wk[i];
+    b = wh[i];
+  }
+  dda_score(G, oo, dd, L, a, b, valid);
 }
 
 __global__ void k_test_philox(int n, const uint32_t* c, const uint32_t* k, uint32_t* out) {
@@ -784,6 +810,11 @@ struct Timer {
 enum { K_SOURCE = 0, K_FLIGHT = 1, K_PHOTO = 2, K_COMPTON = 3, K_RAYLEIGH = 4, K_END = 5, K_OTHER = 6 };
 
 // One batch of photons already in the pool [0, n): rounds until no photon is active.
+// Rounds are issued in chunks without a host wait in between; the active count
+// of every round is copied to pinned memory and inspected once per chunk.  The
+// active population never grows, so the count at the start of a chunk bounds
+// every grid of the chunk; rounds that find an empty list exit at once.
+constexpr int kRoundChunk = 6;
 int run_rounds(Ctx* c, const Launch& L, uint32_t n, Timer& tm, uint32_t* rounds_out) {
   int cur = C_ACT_A, nxt = C_ACT_B;
   int* act_cur = c->act0;
@@ -792,29 +823,38 @@ int run_rounds(Ctx* c, const Launch& L, uint32_t n, Timer& tm, uint32_t* rounds_
   uint32_t rounds = 0;
   while (active > 0) {
     if (rounds >= (uint32_t)kMaxRounds) return fail(CC_ERR_DEVICE, "round cap hit");
-    ++rounds;
     const int nb = grid_blocks(active);
-    hipLaunchKernelGGL(k_round_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, nxt);
-    tm.begin(K_FLIGHT);
-    hipLaunchKernelGGL(k_flight, dim3(nb), dim3(kBlock), 0, c->stream, L, act_cur, c->cnt + cur, c->qp, c->qc, c->qr);
-    tm.end();
-    tm.begin(K_PHOTO);
-    hipLaunchKernelGGL(k_photo, dim3(nb), dim3(kBlock), 0, c->stream, L, c->qp, c->cnt + C_QP, act_nxt, c->cnt + nxt);
-    tm.end();
-    tm.begin(K_COMPTON);
-    hipLaunchKernelGGL(k_compton, dim3(nb), dim3(kBlock), 0, c->stream, L, c->qc, c->cnt + C_QC, act_nxt, c->cnt + nxt);
-    tm.end();
-    tm.begin(K_RAYLEIGH);
-    hipLaunchKernelGGL(k_rayleigh, dim3(nb), dim3(kBlock), 0, c->stream, L, c->qr, c->cnt + C_QR, act_nxt, c->cnt + nxt);
-    tm.end();
+    for (int r = 0; r < kRoundChunk; ++r) {
+      hipLaunchKernelGGL(k_round_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, nxt);
+      tm.begin(K_FLIGHT);
+      hipLaunchKernelGGL(k_flight, dim3(nb), dim3(kBlock), 0, c->stream, L, act_cur, c->cnt + cur, c->qp, c->qc,
+                         c->qr);
+      tm.end();
+      tm.begin(K_PHOTO);
+      hipLaunchKernelGGL(k_photo, dim3(nb), dim3(kBlock), 0, c->stream, L, c->qp, c->cnt + C_QP, act_nxt,
+                         c->cnt + nxt);
+      tm.end();
+      tm.begin(K_COMPTON);
+      hipLaunchKernelGGL(k_compton, dim3(nb), dim3(kBlock), 0, c->stream, L, c->qc, c->cnt + C_QC, act_nxt,
+                         c->cnt + nxt);
+      tm.end();
+      tm.begin(K_RAYLEIGH);
+      hipLaunchKernelGGL(k_rayleigh, dim3(nb), dim3(kBlock), 0, c->stream, L, c->qr, c->cnt + C_QR, act_nxt,
+                         c->cnt + nxt);
+      tm.end();
+      HIP_TRY(hipMemcpyAsync(c->h_cnt + r, c->cnt + nxt, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+      std::swap(cur, nxt);
+      std::swap(act_cur, act_nxt);
+    }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(c->h_cnt, c->cnt + nxt, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    active = *c->h_cnt;
-    std::swap(cur, nxt);
-    std::swap(act_cur, act_nxt);
+    for (int r = 0; r < kRoundChunk; ++r) {
+      if (c->h_cnt[r] == 0) break;
+      ++rounds;
+    }
+    active = c->h_cnt[kRoundChunk - 1];
   }
-  if (rounds_out) *rounds_out = rounds;
+  if (rounds_out) *rounds_out = rounds + 1;
   return CC_OK;
 }
 
@@ -939,7 +979,7 @@ int cc_ctx_create(int32_t device, void** out) {
     delete c;
     return r;
   }
-  HIP_TRY(hipHostMalloc((void**)&c->h_cnt, sizeof(uint32_t) * 4, hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&c->h_cnt, sizeof(uint32_t) * 16, hipHostMallocDefault));
   HIP_TRY(hipMemset(c->err, 0, sizeof(int)));
   *out = c;
   return CC_OK;
