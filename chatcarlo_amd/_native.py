@@ -15,6 +15,12 @@ LIB_PATH = Path(__file__).resolve().parent / "libchatcarlo_hip.so"
 CC_OK, CC_ERR_ARG, CC_ERR_HIP, CC_ERR_STATE, CC_ERR_DEVICE = 0, -1, -2, -3, -4
 CC_MAX_PRIMS, CC_MAX_MAT = 64, 32
 RUN_FLUORESCENCE, RUN_GRID, RUN_TRACK_UNCERTAINTY, RUN_TIMING = 1, 2, 4, 8
+RUN_REGION, RUN_RECORD, RUN_IMPORTANCE, RUN_SOURCE_BIAS = 16, 32, 64, 128
+MLP_IN, MLP_HIDDEN = 16, 64
+MLP_PARAMS = MLP_IN * MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN + 1
+RECORD_FLOATS = 12
+SRC_CELLS = 512
+MAP_FLIGHT, MAP_COLLISION = 0, 1
 DEV_ENERGY_RANGE, DEV_ITER_CAP, DEV_Q_RANGE, DEV_STEP_CAP, DEV_ROUND_CAP = 1, 2, 4, 8, 16
 
 fp = C.POINTER(C.c_float)
@@ -63,7 +69,29 @@ class CcRunOut(C.Structure):
                 ("n_voxel_visits", C.c_uint64), ("error_bits", C.c_uint32),
                 ("rounds_max", C.c_uint32), ("edep_keV", C.c_double * CC_MAX_MAT),
                 ("edep_sum2", C.c_double * CC_MAX_MAT), ("kernel_ms", C.c_double * 8),
-                ("kernel_launches", C.c_uint64 * 8)]
+                ("kernel_launches", C.c_uint64 * 8),
+                ("region_sum", C.c_double), ("region_sum2", C.c_double),
+                ("region_batch_sum2", C.c_double), ("n_split", C.c_uint64),
+                ("n_roulette", C.c_uint64), ("n_records", C.c_uint64)]
+
+
+class CcTrainCfg(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("yscale", C.c_float), ("lr_end", C.c_float),
+                ("seed", C.c_uint64), ("target_power", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class CcTrainOut(C.Structure):
+    _fields_ = [("loss_first", C.c_double), ("loss_last", C.c_double), ("ms", C.c_double),
+                ("yscale", C.c_float), ("n_records", C.c_uint32), ("n_positive", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class CcImportanceCfg(C.Structure):
+    _fields_ = [("cw", C.c_float), ("w_lo", C.c_float), ("w_hi", C.c_float), ("split_max", C.c_int32),
+                ("pool_factor", C.c_float), ("source_bias", C.c_int32), ("bias_mix", C.c_float),
+                ("samples_per_cell", C.c_int32), ("use_post", C.c_int32), ("use_pre", C.c_int32),
+                ("split_gain", C.c_float), ("pad", C.c_uint32), ("seed", C.c_uint64)]
 
 
 _SIGS = {
@@ -94,6 +122,17 @@ _SIGS = {
     "cc_test_philox": (C.c_int, [C.c_int64, u32p, u32p, u32p]),
     "cc_test_source": (C.c_int, [C.c_void_p, C.POINTER(CcSource), C.c_int64, C.c_uint64, C.c_uint64,
                                  fp, fp, fp]),
+    "cc_set_region": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double]),
+    "cc_records_alloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_float]),
+    "cc_records_read": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, fp, u32p]),
+    "cc_records_clear": (C.c_int, [C.c_void_p]),
+    "cc_mlp_set_params": (C.c_int, [C.c_void_p, C.c_int32, fp]),
+    "cc_mlp_get_params": (C.c_int, [C.c_void_p, C.c_int32, fp]),
+    "cc_mlp_features": (C.c_int, [C.c_void_p, C.c_int64, fp, fp, fp, fp]),
+    "cc_mlp_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, fp, fp]),
+    "cc_mlp_grad": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, fp, fp, C.c_float, fp, dp]),
+    "cc_mlp_train": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(CcTrainCfg), C.POINTER(CcTrainOut), dp]),
+    "cc_set_importance": (C.c_int, [C.c_void_p, C.POINTER(CcImportanceCfg), C.POINTER(CcSource), fp, fp]),
 }
 EXPORTS = tuple(_SIGS)
 

@@ -291,16 +291,17 @@ __device__ __forceinline__ int locate(const Prim* prims, int n_prim, int backgro
   return background;
 }
 
-// Min positive root over all primitives vs the world exit (chatcarlo/geometry.py:214-231).
+// Min root beyond tmin over all primitives vs the world exit (chatcarlo/geometry.py:214-231,
+// where tmin is the reference's 1e-6 cm threshold).
 __device__ __forceinline__ float next_boundary(const Prim* prims, int n_prim, const World& W,
-                                               const float o[3], const float d[3], float eps,
+                                               const float o[3], const float d[3], float tmin,
                                                bool& escape) {
   float t_obj = INFINITY;
   for (int i = 0; i < n_prim; ++i) {
     float t1, t2;
     if (isect_prim(prims[i], o, d, t1, t2)) {
-      if (t1 > eps && t1 < t_obj) t_obj = t1;
-      if (t2 > eps && t2 < t_obj) t_obj = t2;
+      if (t1 > tmin && t1 < t_obj) t_obj = t1;
+      if (t2 > tmin && t2 < t_obj) t_obj = t2;
     }
   }
   float ten, tex;

@@ -20,6 +20,7 @@
 
 #include "../../include/chatcarlo_hip.h"
 #include "cc_device.hpp"
+#include "cc_mlp.hpp"
 
 using namespace cc;
 
@@ -53,13 +54,14 @@ constexpr int kTrialCap = 100000;
 
 enum { F_ABSORBED = 1, F_ESCAPED = 2, F_DEAD = 4 };
 // device counter slots (uint32)
-enum { C_ACT_A = 0, C_ACT_B = 1, C_QP = 2, C_QC = 3, C_QR = 4, C_NCNT = 8 };
+enum { C_ACT_A = 0, C_ACT_B = 1, C_QP = 2, C_QC = 3, C_QR = 4, C_IMP = 5, C_POOL = 6, C_COLL = 7, C_NCNT = 8 };
 // device stats slots (u64)
-enum { S_ABS = 0, S_ESC = 1, S_SCAT = 2, S_FLUOR = 3, S_EVT = 4, S_FLIGHT = 5, S_VOX = 6, S_NSTAT = 8 };
+enum { S_ABS = 0, S_ESC = 1, S_SCAT = 2, S_FLUOR = 3, S_EVT = 4, S_FLIGHT = 5, S_VOX = 6, S_SPLIT = 7, S_ROUL = 8,
+       S_NSTAT = 10 };
 
 struct PState {
-  float *x, *y, *z, *dx, *dy, *dz, *e, *tau, *wt;
-  uint32_t* ctr;
+  float *x, *y, *z, *dx, *dy, *dz, *e, *tau, *wt, *score;
+  uint32_t *ctr, *hid, *tag;   // Philox event counter, history offset, lane tag (split copies)
   int *nsc, *mat, *flags;
 };
 
@@ -73,6 +75,55 @@ struct Src {
   const float* spec_cdf;
 };
 
+struct Region {            // spherical scoring region for H*(10) (operator position)
+  float c[3], r, inv_vol;
+  int on;
+};
+
+// Source biasing over the stratified uniforms (u_a, u_b, u_E) of the source
+// sampler: cell k of a kSrcCells^3 grid is drawn with probability p_k and the
+// photon carries weight (1 / K) / p_k, so every cell keeps its analog share.
+constexpr int kSrcSide = 8;
+constexpr int kSrcCells = kSrcSide * kSrcSide * kSrcSide;
+struct SrcBias {
+  const float* cdf;        // [K] cumulative p_k, last = 1
+  const float* wt;         // [K] (1 / K) / p_k
+  int on;
+};
+
+struct Importance {        // weight windows from the learned importance map
+  const float* w;          // fp32 parameters (b2, w3, b3 read from here)
+  const uint16_t* wb;      // bf16 copies of W1 | W2
+  MlpFrame frame;
+  float cw, w_lo, w_hi;    // target weight = clamp(cw / exp(fscale f), w_lo, w_hi)
+  float split_gain;        // collision split count = split_gain w exp(f(region direction)) / cw
+  int split_max;
+  uint32_t pool_cap;       // particle slots; the next free one is counted in cnt[C_POOL]
+  int on;
+};
+
+// Contribution records of analog runs, two kinds: flight starts (state about to
+// fly) in [0, cap / 2) and collisions (collision point, incoming direction and
+// energy) in [cap / 2, cap).
+struct Records {
+  float* rec;              // [cap][12]: x y z dx dy dz E s_before target slot(bits) batch kind
+  uint32_t* cnt;           // [0], [2] records appended of kind 0, 1; [1], [3] counts at the batch start
+  uint32_t cap;
+  float keep;              // probability that a photon is recorded at a flight start or collision
+  int on;
+};
+
+__device__ __forceinline__ void put_record(const Records& R, int kind, uint32_t k, const float p[3], const float d[3],
+                                           float E, float score, int slot, uint32_t batch) {
+  const uint32_t half = R.cap / 2;
+  if (k >= (kind == 0 ? half : R.cap - half)) return;
+  float* r = R.rec + (size_t)(kind == 0 ? k : half + k) * 12;
+  r[0] = p[0]; r[1] = p[1]; r[2] = p[2];
+  r[3] = d[0]; r[4] = d[1]; r[5] = d[2];
+  r[6] = E; r[7] = score; r[8] = 0.f;
+  r[9] = __int_as_float(slot); r[10] = __uint_as_float(batch); r[11] = (float)kind;
+}
+
 struct Launch {
   Tables T;
   const Prim* prims;
@@ -83,8 +134,12 @@ struct Launch {
   uint32_t* cnt;
   unsigned long long* stats;
   double* edep;
+  double* hscore;          // per-history region score of the batch (indexed by history offset)
   int* err;
   uint64_t seed, hist0;
+  Region R;
+  Records Rec;
+  uint32_t rec_batch;      // batch index written into contribution records
 };
 
 // ---------------------------------------------------------------------------
@@ -214,19 +269,77 @@ __device__ __forceinline__ void sample_source(const Src& s, Rng& rng, float p[3]
   E = s.spec_e[lo];
 }
 
-__global__ __launch_bounds__(kBlock) void k_source(Launch L, Src s, int n, int* act) {
+// Philox uniforms with the draws of the field offsets and the energy (calls
+// 2, 3 and 4 of the source sampler) moved into one cell of the 8^3 grid.
+struct CellRng {
+  PhiloxRng& r;
+  int call;
+  float lo[3];
+  __device__ __forceinline__ float next() {
+    float u = r.next();
+    const int k = call++;
+    if (k >= 2 && k <= 4) u = lo[k - 2] + u * (1.0f / (float)kSrcSide);
+    return u;
+  }
+};
+
+__device__ __forceinline__ CellRng cell_rng(PhiloxRng& r, int cell) {
+  const float h = 1.0f / (float)kSrcSide;
+  return CellRng{r, 0, {(float)(cell % kSrcSide) * h, (float)((cell / kSrcSide) % kSrcSide) * h,
+                        (float)(cell / (kSrcSide * kSrcSide)) * h}};
+}
+
+__global__ __launch_bounds__(kBlock) void k_source(Launch L, Src s, SrcBias B, int n, int* act) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   PhiloxRng rng(L.seed, L.hist0 + (uint64_t)i, 0u, 0u);
-  float p[3], d[3], E;
-  sample_source(s, rng, p, d, E);
+  float p[3], d[3], E, w = 1.0f;
+  if (B.on) {
+    // cell with probability p_k (first cdf entry > u), then the analog sampler inside it
+    const float us = rng.next();
+    int lo = 0, hi = kSrcCells - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (B.cdf[mid] > us) hi = mid; else lo = mid + 1;
+    }
+    w = B.wt[lo];
+    CellRng cr = cell_rng(rng, lo);
+    sample_source(s, cr, p, d, E);
+  } else {
+    sample_source(s, rng, p, d, E);
+  }
   float tau = -logf(rng.next());
   PState S = L.S;
   S.x[i] = p[0]; S.y[i] = p[1]; S.z[i] = p[2];
   S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
-  S.e[i] = E; S.tau[i] = tau; S.wt[i] = 1.0f;
-  S.ctr[i] = rng.ctr; S.nsc[i] = 0; S.mat[i] = -1; S.flags[i] = 0;
+  S.e[i] = E; S.tau[i] = tau; S.wt[i] = w; S.score[i] = 0.f;
+  S.ctr[i] = rng.ctr; S.hid[i] = (uint32_t)i; S.tag[i] = 0u; S.nsc[i] = 0; S.mat[i] = -1; S.flags[i] = 0;
   act[i] = i;
+}
+
+// Feature rows of `per` source photons in each of the kSrcCells cells (for the
+// mean importance of every cell).
+__global__ __launch_bounds__(kBlock) void k_source_cells(Src s, uint64_t seed, int per, MlpFrame F, float* feat) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= kSrcCells * per) return;
+  PhiloxRng rng(seed, (uint64_t)t, 0x5EEDu, 0u);
+  CellRng cr = cell_rng(rng, t / per);
+  float p[3], d[3], E;
+  sample_source(s, cr, p, d, E);
+  float x[kMlpIn];
+  mlp_features(F, p, d, E, x);
+  for (int k = 0; k < kMlpIn; ++k) feat[(size_t)t * kMlpIn + k] = x[k];
+}
+
+// Feature rows of caller-supplied states.
+__global__ void k_features(MlpFrame F, int n, const float* pos, const float* dir, const float* e, float* feat) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float p[3] = {pos[3 * t], pos[3 * t + 1], pos[3 * t + 2]};
+  const float d[3] = {dir[3 * t], dir[3 * t + 1], dir[3 * t + 2]};
+  float x[kMlpIn];
+  mlp_features(F, p, d, e[t], x);
+  for (int k = 0; k < kMlpIn; ++k) feat[(size_t)t * kMlpIn + k] = x[k];
 }
 
 // caller-supplied photons: only tau is drawn (chatcarlo/transport.py:124)
@@ -237,13 +350,14 @@ __global__ __launch_bounds__(kBlock) void k_init_given(Launch L, int n, int* act
   PState S = L.S;
   S.tau[i] = -logf(rng.next());
   S.wt[i] = 1.0f;
-  S.ctr[i] = rng.ctr; S.nsc[i] = 0; S.mat[i] = -1; S.flags[i] = 0;
+  S.score[i] = 0.f;
+  S.ctr[i] = rng.ctr; S.hid[i] = (uint32_t)i; S.tag[i] = 0u; S.nsc[i] = 0; S.mat[i] = -1; S.flags[i] = 0;
   act[i] = i;
 }
 
 __global__ void k_round_reset(uint32_t* cnt, int next_slot) {
   if (threadIdx.x == 0) {
-    cnt[C_QP] = 0; cnt[C_QC] = 0; cnt[C_QR] = 0; cnt[next_slot] = 0;
+    cnt[C_QP] = 0; cnt[C_QC] = 0; cnt[C_QR] = 0; cnt[C_IMP] = 0; cnt[C_COLL] = 0; cnt[next_slot] = 0;
   }
 }
 
@@ -251,16 +365,91 @@ __global__ void k_batch_reset(uint32_t* cnt, unsigned long long* stats, double* 
   const int t = threadIdx.x;
   if (t < C_NCNT) cnt[t] = 0;
   if (t < S_NSTAT) stats[t] = 0ull;
-  if (t < CC_MAX_MAT) edep[t] = 0.0;
+  if (t < CC_MAX_MAT + 2) edep[t] = 0.0;   // per-material deposits, then the region sums
   __syncthreads();
-  if (t == 0) cnt[cur_slot] = n;
+  if (t == 0) {
+    cnt[cur_slot] = n;
+    cnt[C_POOL] = n;
+  }
+}
+
+__device__ __forceinline__ uint32_t mix32(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA6Bu ^ (c + 1u) * 0xC2B2AE35u;
+  h ^= h >> 16; h *= 0x7FEB352Du; h ^= h >> 15; h *= 0x846CA68Bu; h ^= h >> 16;
+  return h | 1u;
+}
+
+// Interaction type from the part fractions at (material, E) (chatcarlo/transport.py:72-85).
+__device__ __forceinline__ int sample_type(const Tables& T, int m, float E, const EPos& ep, float r) {
+  const float ph = xs_part(T, m, P_PHOTO, E, ep);
+  const float co = xs_part(T, m, P_COMPT, E, ep);
+  const float ra = xs_part(T, m, P_RAYL, E, ep);
+  const float tot = ph + co + ra;
+  const float pp = tot > 0.f ? ph / tot : 0.f;
+  const float pc = tot > 0.f ? co / tot : 0.f;
+  return (r < pp) ? 0 : ((r < pp + pc) ? 1 : 2);
+}
+
+__device__ __forceinline__ bool record_take(const Records& R, uint32_t hid, uint32_t ctr, uint32_t salt) {
+  if (R.keep >= 1.f) return true;
+  return mix32(hid, ctr, salt) < (uint32_t)(R.keep * 4294967296.0f);
+}
+
+// ---------------------------------------------------------------------------
+// expected region score of one flight
+// ---------------------------------------------------------------------------
+// Expected track length inside the scoring sphere of the uncollided flight
+// o0 + t d, t >= 0: the integral of exp(-optical depth to s) over the chord,
+// walked boundary by boundary up to the sphere's far side.  Its expectation is
+// the realised chord's, so contribution records built from it keep their mean
+// and lose the 0-or-chord noise of single flights.
+__device__ float region_expect(const Prim* sp, const World& W, const Tables& T, const Region& R,
+                               const float o0[3], const float d[3], float E, const EPos& ep, float eps0) {
+  const float ox = o0[0] - R.c[0], oy = o0[1] - R.c[1], oz = o0[2] - R.c[2];
+  const float b = ox * d[0] + oy * d[1] + oz * d[2];
+  const float disc = b * b - (ox * ox + oy * oy + oz * oz - R.r * R.r);
+  if (!(disc > 0.f)) return 0.f;
+  const float sq = sqrtf(disc);
+  const float s_in = fmaxf(-b - sq, 0.f), s_out = -b + sq;
+  if (!(s_out > s_in)) return 0.f;
+  float t = 0.f, tau = 0.f, sum = 0.f;
+  int mc = -1;
+  float mu = 0.f;
+  for (int it = 0; it < 4 * CC_MAX_PRIMS + 4 && t < s_out && tau < 50.f; ++it) {
+    bool esc;
+    const float tn = next_boundary(sp, W.n_prim, W, o0, d, it == 0 ? eps0 : t, esc);
+    if (!(tn < INFINITY) || tn < 0.f) break;
+    const float th = 0.5f * (t + tn);
+    const float mid[3] = {o0[0] + d[0] * th, o0[1] + d[1] * th, o0[2] + d[2] * th};
+    const int m = locate(sp, W.n_prim, W.background, mid);
+    if (m != mc) {
+      mc = m;
+      mu = xs_part(T, m, P_TOTAL, E, ep) * T.density[m];
+    }
+    const float a = fmaxf(t, s_in), e = fminf(tn, s_out);
+    if (e > a) {
+      const float x = mu * (e - a);
+      // integral of exp(-tau - mu (s - t)) over [a, e]; (1 - exp(-x)) / mu written stably
+      const float len = x > 1e-4f ? -expm1f(-x) / mu : (e - a) * (1.f - 0.5f * x);
+      sum += expf(-(tau + mu * (a - t))) * len;
+    }
+    tau += mu * (tn - t);
+    t = tn;
+    if (esc) break;
+  }
+  return sum;
 }
 
 // ---------------------------------------------------------------------------
 // flight: boundary walk until interaction or escape
 // ---------------------------------------------------------------------------
+// coll != nullptr: interacting photons go to the collision list (their type is
+// drawn after the collision's weight window); otherwise to the type queues.
+// kRec: contribution-record runs (expected flight scores, collision records), a
+// separate instantiation so the plain transport keeps its register budget.
+template <bool kRec>
 __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, const uint32_t* cnt_in, int* qp,
-                                                   int* qc, int* qr) {
+                                                   int* qc, int* qr, int* coll) {
   __shared__ Prim sp[CC_MAX_PRIMS];
   stage_prims(L.prims, sp, L.W.n_prim);
   const uint32_t n = *cnt_in;
@@ -276,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     bool escaped = false, alive = false;
     unsigned long long events = 0, visits = 0;
     float p[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 1.f};
-    float E = 1.0f, tau = 0.f, w = 0.f, mu = 0.f, muen = 0.f;
+    float E = 1.0f, tau = 0.f, w = 0.f, mu = 0.f, muen = 0.f, hreg = 0.f, sreg = 0.f;
     double wh = 0.0;
     int mc = -1, it = 0;
     EPos ep{0, 0.f, 0.f};
@@ -293,23 +482,37 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
       } else {
         alive = true;
         ep = energy_pos(T, E);
-        if (L.grid_on) wh = (double)w * (double)h10_coeff(T.h10, T.n_h10, E);
+        const float h10 = (L.grid_on || L.R.on) ? h10_coeff(T.h10, T.n_h10, E) : 0.f;
+        if (L.grid_on) wh = (double)w * (double)h10;
+        hreg = w * h10 * L.R.inv_vol;
       }
     }
+    // The flight is the straight line o0 + t d.  Every boundary distance is a root
+    // measured from o0, so a crossed surface keeps exactly the root it had and the
+    // next boundary is the first root beyond the distance flown, tc; measured from a
+    // rounded current point instead, the surface just crossed reappears a few ulps
+    // ahead (by rounding / cos of the incidence angle) and a grazing photon creeps.
+    const float o0[3] = {p[0], p[1], p[2]};
+    const float eps0 = 1e-6f + 1.2e-7f * fmaxf(fabsf(p[0]), fmaxf(fabsf(p[1]), fabsf(p[2])));
+    float tc = 0.f;
+    float sexp = 0.f;   // expected region score of this flight (contribution records)
+    if (kRec && alive) sexp = hreg * region_expect(sp, L.W, T, L.R, o0, d, E, ep, eps0);
     while (__ballot(alive)) {
-      float ds = 0.f, taub = 0.f;
+      float ds = 0.f, taub = 0.f, t_next = 0.f;
       bool inter = false, esc = false, seg = false;
       int m = 0;
       if (alive) {
         ++events;
-        const float am = fmaxf(fabsf(p[0]), fmaxf(fabsf(p[1]), fabsf(p[2])));
-        const float eps = 1e-6f + 1.2e-7f * am;
-        float tb = next_boundary(sp, L.W.n_prim, L.W, p, d, eps, esc);
-        if (!(tb < INFINITY) || tb < 0.f) {   // outside the world box
+        t_next = next_boundary(sp, L.W.n_prim, L.W, o0, d, it == 0 ? eps0 : tc, esc);
+        if (!(t_next < INFINITY) || t_next < 0.f) {   // outside the world box
           escaped = true;
           alive = false;
         } else {
-          const float mid[3] = {p[0] + d[0] * 0.5f * tb, p[1] + d[1] * 0.5f * tb, p[2] + d[2] * 0.5f * tb};
+          const float tb = t_next - tc;
+          const float th = tc + 0.5f * tb;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) p[k] = o0[k] + d[k] * tc;
+          const float mid[3] = {o0[0] + d[0] * th, o0[1] + d[1] * th, o0[2] + d[2] * th};
           m = locate(sp, L.W.n_prim, L.W.background, mid);
           if (m != mc) {
             mc = m;
@@ -324,24 +527,34 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
         }
       }
       if (L.grid_on) visits += dda_score(L.G, p, d, ds, (double)w * (double)E * (double)muen, wh, seg);
+      if (alive && L.R.on) {
+        // chord of the segment [0, ds] through the scoring sphere
+        const float ox = p[0] - L.R.c[0], oy = p[1] - L.R.c[1], oz = p[2] - L.R.c[2];
+        const float b = ox * d[0] + oy * d[1] + oz * d[2];
+        const float cc = ox * ox + oy * oy + oz * oz - L.R.r * L.R.r;
+        const float disc = b * b - cc;
+        if (disc > 0.f) {
+          const float sq = sqrtf(disc);
+          const float lo = fmaxf(-b - sq, 0.f), hi = fminf(-b + sq, ds);
+          if (hi > lo) sreg += hreg * (hi - lo);
+        }
+      }
       if (alive) {
-        p[0] += d[0] * ds; p[1] += d[1] * ds; p[2] += d[2] * ds;
         if (inter) {
-          PhiloxRng rng(L.seed, L.hist0 + (uint64_t)i, 0u, S.ctr[i]);
-          const float r = rng.next();
-          S.ctr[i] = rng.ctr;
-          const float ph = xs_part(T, m, P_PHOTO, E, ep);
-          const float co = xs_part(T, m, P_COMPT, E, ep);
-          const float ra = xs_part(T, m, P_RAYL, E, ep);
-          const float tot = ph + co + ra;
-          const float pp = tot > 0.f ? ph / tot : 0.f;
-          const float pc = tot > 0.f ? co / tot : 0.f;
-          type = (r < pp) ? 0 : ((r < pp + pc) ? 1 : 2);
+          tc += ds;
+          if (coll) {
+            type = 3;
+          } else {
+            PhiloxRng rng(L.seed, L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
+            type = sample_type(T, m, E, ep, rng.next());
+            S.ctr[i] = rng.ctr;
+          }
           S.mat[i] = m;
           tau = 0.f;
           alive = false;
         } else {
           tau -= taub;
+          tc = t_next;
           if (esc) {
             escaped = true;
             alive = false;
@@ -355,9 +568,18 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     }
     if (valid && energy_ok(E)) {
       if (escaped) S.flags[i] |= F_ESCAPED;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) p[k] = o0[k] + d[k] * tc;
       S.x[i] = p[0]; S.y[i] = p[1]; S.z[i] = p[2];
       S.tau[i] = tau;
+      if (L.R.on) S.score[i] += kRec ? sexp : sreg;
     }
+    if (kRec) {   // collision record: collision point, incoming direction and energy
+      const bool take = type >= 0 && record_take(L.Rec, S.hid[i] ^ (uint32_t)L.hist0, S.ctr[i], 0x9E37u);
+      const int k = wave_append(take, L.Rec.cnt + 2);
+      if (take) put_record(L.Rec, 1, (uint32_t)k, p, d, E, S.score[i], i, L.rec_batch);
+    }
+    if (sreg > 0.f) atomic_add_f64(L.hscore + S.hid[i], (double)sreg);
     wave_count(valid && escaped, L.stats + S_ESC);
     wave_count(valid, L.stats + S_FLIGHT);
     wave_sum_u64(events, L.stats + S_EVT);
@@ -369,6 +591,8 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     if (slot >= 0) qc[slot] = i;
     slot = wave_append(type == 2, L.cnt + C_QR);
     if (slot >= 0) qr[slot] = i;
+    slot = wave_append(type == 3, L.cnt + C_COLL);
+    if (slot >= 0) coll[slot] = i;
   }
 }
 
@@ -391,7 +615,7 @@ __global__ __launch_bounds__(kBlock) void k_photo(Launch L, const int* q, const 
       m = S.mat[i];
       const float E = S.e[i];
       const float w = S.wt[i];
-      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)i, 0u, S.ctr[i]);
+      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
       float eline = 0.f;
       if (L.fluor) {
         const EPos ep = energy_pos(T, E);
@@ -435,7 +659,7 @@ __global__ __launch_bounds__(kBlock) void k_compton(Launch L, const int* q, cons
       m = S.mat[i];
       const float E = S.e[i];
       const float w = S.wt[i];
-      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)i, 0u, S.ctr[i]);
+      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
       const EPos ep = energy_pos(T, E);
       float cs, sn;
       int el, trials;
@@ -471,7 +695,7 @@ __global__ __launch_bounds__(kBlock) void k_rayleigh(Launch L, const int* q, con
       i = q[j];
       const int m = S.mat[i];
       const float E = S.e[i];
-      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)i, 0u, S.ctr[i]);
+      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
       const EPos ep = energy_pos(T, E);
       int el, trials;
       bool qerr;
@@ -490,6 +714,573 @@ __global__ __launch_bounds__(kBlock) void k_rayleigh(Launch L, const int* q, con
     const int slot = wave_append(valid, cnt_next);
     if (slot >= 0) act_next[slot] = i;
   }
+}
+
+// ---------------------------------------------------------------------------
+// learned importance: weight windows, splitting and Russian roulette
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t& total) {
+  const int lane = __lane_id();
+  uint32_t x = v;
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  total = __shfl(x, kWave - 1);
+  return x - v;
+}
+
+__device__ __forceinline__ void copy_particle(const PState& S, int from, int to) {
+  S.x[to] = S.x[from]; S.y[to] = S.y[from]; S.z[to] = S.z[from];
+  S.dx[to] = S.dx[from]; S.dy[to] = S.dy[from]; S.dz[to] = S.dz[from];
+  S.e[to] = S.e[from]; S.wt[to] = S.wt[from]; S.score[to] = 0.f;
+  S.hid[to] = S.hid[from]; S.nsc[to] = S.nsc[from]; S.mat[to] = S.mat[from]; S.flags[to] = S.flags[from];
+}
+
+// Weight window of one photon (wave-uniform call, one photon per lane): centre
+// wt = clamp(cw / exp(f), w_lo, w_hi), window [wt / 2, 2 wt].  Above it the
+// photon becomes n = min(floor(w / wt), split_max) photons of weight w / n (the
+// parent and n - 1 copies in fresh pool slots, each with its own Philox lane
+// tag); below it, it survives with probability w / wt at weight wt.  Both keep
+// the expected weight.  When the pool is full a photon stays unsplit.
+struct Window {
+  bool keep;
+  uint32_t ncopy, first;   // copies in slots [first, first + ncopy)
+};
+__device__ Window weight_window(const Launch& L, const Importance& I, bool valid, int i, float f) {
+  PState S = L.S;
+  Window o{valid, 0u, 0u};
+  float w = 0.f;
+  if (valid) {
+    w = S.wt[i];
+    const float wt = fminf(fmaxf(I.cw * expf(-fminf(fmaxf(f, -80.f), 80.f)), I.w_lo), I.w_hi);
+    if (w > 2.f * wt) {
+      o.ncopy = (uint32_t)fminf(w / wt, (float)I.split_max) - 1u;   // w / wt > 2: at least 2 photons
+    } else if (w < 0.5f * wt) {
+      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
+      const float u = rng.next();
+      S.ctr[i] = rng.ctr;
+      if (u < w / wt) S.wt[i] = wt;
+      else o.keep = false;
+    }
+  }
+  wave_count(valid && !o.keep, L.stats + S_ROUL);
+  uint32_t tot = 0;
+  const uint32_t off = wave_excl_scan(o.ncopy, tot);
+  uint32_t slot0 = 0;
+  if (tot > 0) {
+    if (__lane_id() == 0) slot0 = atomicAdd(L.cnt + C_POOL, tot);
+    slot0 = __shfl(slot0, 0);
+    if (slot0 + tot > I.pool_cap) o.ncopy = 0;    // pool full: the whole wave stays unsplit
+  }
+  o.first = slot0 + off;
+  if (o.ncopy > 0) {
+    const float wn = w / (float)(o.ncopy + 1u);
+    S.wt[i] = wn;
+    for (uint32_t c = 0; c < o.ncopy; ++c) {
+      const int q = (int)(o.first + c);
+      copy_particle(S, i, q);
+      S.wt[q] = wn;
+      const uint32_t tg = mix32(S.tag[i], S.ctr[i], c);
+      S.tag[q] = tg;
+      PhiloxRng rc(L.seed, L.hist0 + (uint64_t)S.hid[i], tg, 0u);
+      S.tau[q] = -logf(rc.next());
+      S.ctr[q] = rc.ctr;
+    }
+  }
+  wave_sum_u64((unsigned long long)o.ncopy, L.stats + S_SPLIT);
+  return o;
+}
+
+// Features of the listed photon of this lane (zeros past the list end).
+__device__ __forceinline__ void lane_features(const Launch& L, const Importance& I, const int* in, uint32_t j,
+                                              uint32_t n, float x[kMlpIn]) {
+  if (j < n) {
+    PState S = L.S;
+    const int i = in[j];
+    const float p[3] = {S.x[i], S.y[i], S.z[i]}, d[3] = {S.dx[i], S.dy[i], S.dz[i]};
+    mlp_features(I.frame, p, d, S.e[i], x);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kMlpIn; ++k) x[k] = 0.f;
+  }
+}
+
+// Weight window at a flight start: reads the active list `in` (photons about to
+// fly, post-collision directions), writes survivors and copies to `out`.  Each
+// wave evaluates the map for its own 64 photons.
+__global__ __launch_bounds__(kBlock) void k_importance(Launch L, Importance I, const int* in, const uint32_t* cnt_in,
+                                                       int* out, uint32_t* cnt_out) {
+  MlpWave M;
+  mlp_wave_load(I.wb, I.w, M);
+  const uint32_t n = *cnt_in;
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t base = (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n;
+       base += waves * kWave) {
+    const uint32_t j = base + __lane_id();
+    float x[kMlpIn];
+    lane_features(L, I, in, j, n, x);
+    const float f = mlp_wave_eval(M, x);
+    const bool valid = j < n;
+    const int i = valid ? in[j] : 0;
+    const Window o = weight_window(L, I, valid, i, f);
+    const uint32_t mine = (o.keep ? 1u : 0u) + o.ncopy;
+    uint32_t tot = 0;
+    const uint32_t off = wave_excl_scan(mine, tot);
+    uint32_t o0 = 0;
+    if (tot > 0) {
+      if (__lane_id() == 0) o0 = atomicAdd(cnt_out, tot);
+      o0 = __shfl(o0, 0);
+    }
+    uint32_t k = o0 + off;
+    if (o.keep) out[k++] = i;
+    for (uint32_t c = 0; c < o.ncopy; ++c) out[k++] = (int)(o.first + c);
+  }
+}
+
+// One interaction at (material m, energy E) of a photon with incoming direction
+// d: the type, then the outcome, with the draws of the analog interaction
+// kernels.  Returns whether a photon leaves the collision (E_out, d_out); dep is
+// the energy it leaves behind.
+__device__ __forceinline__ bool collide_outcome(const Tables& T, int m, float E, const EPos& ep, const float d[3],
+                                                int fluor, PhiloxRng& rng, int* err, float& E_out, float d_out[3],
+                                                float& dep, int& type) {
+  type = sample_type(T, m, E, ep, rng.next());
+  if (type == 0) {
+    int el;
+    float eline = 0.f;
+    const bool emit = fluor && fluor_sample(T, m, E, ep, rng, el, eline);
+    dep = emit ? E - eline : E;
+    if (!emit) return false;
+    const float u1 = rng.next();
+    const float u2 = rng.next();
+    isotropic_dir(d_out, u1, u2);
+    E_out = eline;
+    return true;
+  }
+  float cs, sn;
+  int el, trials;
+  if (type == 1) {
+    const float eps = compton_sample(T, m, E, ep, rng, kTrialCap, cs, sn, el, trials);
+    E_out = E * eps;
+    dep = E - E_out;
+  } else {
+    bool qerr;
+    cs = rayleigh_sample(T, m, E, ep, rng, kTrialCap, sn, el, trials, qerr);
+    if (qerr) raise_err(err, CC_DEV_Q_RANGE);
+    E_out = E;
+    dep = 0.f;
+  }
+  if (trials == 0) raise_err(err, CC_DEV_ITER_CAP);
+  d_out[0] = d[0]; d_out[1] = d[1]; d_out[2] = d[2];
+  rotate_dir(d_out, cs, sn, rng.next());
+  return true;
+}
+
+// Collisions of a learned-importance run, fused: split, sample, window.
+//
+// A photon of weight w colliding at x is split into n = min(gain w I(x, u) / cw,
+// split_max) copies of weight w / n, where I(x, u) = exp(f) is the map for a
+// photon at x heading straight for the scoring region (u = direction to its
+// centre): a copy that scatters towards the region then arrives at that
+// direction's window centre.  Each copy draws its own interaction (type,
+// energy, direction) from its own Philox lane, deposits its share, and its
+// outgoing photon is judged by the map at once: below the window it survives
+// roulette with probability w' / wt at weight wt.  Only survivors take a pool
+// slot; killed copies never leave registers.  Copies 1 .. n-1 use fresh lanes
+// and fresh slots; copy 0 runs last on the parent's lane and slot.  Before each
+// round of copies a wave checks that the pool has room for one slot per lane of
+// every wave of the grid (each wave has at most one such round between its
+// check and its allocation, so the pool cannot overflow); when it has not, the
+// copies not yet drawn are merged into copy 0 (its weight becomes the sum of
+// theirs), which keeps the split unbiased and needs no new slot.
+__global__ __launch_bounds__(kBlock) void k_scatter_split(Launch L, Importance I, const int* in,
+                                                          const uint32_t* cnt_in, int* act_next, uint32_t* cnt_next) {
+  MlpWave M;
+  mlp_wave_load(I.wb, I.w, M);
+  const uint32_t n_in = *cnt_in;
+  PState S = L.S;
+  const Tables& T = L.T;
+  const int lane = __lane_id();
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t base = (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n_in;
+       base += waves * kWave) {
+    const uint32_t j = base + lane;
+    const bool valid = j < n_in;
+    int i = 0, m = 0, nsc = 0;
+    uint32_t hid = 0, tag = 0, ctr0 = 0;
+    float p[3] = {I.frame.c[0], I.frame.c[1], I.frame.c[2] - 1.f}, d[3] = {0.f, 0.f, 1.f}, E = 50.f, w = 0.f;
+    if (valid) {
+      i = in[j];
+      p[0] = S.x[i]; p[1] = S.y[i]; p[2] = S.z[i];
+      d[0] = S.dx[i]; d[1] = S.dy[i]; d[2] = S.dz[i];
+      E = S.e[i]; w = S.wt[i]; m = S.mat[i]; nsc = S.nsc[i];
+      hid = S.hid[i]; tag = S.tag[i]; ctr0 = S.ctr[i];
+    }
+    const EPos ep = energy_pos(T, E);
+    // split count from the map for the direction to the region centre
+    float x[kMlpIn];
+    {
+      float u[3] = {I.frame.c[0] - p[0], I.frame.c[1] - p[1], I.frame.c[2] - p[2]};
+      const float ul = sqrtf(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      if (ul > 0.f) { u[0] /= ul; u[1] /= ul; u[2] /= ul; } else { u[0] = 0.f; u[1] = 0.f; u[2] = 1.f; }
+      mlp_features(I.frame, p, u, E, x);
+    }
+    const float fr = mlp_wave_eval(M, x);
+    int ncp = 1;
+    if (valid) {
+      const float want = I.split_gain * w * expf(fminf(fmaxf(fr, -80.f), 80.f)) / I.cw;
+      if (want >= 2.f) ncp = (int)fminf(want, (float)I.split_max);
+    }
+    int most = ncp;
+    for (int off = kWave / 2; off > 0; off >>= 1) most = max(most, __shfl_xor(most, off));
+    const float wc = w / (float)ncp;
+    int drawn = 0;   // copies 1 .. drawn already sampled
+    for (int c = 1; c < most; ++c) {
+      const bool act = valid && c < ncp;
+      if (__ballot(act) == 0ull) break;
+      int ok = 0;
+      if (lane == 0) {
+        const uint32_t used = atomicAdd(L.cnt + C_POOL, 0u);
+        ok = (uint64_t)used + (uint64_t)kWave * waves <= I.pool_cap;
+      }
+      if (!__shfl(ok, 0)) break;   // pool nearly full: the rest merge into copy 0
+      bool alive = false;
+      float eo = E, dout[3] = {0.f, 0.f, 1.f}, dep = 0.f, wo = wc;
+      uint32_t tg = 0, ctr = 0;
+      int type = -1;
+      if (act) {
+        tg = mix32(tag, ctr0, (uint32_t)c);
+        PhiloxRng rng(L.seed, L.hist0 + (uint64_t)hid, tg, 0u);
+        alive = collide_outcome(T, m, E, ep, d, L.fluor, rng, L.err, eo, dout, dep, type);
+        ctr = rng.ctr;
+        drawn = c;
+      }
+      wave_add_by_key(act, m, (double)wc * (double)dep, L.edep);
+      wave_count(act && type == 0 && !alive, L.stats + S_ABS);
+      wave_count(act && type == 0 && alive, L.stats + S_FLUOR);
+      wave_count(act, L.stats + S_SCAT);
+      // window of the outgoing photon (roulette only)
+      if (alive) mlp_features(I.frame, p, dout, eo, x);
+      else for (int k = 0; k < kMlpIn; ++k) x[k] = 0.f;
+      const float fo = mlp_wave_eval(M, x);
+      float tau = 0.f;
+      bool killed = false;
+      if (alive) {
+        PhiloxRng rng(L.seed, L.hist0 + (uint64_t)hid, tg, ctr);
+        const float wt = fminf(fmaxf(I.cw * expf(-fminf(fmaxf(fo, -80.f), 80.f)), I.w_lo), I.w_hi);
+        if (wo < 0.5f * wt) {
+          if (rng.next() < wo / wt) wo = wt;
+          else { alive = false; killed = true; }
+        }
+        tau = -logf(rng.next());
+        ctr = rng.ctr;
+      }
+      wave_count(killed, L.stats + S_ROUL);
+      // slots for the survivors
+      uint32_t tot = 0;
+      const uint32_t off = wave_excl_scan(alive ? 1u : 0u, tot);
+      uint32_t s0 = 0;
+      if (lane == 0 && tot > 0) s0 = atomicAdd(L.cnt + C_POOL, tot);
+      s0 = __shfl(s0, 0);
+      if (alive) {
+        const int q = (int)(s0 + off);
+        S.x[q] = p[0]; S.y[q] = p[1]; S.z[q] = p[2];
+        S.dx[q] = dout[0]; S.dy[q] = dout[1]; S.dz[q] = dout[2];
+        S.e[q] = eo; S.wt[q] = wo; S.tau[q] = tau; S.score[q] = 0.f;
+        S.ctr[q] = ctr; S.hid[q] = hid; S.tag[q] = tg; S.nsc[q] = nsc + 1; S.mat[q] = m; S.flags[q] = 0;
+      }
+      wave_sum_u64(alive ? 1ull : 0ull, L.stats + S_SPLIT);
+      const int k = wave_append(alive, cnt_next);
+      if (alive) act_next[k] = (int)(s0 + off);
+    }
+    // copy 0: the parent's lane and slot, carrying the weight of every copy not drawn
+    {
+      const float w0 = valid ? w * (float)(ncp - drawn) / (float)ncp : 0.f;
+      bool alive = false;
+      float eo = E, dout[3] = {0.f, 0.f, 1.f}, dep = 0.f, wo = w0, tau = 0.f;
+      int type = -1;
+      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)hid, tag, ctr0);
+      if (valid) alive = collide_outcome(T, m, E, ep, d, L.fluor, rng, L.err, eo, dout, dep, type);
+      wave_add_by_key(valid, m, (double)w0 * (double)dep, L.edep);
+      wave_count(valid && type == 0 && !alive, L.stats + S_ABS);
+      wave_count(valid && type == 0 && alive, L.stats + S_FLUOR);
+      wave_count(valid, L.stats + S_SCAT);
+      if (alive) mlp_features(I.frame, p, dout, eo, x);
+      else for (int k = 0; k < kMlpIn; ++k) x[k] = 0.f;
+      const float fo = mlp_wave_eval(M, x);
+      bool killed = false;
+      if (alive) {
+        const float wt = fminf(fmaxf(I.cw * expf(-fminf(fmaxf(fo, -80.f), 80.f)), I.w_lo), I.w_hi);
+        if (wo < 0.5f * wt) {
+          if (rng.next() < wo / wt) wo = wt;
+          else { alive = false; killed = true; }
+        }
+        tau = -logf(rng.next());
+      }
+      wave_count(killed, L.stats + S_ROUL);
+      if (valid) {
+        S.ctr[i] = rng.ctr;
+        S.nsc[i] = nsc + 1;
+        if (alive) {
+          S.dx[i] = dout[0]; S.dy[i] = dout[1]; S.dz[i] = dout[2];
+          S.e[i] = eo; S.wt[i] = wo; S.tau[i] = tau;
+        } else if (type == 0 && !killed) {
+          S.flags[i] |= F_ABSORBED;
+        }
+      }
+      const int k = wave_append(alive, cnt_next);
+      if (alive) act_next[k] = i;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_record(Launch L, const int* act, const uint32_t* cnt, uint32_t batch) {
+  const uint32_t n = *cnt;
+  PState S = L.S;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t j = base + threadIdx.x;
+    int i = 0;
+    bool take = false;
+    if (j < n) {
+      i = act[j];
+      take = record_take(L.Rec, S.hid[i] ^ (uint32_t)L.hist0, S.ctr[i], batch);
+    }
+    const int k = wave_append(take, L.Rec.cnt);
+    if (take) {
+      const float p[3] = {S.x[i], S.y[i], S.z[i]}, d[3] = {S.dx[i], S.dy[i], S.dz[i]};
+      put_record(L.Rec, 0, (uint32_t)k, p, d, S.e[i], S.score[i], i, batch);
+    }
+  }
+}
+
+__global__ void k_record_mark(uint32_t* rc) {
+  if (threadIdx.x == 0) {
+    rc[1] = rc[0];
+    rc[3] = rc[2];
+  }
+}
+
+// target = score of the slot at the end of the batch - score when recorded
+__global__ __launch_bounds__(kBlock) void k_record_targets(Launch L) {
+  const uint32_t half = L.Rec.cap / 2;
+  for (int kind = 0; kind < 2; ++kind) {
+    const uint32_t room = kind == 0 ? half : L.Rec.cap - half;
+    const uint32_t first = min(L.Rec.cnt[2 * kind + 1], room);
+    const uint32_t last = min(L.Rec.cnt[2 * kind], room);
+    float* base = L.Rec.rec + (size_t)(kind == 0 ? 0 : half) * 12;
+    for (uint32_t k = first + blockIdx.x * blockDim.x + threadIdx.x; k < last; k += gridDim.x * blockDim.x) {
+      float* r = base + (size_t)k * 12;
+      const int i = __float_as_int(r[9]);
+      r[8] = fmaxf(L.S.score[i] - r[7], 0.f);
+    }
+  }
+}
+
+// Batch end of the region tally: sum and sum of squares of the per-history
+// scores into acc[0], acc[1]; the per-history buffer is cleared for the next batch.
+__global__ __launch_bounds__(kBlock) void k_region_end(double* hscore, uint32_t n, double* acc) {
+  double s = 0.0, q = 0.0;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const double v = hscore[k];
+    if (v != 0.0) {
+      s += v;
+      q += v * v;
+      hscore[k] = 0.0;
+    }
+  }
+  s = wave_sum_f64(s);
+  q = wave_sum_f64(q);
+  if (__lane_id() == 0 && (s != 0.0 || q != 0.0)) {
+    atomic_add_f64(acc, s);
+    atomic_add_f64(acc + 1, q);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MLP inference and training kernels
+// ---------------------------------------------------------------------------
+// f of rows of 16 features with the wave-level inference of the transport kernels.
+__global__ __launch_bounds__(kBlock) void k_mlp_forward(const float* w, const uint16_t* wb, int n, const float* feat,
+                                                        float* out) {
+  MlpWave M;
+  mlp_wave_load(wb, w, M);
+  const int waves = gridDim.x * (blockDim.x / kWave);
+  for (int base = (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n; base += waves * kWave) {
+    const int row = base + __lane_id();
+    float x[kMlpIn];
+#pragma unroll
+    for (int k = 0; k < kMlpIn; ++k) x[k] = row < n ? feat[(size_t)row * kMlpIn + k] : 0.f;
+    const float f = mlp_wave_eval(M, x);
+    if (row < n) out[row] = f;
+  }
+}
+
+// C (global fp32) += A B with A, B bf16 in LDS (atomic, for gradient accumulation).
+__device__ __forceinline__ void lds_gemm_bf16_atomic(int M, int N, int K, const uint16_t* A, int a_si, int a_sk,
+                                                     const uint16_t* B, int b_sk, int b_sj, float* C, int ldc) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const int tiles_n = N / 16;
+  const int tiles = (M / 16) * tiles_n;
+  const int r = lane & 15;
+  const int g = lane >> 4;
+  for (int t = wave; t < tiles; t += nwaves) {
+    const int tm = t / tiles_n, tn = t % tiles_n;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int kb = 0; kb < K; kb += 16) {
+      bf16x4 a, b;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int k = kb + 4 * g + e;
+        a[e] = (short)A[(tm * 16 + r) * a_si + k * a_sk];
+        b[e] = (short)B[k * b_sk + (tn * 16 + r) * b_sj];
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) atomicAdd(C + (tm * 16 + 4 * g + e) * ldc + tn * 16 + r, acc[e]);
+  }
+}
+
+// Gradient of the Poisson deviance  sum_r [exp(f_r) - t_r f_r] * gscale  over rows, added
+// into grad (fp32, kMlpParams), with target t = yscale y (power 1) or (yscale y)^2
+// (power 2: exp(f) then estimates the second moment).  Rows come either from `feat`
+// (16 floats each, with y) or from the record buffer through the affine permutation
+// idx = (a k + b) mod n_rec.
+constexpr int kGR = 32;
+__global__ __launch_bounds__(kBlock) void k_mlp_grad(const float* w, const uint16_t* wb, int n, const float* feat,
+                                                     const float* yv, const float* rec, uint32_t n_rec,
+                                                     uint64_t pa, uint64_t pb, uint32_t k0, MlpFrame frame,
+                                                     float yscale, int power, float gscale, float* grad,
+                                                     double* loss) {
+  __shared__ uint16_t sX[kGR * kMlpIn], sW1[kMlpIn * kMlpH], sW2[kMlpH * kMlpH], sH1[kGR * kMlpH];
+  __shared__ uint16_t sD2[kGR * kMlpH], sD1[kGR * kMlpH];
+  __shared__ float sZ1[kGR * kMlpH], sZ2[kGR * kMlpH], sOut[kGR], sG[kGR], sY[kGR];
+  MlpTile tile{sX, sW1, sW2, sZ1, sH1, sZ2};
+  mlp_stage_weights(wb, wb + kMlpIn * kMlpH, tile);
+  const float* b2 = w + kOffB2;
+  const float* w3 = w + kOffW3;
+  double lsum = 0.0;
+  for (int base = blockIdx.x * kGR; base < n; base += gridDim.x * kGR) {
+    const int r = threadIdx.x;
+    if (r < kGR) {
+      const int row = base + r;
+      float x[kMlpIn];
+      float y = 0.f;
+      if (row < n) {
+        if (feat) {
+          for (int k = 0; k < kMlpIn; ++k) x[k] = feat[(size_t)row * kMlpIn + k];
+          y = yv[row];
+        } else {
+          const uint64_t q = (pa * (uint64_t)(k0 + row) + pb) % (uint64_t)n_rec;
+          const float* rr = rec + q * 12;
+          const float p[3] = {rr[0], rr[1], rr[2]}, d[3] = {rr[3], rr[4], rr[5]};
+          mlp_features(frame, p, d, rr[6], x);
+          y = rr[8];
+        }
+      } else {
+        for (int k = 0; k < kMlpIn; ++k) x[k] = 0.f;
+      }
+      for (int k = 0; k < kMlpIn; ++k) sX[r * kMlpIn + k] = f2bf(x[k]);
+      const float t = y * yscale;
+      sY[r] = power == 2 ? t * t : t;
+    }
+    mlp_forward_tile(kGR, tile, b2, w3, w[kOffB3], sOut);
+    if (r < kGR) {
+      const bool ok = base + r < n;
+      const float f = sOut[r];
+      const float ef = expf(fminf(f, 80.f));
+      sG[r] = ok ? (ef - sY[r]) * gscale : 0.f;
+      if (ok) lsum += (double)(ef - sY[r] * f) * gscale;
+    }
+    __syncthreads();
+    // output layer: dw3 = sum_r g relu(Z2), db3 = sum_r g; D2 = g w3 1[Z2 > 0]; db2 = sum_r D2
+    if (threadIdx.x < kMlpH) {
+      const int jj = threadIdx.x;
+      float s3 = 0.f, s2 = 0.f;
+      for (int rr = 0; rr < kGR; ++rr) {
+        const float z = sZ2[rr * kMlpH + jj];
+        s3 += sG[rr] * fmaxf(z, 0.f);
+        const float d2 = z > 0.f ? sG[rr] * w3[jj] : 0.f;
+        s2 += d2;
+      }
+      atomicAdd(grad + kOffW3 + jj, s3);
+      atomicAdd(grad + kOffB2 + jj, s2);
+    }
+    if (threadIdx.x == 64) {
+      float s = 0.f;
+      for (int rr = 0; rr < kGR; ++rr) s += sG[rr];
+      atomicAdd(grad + kOffB3, s);
+    }
+    for (int k = threadIdx.x; k < kGR * kMlpH; k += blockDim.x) {
+      const int rr = k / kMlpH, jj = k % kMlpH;
+      sD2[k] = f2bf(sZ2[k] > 0.f ? sG[rr] * w3[jj] : 0.f);
+    }
+    __syncthreads();
+    // dW2 += H1^T D2  (M = 64 hidden1, N = 64 hidden2, K = rows)
+    lds_gemm_bf16_atomic(kMlpH, kMlpH, kGR, sH1, 1, kMlpH, sD2, kMlpH, 1, grad + kOffW2, kMlpH);
+    // D1 = (D2 W2^T) 1[Z1 > 0]  (M = rows, N = 64 hidden1, K = 64 hidden2), into sZ1
+    lds_gemm_bf16(kGR, kMlpH, kMlpH, sD2, kMlpH, 1, sW2, 1, kMlpH, sZ1, kMlpH, false);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kGR * kMlpH; k += blockDim.x) sD1[k] = f2bf(bf2f(sH1[k]) > 0.f ? sZ1[k] : 0.f);
+    __syncthreads();
+    // dW1 += X^T D1  (M = 16 inputs, N = 64, K = rows)
+    lds_gemm_bf16_atomic(kMlpIn, kMlpH, kGR, sX, 1, kMlpIn, sD1, kMlpH, 1, grad + kOffW1, kMlpH);
+    __syncthreads();
+  }
+  // block loss
+  __shared__ double sL[kBlock];
+  sL[threadIdx.x] = lsum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < (int)blockDim.x; ++k) t += sL[k];
+    atomic_add_f64(loss, t);
+  }
+}
+
+// Adam on the fp32 master weights; refreshes the bf16 copies and clears grad.
+// Rows 8..15 of W1 (padding columns of the input) stay at zero.
+__global__ void k_adam(float* w, float* m, float* v, float* grad, uint16_t* wb, float lr, float b1, float b2,
+                       float c1, float c2, float eps) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < kMlpParams; k += gridDim.x * blockDim.x) {
+    const bool pad = k < kOffW2 && (k / kMlpH) >= 8;
+    float g = pad ? 0.f : grad[k];
+    float mk = b1 * m[k] + (1.f - b1) * g;
+    float vk = b2 * v[k] + (1.f - b2) * g * g;
+    m[k] = mk;
+    v[k] = vk;
+    float wk = pad ? 0.f : w[k] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+    w[k] = wk;
+    grad[k] = 0.f;
+    if (k < kOffB2) wb[k] = f2bf(wk);
+  }
+}
+
+__global__ void k_fill_f32(float* p, size_t n, float v) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = v;
+}
+
+// sum of the targets of the first n records
+__global__ __launch_bounds__(kBlock) void k_record_sum(const float* rec, uint32_t n, double* acc) {
+  double s = 0.0, c = 0.0;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const float y = rec[(size_t)k * 12 + 8];
+    s += y;
+    c += y > 0.f ? 1.0 : 0.0;
+  }
+  s = wave_sum_f64(s);
+  c = wave_sum_f64(c);
+  if (__lane_id() == 0) {
+    atomic_add_f64(acc, s);
+    atomic_add_f64(acc + 1, c);
+  }
+}
+
+__global__ void k_mlp_refresh(const float* w, uint16_t* wb) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < kOffB2; k += gridDim.x * blockDim.x) wb[k] = f2bf(w[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -674,6 +1465,13 @@ int dalloc(T** p, size_t n) {
   return CC_OK;
 }
 
+int grid_blocks(size_t n) {
+  size_t b = (n + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  if (b > 8192) b = 8192;
+  return (int)b;
+}
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -696,7 +1494,9 @@ struct Ctx {
   // particle pool
   size_t cap = 0;
   PState S{};
-  int *act0 = nullptr, *act1 = nullptr, *qp = nullptr, *qc = nullptr, *qr = nullptr;
+  int *act0 = nullptr, *act1 = nullptr, *act2 = nullptr, *qp = nullptr, *qc = nullptr, *qr = nullptr;
+  int* coll = nullptr;         // collision list (weight windows at collisions)
+  double* hscore = nullptr;    // per-history region score of a batch
   uint32_t* cnt = nullptr;
   unsigned long long* stats = nullptr;
   double* edep = nullptr;
@@ -706,15 +1506,43 @@ struct Ctx {
   float* spec = nullptr;
   int spec_cap = 0;
   std::vector<hipEvent_t> ev;
+  // learned importance
+  Region reg{};
+  float* rec = nullptr;
+  uint32_t* rcnt = nullptr;
+  uint32_t rcap = 0;
+  float rkeep = 1.f;
+  // two maps: 0 = states about to fly, 1 = states about to interact
+  bool have_mlp[2] = {false, false};
+  float *mw[2] = {}, *mm[2] = {}, *mv[2] = {}, *mgrad[2] = {};
+  uint16_t* mwb[2] = {};
+  uint32_t adam_t[2] = {0, 0};
+  MlpFrame frame{};
+  bool have_imp = false, have_bias = false;
+  cc_importance_cfg icfg{};
+  float cw = 1.f;
+  float *bcdf = nullptr, *bwt = nullptr;
 };
 
+// Feature frame without a scoring region: world-box centre, half the largest extent.
+void set_default_frame(Ctx* c) {
+  float h = 0.f;
+  for (int k = 0; k < 3; ++k) {
+    c->frame.c[k] = 0.5f * (c->W.lo[k] + c->W.hi[k]);
+    h = std::max(h, 0.5f * (c->W.hi[k] - c->W.lo[k]));
+  }
+  c->frame.inv_l = h > 0.f ? 1.0f / h : 1.0f;
+}
+
 void free_pool(Ctx* c) {
-  void* ps[] = {c->S.x, c->S.y, c->S.z, c->S.dx, c->S.dy, c->S.dz, c->S.e, c->S.tau, c->S.wt,
-                c->S.ctr, c->S.nsc, c->S.mat, c->S.flags, c->act0, c->act1, c->qp, c->qc, c->qr};
+  void* ps[] = {c->S.x, c->S.y, c->S.z, c->S.dx, c->S.dy, c->S.dz, c->S.e, c->S.tau, c->S.wt, c->S.score,
+                c->S.ctr, c->S.hid, c->S.tag, c->S.nsc, c->S.mat, c->S.flags, c->act0, c->act1, c->act2,
+                c->qp, c->qc, c->qr, c->hscore, c->coll};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   c->S = PState{};
-  c->act0 = c->act1 = c->qp = c->qc = c->qr = nullptr;
+  c->act0 = c->act1 = c->act2 = c->qp = c->qc = c->qr = c->coll = nullptr;
+  c->hscore = nullptr;
   c->cap = 0;
 }
 
@@ -727,8 +1555,12 @@ int ensure_pool(Ctx* c, size_t n) {
       (r = dalloc(&c->S.e, n)) || (r = dalloc(&c->S.tau, n)) || (r = dalloc(&c->S.wt, n)) ||
       (r = dalloc(&c->S.ctr, n)) || (r = dalloc(&c->S.nsc, n)) || (r = dalloc(&c->S.mat, n)) ||
       (r = dalloc(&c->S.flags, n)) || (r = dalloc(&c->act0, n)) || (r = dalloc(&c->act1, n)) ||
-      (r = dalloc(&c->qp, n)) || (r = dalloc(&c->qc, n)) || (r = dalloc(&c->qr, n)))
+      (r = dalloc(&c->qp, n)) || (r = dalloc(&c->qc, n)) || (r = dalloc(&c->qr, n)) ||
+      (r = dalloc(&c->S.score, n)) || (r = dalloc(&c->S.hid, n)) || (r = dalloc(&c->S.tag, n)) ||
+      (r = dalloc(&c->act2, n)) || (r = dalloc(&c->hscore, n)) || (r = dalloc(&c->coll, n)))
     return r;
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(n)), dim3(kBlock), 0, c->stream, c->hscore, n, 0.0);
+  HIP_TRY(hipGetLastError());
   c->cap = n;
   return CC_OK;
 }
@@ -754,17 +1586,13 @@ Launch make_launch(Ctx* c, uint64_t seed, uint64_t hist0, int grid_on, int fluor
   L.cnt = c->cnt;
   L.stats = c->stats;
   L.edep = c->edep;
+  L.hscore = c->hscore;
   L.err = c->err;
   L.seed = seed;
   L.hist0 = hist0;
+  L.R = Region{};
+  L.Rec = Records{c->rec, c->rcnt, c->rcap, c->rkeep, 0};
   return L;
-}
-
-int grid_blocks(size_t n) {
-  size_t b = (n + kBlock - 1) / kBlock;
-  if (b < 1) b = 1;
-  if (b > 8192) b = 8192;
-  return (int)b;
 }
 
 struct Timer {
@@ -809,13 +1637,26 @@ struct Timer {
 
 enum { K_SOURCE = 0, K_FLIGHT = 1, K_PHOTO = 2, K_COMPTON = 3, K_RAYLEIGH = 4, K_END = 5, K_OTHER = 6 };
 
+// Per-round extras of the learned-importance runs.
+struct RoundOpts {
+  bool record = false;     // contribution records at every round start
+  bool imp = false;        // weight windows (flight starts and / or collisions)
+  bool post = false;       // window at flight starts (map of states about to fly)
+  bool pre = false;        // window at collisions (map of states about to interact)
+  Importance I{};          // flight-start map
+  Importance Ic{};         // collision map
+  uint32_t batch_index = 0;
+};
+
 // One batch of photons already in the pool [0, n): rounds until no photon is active.
 // Rounds are issued in chunks without a host wait in between; the active count
-// of every round is copied to pinned memory and inspected once per chunk.  The
-// active population never grows, so the count at the start of a chunk bounds
-// every grid of the chunk; rounds that find an empty list exit at once.
+// of every round is copied to pinned memory and inspected once per chunk.  In
+// analog runs the active population never grows, so the count at the start of
+// a chunk bounds every grid of the chunk; with splitting it can grow, and the
+// grids (all kernels stride over their lists) are sized for four times that count.
+// Rounds that find an empty list exit at once.
 constexpr int kRoundChunk = 6;
-int run_rounds(Ctx* c, const Launch& L, uint32_t n, Timer& tm, uint32_t* rounds_out) {
+int run_rounds(Ctx* c, const Launch& L, uint32_t n, Timer& tm, uint32_t* rounds_out, const RoundOpts& o) {
   int cur = C_ACT_A, nxt = C_ACT_B;
   int* act_cur = c->act0;
   int* act_nxt = c->act1;
@@ -823,13 +1664,40 @@ int run_rounds(Ctx* c, const Launch& L, uint32_t n, Timer& tm, uint32_t* rounds_
   uint32_t rounds = 0;
   while (active > 0) {
     if (rounds >= (uint32_t)kMaxRounds) return fail(CC_ERR_DEVICE, "round cap hit");
-    const int nb = grid_blocks(active);
+    const size_t bound = o.imp ? std::min<size_t>(c->cap, 4 * (size_t)active) : active;
+    const int ib = (int)std::min<size_t>((bound + 63) / 64, 4096);
+    const int nb = grid_blocks(bound);
     for (int r = 0; r < kRoundChunk; ++r) {
       hipLaunchKernelGGL(k_round_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, nxt);
+      const int* fl = act_cur;
+      const uint32_t* fc = c->cnt + cur;
+      if (o.record) {
+        tm.begin(K_OTHER);
+        hipLaunchKernelGGL(k_record, dim3(nb), dim3(kBlock), 0, c->stream, L, act_cur, c->cnt + cur, o.batch_index);
+        tm.end();
+      }
+      if (o.post && (!o.pre || (rounds == 0 && r == 0))) {   // with collision splitting: source photons only
+        tm.begin(K_OTHER);
+        hipLaunchKernelGGL(k_importance, dim3(ib), dim3(kBlock), 0, c->stream, L, o.I, act_cur, c->cnt + cur,
+                           c->act2, c->cnt + C_IMP);
+        tm.end();
+        fl = c->act2;
+        fc = c->cnt + C_IMP;
+      }
       tm.begin(K_FLIGHT);
-      hipLaunchKernelGGL(k_flight, dim3(nb), dim3(kBlock), 0, c->stream, L, act_cur, c->cnt + cur, c->qp, c->qc,
-                         c->qr);
+      if (o.record)
+        hipLaunchKernelGGL(k_flight<true>, dim3(nb), dim3(kBlock), 0, c->stream, L, fl, fc, c->qp, c->qc, c->qr,
+                           (int*)nullptr);
+      else
+        hipLaunchKernelGGL(k_flight<false>, dim3(nb), dim3(kBlock), 0, c->stream, L, fl, fc, c->qp, c->qc, c->qr,
+                           o.pre ? c->coll : (int*)nullptr);
       tm.end();
+      if (o.pre) {
+        tm.begin(K_OTHER);
+        hipLaunchKernelGGL(k_scatter_split, dim3(ib), dim3(kBlock), 0, c->stream, L, o.I, (const int*)c->coll,
+                           c->cnt + C_COLL, act_nxt, c->cnt + nxt);
+        tm.end();
+      } else {
       tm.begin(K_PHOTO);
       hipLaunchKernelGGL(k_photo, dim3(nb), dim3(kBlock), 0, c->stream, L, c->qp, c->cnt + C_QP, act_nxt,
                          c->cnt + nxt);
@@ -842,6 +1710,7 @@ int run_rounds(Ctx* c, const Launch& L, uint32_t n, Timer& tm, uint32_t* rounds_
       hipLaunchKernelGGL(k_rayleigh, dim3(nb), dim3(kBlock), 0, c->stream, L, c->qr, c->cnt + C_QR, act_nxt,
                          c->cnt + nxt);
       tm.end();
+      }
       HIP_TRY(hipMemcpyAsync(c->h_cnt + r, c->cnt + nxt, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
       std::swap(cur, nxt);
       std::swap(act_cur, act_nxt);
@@ -860,7 +1729,7 @@ int run_rounds(Ctx* c, const Launch& L, uint32_t n, Timer& tm, uint32_t* rounds_
 
 int read_batch(Ctx* c, uint32_t nb, cc_run_out* out, int* err_bits) {
   unsigned long long st[S_NSTAT];
-  double ed[CC_MAX_MAT];
+  double ed[CC_MAX_MAT + 2];
   int e = 0;
   HIP_TRY(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(ed, c->edep, sizeof(ed), hipMemcpyDeviceToHost, c->stream));
@@ -873,6 +1742,11 @@ int read_batch(Ctx* c, uint32_t nb, cc_run_out* out, int* err_bits) {
   out->n_events += st[S_EVT];
   out->n_flights += st[S_FLIGHT];
   out->n_voxel_visits += st[S_VOX];
+  out->n_split += st[S_SPLIT];
+  out->n_roulette += st[S_ROUL];
+  out->region_sum += ed[CC_MAX_MAT];
+  out->region_sum2 += ed[CC_MAX_MAT + 1];
+  if (nb > 0) out->region_batch_sum2 += ed[CC_MAX_MAT] * ed[CC_MAX_MAT] / (double)nb;
   for (int m = 0; m < CC_MAX_MAT; ++m) {
     out->edep_keV[m] += ed[m];
     if (nb > 0) out->edep_sum2[m] += ed[m] * ed[m] / (double)nb;
@@ -949,6 +1823,16 @@ int make_src_impl(Ctx* c, const cc_source* s, Src* d) {
   return CC_OK;
 }
 
+// held records of one kind: device offset (in records) and count
+int records_held(Ctx* c, int kind, uint32_t* base, uint32_t* n) {
+  uint32_t cnt[4];
+  HIP_TRY(hipMemcpy(cnt, c->rcnt, sizeof(cnt), hipMemcpyDeviceToHost));
+  const uint32_t half = c->rcap / 2;
+  *base = kind == 0 ? 0u : half;
+  *n = std::min(cnt[2 * kind], kind == 0 ? half : c->rcap - half);
+  return CC_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -974,7 +1858,7 @@ int cc_ctx_create(int32_t device, void** out) {
   c->device = device;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   int r;
-  if ((r = dalloc(&c->cnt, C_NCNT)) || (r = dalloc(&c->stats, S_NSTAT)) || (r = dalloc(&c->edep, CC_MAX_MAT)) ||
+  if ((r = dalloc(&c->cnt, C_NCNT)) || (r = dalloc(&c->stats, S_NSTAT)) || (r = dalloc(&c->edep, CC_MAX_MAT + 2)) ||
       (r = dalloc(&c->err, 1))) {
     delete c;
     return r;
@@ -992,7 +1876,9 @@ int cc_ctx_destroy(void* ctx) {
   (void)hipStreamSynchronize(c->stream);
   free_pool(c);
   for (void* p : c->tbufs) (void)hipFree(p);
-  void* ps[] = {c->prims, c->tk, c->th, c->qk, c->qh, c->hits, c->G.dk, c->G.dh, c->cnt, c->stats, c->edep, c->err, c->spec};
+  void* ps[] = {c->prims, c->tk, c->th, c->qk, c->qh, c->hits, c->G.dk, c->G.dh, c->cnt, c->stats, c->edep, c->err,
+                c->spec, c->rec, c->rcnt, c->mw[0], c->mm[0], c->mv[0], c->mgrad[0], c->mwb[0],
+                c->mw[1], c->mm[1], c->mv[1], c->mgrad[1], c->mwb[1], c->bcdf, c->bwt};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
@@ -1069,6 +1955,7 @@ int cc_upload_scene(void* ctx, const cc_prim* prims, int32_t n_prim, const cc_wo
   if (n_prim) HIP_TRY(hipMemcpy(c->prims, prims, sizeof(Prim) * n_prim, hipMemcpyHostToDevice));
   std::memcpy(&c->W, world, sizeof(World));
   c->W.n_prim = n_prim;
+  if (!c->reg.on) set_default_frame(c);
   c->have_scene = true;
   return CC_OK;
 }
@@ -1175,45 +2062,96 @@ int cc_run(void* ctx, const cc_source* src, uint64_t n_hist, uint64_t seed, uint
   const bool grid_on = (flags & CC_RUN_GRID) != 0;
   if (grid_on && !c->have_grid) return fail(CC_ERR_STATE, "CC_RUN_GRID without a bound grid");
   if (batch == 0) return fail(CC_ERR_ARG, "batch must be > 0");
+  const bool region = (flags & CC_RUN_REGION) != 0;
+  const bool record = (flags & CC_RUN_RECORD) != 0;
+  const bool imp = (flags & CC_RUN_IMPORTANCE) != 0;
+  const bool bias = (flags & CC_RUN_SOURCE_BIAS) != 0;
+  if (region && !c->reg.on) return fail(CC_ERR_STATE, "CC_RUN_REGION without a region (cc_set_region)");
+  if (record && (!region || !c->rec)) return fail(CC_ERR_STATE, "CC_RUN_RECORD needs a region and records");
+  if (record && (imp || bias)) return fail(CC_ERR_ARG, "contribution records come from analog runs only");
+  if ((imp || bias) && !c->have_imp) return fail(CC_ERR_STATE, "importance not set (cc_set_importance)");
+  if (bias && !c->have_bias) return fail(CC_ERR_STATE, "source cells not built (cc_importance_cfg.source_bias)");
   HIP_TRY(hipSetDevice(c->device));
   std::memset(out, 0, sizeof(*out));
   Src s;
   int r = make_src_impl(c, src, &s);
   if (r) return r;
-  r = ensure_pool(c, batch);
+  size_t pool = batch;
+  if (imp) {
+    const double f = std::max(1.0, (double)c->icfg.pool_factor);
+    pool = (size_t)std::min(f * (double)batch, (double)(1u << 31));
+  }
+  r = ensure_pool(c, pool);
   if (r) return r;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->err, (size_t)1, 0);
+  if (region)
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(c->cap)), dim3(kBlock), 0, c->stream, c->hscore, c->cap, 0.0);
   Timer tm{c, (flags & CC_RUN_TIMING) != 0};
   const bool track = c->track && (flags & CC_RUN_TRACK_UNCERTAINTY);
+  SrcBias B{c->bcdf, c->bwt, bias ? 1 : 0};
+  RoundOpts ro;
+  ro.record = record;
+  ro.imp = imp;
+  ro.post = imp && c->icfg.use_post;
+  ro.pre = imp && c->icfg.use_pre;
+  for (int net = 0; net < 2; ++net) {
+    Importance& I = net == 0 ? ro.I : ro.Ic;
+    I.w = c->mw[net];
+    I.wb = c->mwb[net];
+    I.frame = c->frame;
+    I.cw = c->cw;
+    I.w_lo = c->icfg.w_lo;
+    I.w_hi = c->icfg.w_hi;
+    I.split_gain = c->icfg.split_gain > 0.f ? c->icfg.split_gain : 1.0f;
+    I.split_max = c->icfg.split_max;
+    I.pool_cap = (uint32_t)c->cap;
+    I.on = 1;
+  }
   int err_bits = 0;
   uint64_t remaining = n_hist, h0 = first;
+  uint32_t bidx = 0;
   while (remaining > 0) {
     const uint32_t nb = (uint32_t)(remaining < batch ? remaining : batch);
     Launch L = make_launch(c, seed, h0, grid_on ? 1 : 0, (flags & CC_RUN_FLUORESCENCE) ? 1 : 0);
+    if (region) L.R = c->reg;
+    L.Rec.on = record ? 1 : 0;
+    L.rec_batch = bidx;
+    ro.batch_index = bidx++;
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, c->stats, c->edep, (int)C_ACT_A, nb);
+    if (record) hipLaunchKernelGGL(k_record_mark, dim3(1), dim3(64), 0, c->stream, c->rcnt);
     tm.begin(K_SOURCE);
-    hipLaunchKernelGGL(k_source, dim3((nb + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, L, s, (int)nb, c->act0);
+    hipLaunchKernelGGL(k_source, dim3((nb + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, L, s, B, (int)nb,
+                       c->act0);
     tm.end();
     HIP_TRY(hipGetLastError());
     uint32_t rounds = 0;
-    r = run_rounds(c, L, nb, tm, &rounds);
+    r = run_rounds(c, L, nb, tm, &rounds, ro);
     if (r) return r;
     if (rounds > out->rounds_max) out->rounds_max = rounds;
+    if (record) hipLaunchKernelGGL(k_record_targets, dim3(1024), dim3(kBlock), 0, c->stream, L);
+    if (region)
+      hipLaunchKernelGGL(k_region_end, dim3(grid_blocks(nb)), dim3(kBlock), 0, c->stream, c->hscore, nb,
+                         c->edep + CC_MAX_MAT);
     if (grid_on) {
       tm.begin(K_END);
       hipLaunchKernelGGL(k_end_batch, dim3(grid_blocks(c->nvox)), dim3(kBlock), 0, c->stream, c->G.dk, c->G.dh, c->tk,
                          c->th, c->qk, c->qh, c->hits, c->nvox, 1.0 / (double)nb, track ? 1 : 0);
       tm.end();
-      HIP_TRY(hipGetLastError());
       if (track) c->g_batches += 1;
       c->g_hist += nb;
     }
+    HIP_TRY(hipGetLastError());
     r = read_batch(c, nb, out, &err_bits);
     if (r) return r;
     tm.collect(out->kernel_ms, out->kernel_launches);
     out->n_batches += 1;
     remaining -= nb;
     h0 += nb;
+  }
+  if (c->rcnt) {
+    uint32_t b0, n0, b1, n1;
+    if ((r = records_held(c, 0, &b0, &n0)) || (r = records_held(c, 1, &b1, &n1))) return r;
+    out->n_records = (uint64_t)n0 + n1;
   }
   out->n_histories = n_hist;
   out->error_bits = (uint32_t)err_bits;
@@ -1250,7 +2188,7 @@ int cc_transport(void* ctx, int64_t n, double* pos, double* dir, double* energy,
   hipLaunchKernelGGL(k_init_given, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, L, (int)n, c->act0);
   HIP_TRY(hipGetLastError());
   uint32_t rounds = 0;
-  r = run_rounds(c, L, (uint32_t)n, tm, &rounds);
+  r = run_rounds(c, L, (uint32_t)n, tm, &rounds, RoundOpts{});
   if (r) return r;
   out->rounds_max = rounds;
   if (grid_on) {
@@ -1498,6 +2436,334 @@ int cc_test_source(void* ctx, const cc_source* src, int64_t n, uint64_t seed, ui
   HIP_TRY(hipDeviceSynchronize());
   if ((r = to_host(pos, dp, 3 * (size_t)n)) || (r = to_host(dir, dd, 3 * (size_t)n))) return r;
   return to_host(energy, de, (size_t)n);
+}
+
+// ---- learned importance ----------------------------------------------------
+int cc_set_region(void* ctx, const double* center, double radius_cm, double scale_cm) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c || !center) return fail(CC_ERR_ARG, "null argument");
+  if (!(radius_cm > 0.0)) {
+    c->reg = Region{};
+    set_default_frame(c);
+    return CC_OK;
+  }
+  if (!(scale_cm > 0.0)) scale_cm = 100.0;
+  for (int k = 0; k < 3; ++k) c->frame.c[k] = (float)center[k];
+  c->frame.inv_l = (float)(1.0 / scale_cm);
+  Region R{};
+  for (int k = 0; k < 3; ++k) R.c[k] = (float)center[k];
+  R.r = (float)radius_cm;
+  R.inv_vol = (float)(1.0 / (4.0 / 3.0 * 3.14159265358979323846 * radius_cm * radius_cm * radius_cm));
+  R.on = 1;
+  c->reg = R;
+  return CC_OK;
+}
+
+int cc_records_alloc(void* ctx, uint32_t capacity, float keep_prob) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return fail(CC_ERR_ARG, "null ctx");
+  if (capacity < 2 || capacity > (1u << 28)) return fail(CC_ERR_ARG, "record capacity %u out of range", capacity);
+  if (!(keep_prob > 0.f && keep_prob <= 1.f)) return fail(CC_ERR_ARG, "keep probability must be in (0, 1]");
+  HIP_TRY(hipSetDevice(c->device));
+  if (c->rec) (void)hipFree(c->rec);
+  c->rec = nullptr;
+  c->rcap = 0;
+  int r = dalloc(&c->rec, (size_t)capacity * CC_RECORD_FLOATS);
+  if (r) return r;
+  if (!c->rcnt && (r = dalloc(&c->rcnt, 4))) return r;
+  HIP_TRY(hipMemset(c->rcnt, 0, 4 * sizeof(uint32_t)));
+  c->rcap = capacity;
+  c->rkeep = keep_prob;
+  return CC_OK;
+}
+
+int cc_records_clear(void* ctx) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c || !c->rcnt) return fail(CC_ERR_STATE, "no records allocated");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemset(c->rcnt, 0, 4 * sizeof(uint32_t)));
+  return CC_OK;
+}
+
+int cc_records_read(void* ctx, int32_t kind, uint32_t first, uint32_t n, float* out, uint32_t* n_held) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c || !c->rcnt) return fail(CC_ERR_STATE, "no records allocated");
+  if (kind != 0 && kind != 1) return fail(CC_ERR_ARG, "record kind %d (0 flight starts, 1 collisions)", kind);
+  HIP_TRY(hipSetDevice(c->device));
+  uint32_t base = 0, nr = 0;
+  int r = records_held(c, kind, &base, &nr);
+  if (r) return r;
+  if (n_held) *n_held = nr;
+  if (n == 0) return CC_OK;
+  if (!out) return fail(CC_ERR_ARG, "null output");
+  if ((uint64_t)first + n > nr) return fail(CC_ERR_ARG, "records [%u, %u) beyond the %u held", first, first + n, nr);
+  HIP_TRY(hipMemcpy(out, c->rec + (size_t)(base + first) * CC_RECORD_FLOATS,
+                    (size_t)n * CC_RECORD_FLOATS * sizeof(float), hipMemcpyDeviceToHost));
+  return CC_OK;
+}
+
+static int check_net(Ctx* c, int32_t net, bool need_params) {
+  if (!c) return fail(CC_ERR_ARG, "null ctx");
+  if (net != 0 && net != 1) return fail(CC_ERR_ARG, "map %d (0 flight starts, 1 collisions)", net);
+  if (need_params && !c->have_mlp[net]) return fail(CC_ERR_STATE, "no parameters set for map %d", net);
+  return CC_OK;
+}
+
+int cc_mlp_set_params(void* ctx, int32_t net, const float* w) {
+  Ctx* c = (Ctx*)ctx;
+  int r = check_net(c, net, false);
+  if (r) return r;
+  if (!w) return fail(CC_ERR_ARG, "null argument");
+  for (int k = 0; k < kMlpParams; ++k)
+    if (!std::isfinite(w[k])) return fail(CC_ERR_ARG, "parameter %d is not finite", k);
+  HIP_TRY(hipSetDevice(c->device));
+  if (!c->mw[net] && ((r = dalloc(&c->mw[net], kMlpParams)) || (r = dalloc(&c->mm[net], kMlpParams)) ||
+                      (r = dalloc(&c->mv[net], kMlpParams)) || (r = dalloc(&c->mgrad[net], kMlpParams)) ||
+                      (r = dalloc(&c->mwb[net], kOffB2))))
+    return r;
+  std::vector<float> h(w, w + kMlpParams);
+  for (int k = kMlpIn / 2 * kMlpH; k < kOffW2; ++k) h[k] = 0.f;   // rows 8..15 of W1
+  HIP_TRY(hipMemcpy(c->mw[net], h.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+  float* zs[] = {c->mm[net], c->mv[net], c->mgrad[net]};
+  for (float* z : zs) HIP_TRY(hipMemset(z, 0, kMlpParams * sizeof(float)));
+  hipLaunchKernelGGL(k_mlp_refresh, dim3(32), dim3(kBlock), 0, c->stream, c->mw[net], c->mwb[net]);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->adam_t[net] = 0;
+  c->have_mlp[net] = true;
+  return CC_OK;
+}
+
+int cc_mlp_get_params(void* ctx, int32_t net, float* w) {
+  Ctx* c = (Ctx*)ctx;
+  int r = check_net(c, net, true);
+  if (r) return r;
+  if (!w) return fail(CC_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemcpy(w, c->mw[net], kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+  return CC_OK;
+}
+
+int cc_mlp_features(void* ctx, int64_t n, const float* pos, const float* dir, const float* energy, float* feat) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c || (n > 0 && (!pos || !dir || !energy || !feat))) return fail(CC_ERR_ARG, "null argument");
+  if (!c->have_scene) return fail(CC_ERR_STATE, "no scene (the default feature frame is the world box)");
+  if (n < 0 || n > (1ll << 28)) return fail(CC_ERR_ARG, "n out of range");
+  if (n == 0) return CC_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  DevBufs Bf;
+  float *dp, *dd, *de, *df;
+  int r;
+  if ((r = Bf.in(pos, 3 * (size_t)n, &dp)) || (r = Bf.in(dir, 3 * (size_t)n, &dd)) ||
+      (r = Bf.in(energy, (size_t)n, &de)) || (r = Bf.out((size_t)n * kMlpIn, &df)))
+    return r;
+  hipLaunchKernelGGL(k_features, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->frame, (int)n, dp,
+                     dd, de, df);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return to_host(feat, df, (size_t)n * kMlpIn);
+}
+
+int cc_mlp_forward(void* ctx, int32_t net, int64_t n, const float* feat, float* out) {
+  Ctx* c = (Ctx*)ctx;
+  int r = check_net(c, net, true);
+  if (r) return r;
+  if (n > 0 && (!feat || !out)) return fail(CC_ERR_ARG, "null argument");
+  if (n < 0 || n > (1ll << 28)) return fail(CC_ERR_ARG, "n out of range");
+  if (n == 0) return CC_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  DevBufs Bf;
+  float *df, *dout;
+  if ((r = Bf.in(feat, (size_t)n * kMlpIn, &df)) || (r = Bf.out((size_t)n, &dout))) return r;
+  const int nb = (int)std::min<int64_t>((n + 63) / 64, 2048);
+  hipLaunchKernelGGL(k_mlp_forward, dim3(nb), dim3(kBlock), 0, c->stream, c->mw[net], c->mwb[net], (int)n, df, dout);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return to_host(out, dout, (size_t)n);
+}
+
+int cc_mlp_grad(void* ctx, int32_t net, int64_t n, const float* feat, const float* y, float yscale, float* grad,
+                double* loss) {
+  Ctx* c = (Ctx*)ctx;
+  int r = check_net(c, net, true);
+  if (r) return r;
+  if (!grad || (n > 0 && (!feat || !y))) return fail(CC_ERR_ARG, "null argument");
+  if (n < 0 || n > (1ll << 28)) return fail(CC_ERR_ARG, "n out of range");
+  HIP_TRY(hipSetDevice(c->device));
+  DevBufs Bf;
+  float *df = nullptr, *dy = nullptr, *dg;
+  double* dl;
+  if ((r = Bf.out(kMlpParams, &dg)) || (r = Bf.out(1, &dl))) return r;
+  if (n > 0 && ((r = Bf.in(feat, (size_t)n * kMlpIn, &df)) || (r = Bf.in(y, (size_t)n, &dy)))) return r;
+  hipLaunchKernelGGL(k_fill_f32, dim3(32), dim3(kBlock), 0, c->stream, dg, (size_t)kMlpParams, 0.f);
+  hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, c->stream, dl, (size_t)1, 0.0);
+  if (n > 0) {
+    const int nb = (int)std::min<int64_t>((n + kGR - 1) / kGR, 2048);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(kBlock), 0, c->stream, c->mw[net], c->mwb[net], (int)n, df, dy,
+                       (const float*)nullptr, 0u, (uint64_t)0, (uint64_t)0, 0u, c->frame, yscale, 1, 1.0f, dg, dl);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if ((r = to_host(grad, dg, kMlpParams))) return r;
+  if (loss && (r = to_host(loss, dl, 1))) return r;
+  return CC_OK;
+}
+
+int cc_mlp_train(void* ctx, int32_t net, const cc_train_cfg* cfg, cc_train_out* out, double* loss_curve) {
+  Ctx* c = (Ctx*)ctx;
+  int r = check_net(c, net, true);
+  if (r) return r;
+  if (!cfg || !out) return fail(CC_ERR_ARG, "null argument");
+  if (!c->rcnt) return fail(CC_ERR_STATE, "no records allocated");
+  if (cfg->steps == 0 || cfg->batch == 0 || cfg->batch > (1u << 20)) return fail(CC_ERR_ARG, "steps / batch out of range");
+  if (cfg->target_power != 0 && cfg->target_power != 1 && cfg->target_power != 2)
+    return fail(CC_ERR_ARG, "target power must be 1 or 2");
+  if (!(cfg->lr > 0.f) || !(cfg->beta1 >= 0.f && cfg->beta1 < 1.f) || !(cfg->beta2 >= 0.f && cfg->beta2 < 1.f) ||
+      !(cfg->eps > 0.f))
+    return fail(CC_ERR_ARG, "Adam hyper-parameters out of range");
+  HIP_TRY(hipSetDevice(c->device));
+  std::memset(out, 0, sizeof(*out));
+  uint32_t base = 0, nr = 0;
+  if ((r = records_held(c, net, &base, &nr))) return r;
+  if (nr == 0) return fail(CC_ERR_STATE, "no contribution records of kind %d held", net);
+  const float* rec = c->rec + (size_t)base * CC_RECORD_FLOATS;
+  DevBufs Bf;
+  double *acc, *dl;
+  if ((r = Bf.out(2, &acc)) || (r = Bf.out(cfg->steps, &dl))) return r;
+  hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, c->stream, acc, (size_t)2, 0.0);
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(cfg->steps)), dim3(kBlock), 0, c->stream, dl, (size_t)cfg->steps, 0.0);
+  hipLaunchKernelGGL(k_record_sum, dim3(grid_blocks(nr)), dim3(kBlock), 0, c->stream, rec, nr, acc);
+  double h_acc[2];
+  HIP_TRY(hipMemcpyAsync(h_acc, acc, sizeof(h_acc), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  out->n_records = nr;
+  out->n_positive = (uint32_t)h_acc[1];
+  float ys = cfg->yscale;
+  if (!(ys > 0.f)) {
+    if (!(h_acc[0] > 0.0)) return fail(CC_ERR_STATE, "every record has a zero target: the region is never reached");
+    ys = (float)((double)nr / h_acc[0]);
+  }
+  out->yscale = ys;
+  const int power = cfg->target_power == 2 ? 2 : 1;
+  // minibatch rows: k -> (pa k + pb) mod nr, pa coprime with nr (a permutation of every nr consecutive k)
+  uint64_t z = cfg->seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  auto next64 = [&z]() {
+    z += 0x9E3779B97F4A7C15ull;
+    uint64_t x = z;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+  };
+  uint64_t pa = (nr > 1) ? (next64() % nr) | 1ull : 1ull;
+  auto gcd = [](uint64_t a, uint64_t b) { while (b) { uint64_t t = a % b; a = b; b = t; } return a; };
+  while (nr > 1 && gcd(pa, nr) != 1) pa = (pa + 2) % nr | 1ull;
+  const uint64_t pb = next64() % nr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, c->stream));
+  const int gb = (int)std::min<uint32_t>((cfg->batch + kGR - 1) / kGR, 4096);
+  const float gscale = 1.0f / (float)cfg->batch;
+  double b1t = std::pow((double)cfg->beta1, (double)c->adam_t[net]);
+  double b2t = std::pow((double)cfg->beta2, (double)c->adam_t[net]);
+  for (uint32_t t = 0; t < cfg->steps; ++t) {
+    const uint32_t k0 = (uint32_t)(((uint64_t)t * cfg->batch) % nr);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(gb), dim3(kBlock), 0, c->stream, (const float*)c->mw[net],
+                       (const uint16_t*)c->mwb[net], (int)cfg->batch, (const float*)nullptr, (const float*)nullptr,
+                       rec, nr, pa, pb, k0, c->frame, ys, power, gscale, c->mgrad[net], dl + t);
+    ++c->adam_t[net];
+    b1t *= cfg->beta1;
+    b2t *= cfg->beta2;
+    float lr = cfg->lr;
+    if (cfg->lr_end > 0.f && cfg->steps > 1)   // cosine decay from lr to lr_end
+      lr = (float)(cfg->lr_end + 0.5 * (cfg->lr - cfg->lr_end) * (1.0 + std::cos(M_PI * t / (cfg->steps - 1))));
+    hipLaunchKernelGGL(k_adam, dim3((kMlpParams + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->mw[net],
+                       c->mm[net], c->mv[net], c->mgrad[net], c->mwb[net], lr, cfg->beta1, cfg->beta2,
+                       (float)(1.0 - b1t), (float)(1.0 - b2t), cfg->eps);
+  }
+  HIP_TRY(hipEventRecord(e1, c->stream));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  out->ms = ms;
+  std::vector<double> curve(cfg->steps);
+  if ((r = to_host(curve.data(), dl, cfg->steps))) return r;
+  const uint32_t m = std::max<uint32_t>(1, cfg->steps / 100);
+  for (uint32_t t = 0; t < m; ++t) {
+    out->loss_first += curve[t] / m;
+    out->loss_last += curve[cfg->steps - m + t] / m;
+  }
+  if (loss_curve) std::memcpy(loss_curve, curve.data(), cfg->steps * sizeof(double));
+  if (!std::isfinite(out->loss_last)) return fail(CC_ERR_DEVICE, "training diverged (loss %g)", out->loss_last);
+  return CC_OK;
+}
+
+int cc_set_importance(void* ctx, const cc_importance_cfg* cfg, const cc_source* src, float* cw_out,
+                      float* cell_prob) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c || !cfg) return fail(CC_ERR_ARG, "null argument");
+  if (!c->have_tables || !c->have_scene) return fail(CC_ERR_STATE, "tables and scene must be uploaded");
+  if (!(cfg->w_lo > 0.f) || !(cfg->w_hi >= cfg->w_lo) || cfg->split_max < 1 || cfg->split_max > 1024 ||
+      !(cfg->pool_factor >= 1.f && cfg->pool_factor <= 64.f))
+    return fail(CC_ERR_ARG, "weight-window parameters out of range");
+  if (!c->have_mlp[0]) return fail(CC_ERR_STATE, "no parameters set for map 0 (flight starts)");
+  const bool need_cells = cfg->source_bias || !(cfg->cw > 0.f);
+  if (need_cells && !src) return fail(CC_ERR_ARG, "the source is needed for the source cells");
+  if (cfg->source_bias && !(cfg->bias_mix >= 0.f && cfg->bias_mix <= 1.f))
+    return fail(CC_ERR_ARG, "bias_mix must be in [0, 1]");
+  HIP_TRY(hipSetDevice(c->device));
+  float cw = cfg->cw;
+  if (need_cells) {
+    const int per = std::max(1, std::min(cfg->samples_per_cell, 4096));
+    Src s;
+    int r = make_src_impl(c, src, &s);
+    if (r) return r;
+    const size_t n = (size_t)kSrcCells * per;
+    DevBufs Bf;
+    float *df, *dout;
+    if ((r = Bf.out(n * kMlpIn, &df)) || (r = Bf.out(n, &dout))) return r;
+    hipLaunchKernelGGL(k_source_cells, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, s, cfg->seed, per,
+                       c->frame, df);
+    const int nb = (int)std::min<size_t>((n + 63) / 64, 2048);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(nb), dim3(kBlock), 0, c->stream, c->mw[0], c->mwb[0], (int)n, df, dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<float> f(n);
+    if ((r = to_host(f.data(), dout, n))) return r;
+    std::vector<double> imp(kSrcCells, 0.0);
+    double tot = 0.0;
+    for (int k = 0; k < kSrcCells; ++k) {
+      for (int j = 0; j < per; ++j) imp[k] += std::exp((double)std::min(std::max(f[(size_t)k * per + j], -80.f), 80.f));
+      imp[k] /= per;
+      tot += imp[k];
+    }
+    if (!(tot > 0.0) || !std::isfinite(tot)) return fail(CC_ERR_STATE, "source importance is not finite");
+    if (!(cw > 0.f)) cw = (float)(tot / kSrcCells);
+    if (cfg->source_bias) {
+      std::vector<float> cdf(kSrcCells), wt(kSrcCells);
+      double acc = 0.0;
+      for (int k = 0; k < kSrcCells; ++k) {
+        const double p = (1.0 - cfg->bias_mix) * imp[k] / tot + cfg->bias_mix / kSrcCells;
+        acc += p;
+        cdf[k] = (float)acc;
+        wt[k] = (float)((1.0 / kSrcCells) / p);
+        if (cell_prob) cell_prob[k] = (float)p;
+      }
+      cdf[kSrcCells - 1] = 1.0f;
+      if (!c->bcdf && ((r = dalloc(&c->bcdf, kSrcCells)) || (r = dalloc(&c->bwt, kSrcCells)))) return r;
+      HIP_TRY(hipMemcpy(c->bcdf, cdf.data(), kSrcCells * sizeof(float), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(c->bwt, wt.data(), kSrcCells * sizeof(float), hipMemcpyHostToDevice));
+    }
+  }
+  c->icfg = *cfg;
+  c->cw = cw;
+  c->have_imp = true;
+  c->have_bias = cfg->source_bias != 0;
+  if (cw_out) *cw_out = cw;
+  return CC_OK;
 }
 
 }  // extern "C"

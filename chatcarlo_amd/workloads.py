@@ -103,6 +103,11 @@ def angio_room(kvp=90.0, filtration_mm_al=3.0):
     return raw, None
 
 
+def angio_region():
+    """Operator H*(10) scoring region of angio_room: a 20 cm air sphere at 60 cm lateral, 150 cm high."""
+    return np.array([60.0, 0.0, 150.0]), 10.0
+
+
 def batched_scenes(n_scenes=64, master_seed=2026, max_prims=40):
     """64 random scenes: 1-40 box/cylinder/sphere primitives (water, bone, lead, concrete, pmma;
     sizes 0.05-50 cm) in a 300 cm air room; kVp U(50, 150) Kramers + 2.5 mm Al; histories

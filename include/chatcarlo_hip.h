@@ -105,7 +105,11 @@ enum {
   CC_RUN_FLUORESCENCE = 1,     /* physics.fluorescence (chatcarlo/transport.py:388) */
   CC_RUN_GRID = 2,             /* score the bound grid */
   CC_RUN_TRACK_UNCERTAINTY = 4, /* batch moments (chatcarlo/tally.py:59-94) */
-  CC_RUN_TIMING = 8             /* per-kernel HIP-event timing into cc_run_out.kernel_ms */
+  CC_RUN_TIMING = 8,            /* per-kernel HIP-event timing into cc_run_out.kernel_ms */
+  CC_RUN_REGION = 16,           /* H*(10) track-length score in the region set by cc_set_region */
+  CC_RUN_RECORD = 32,           /* contribution records (analog runs only, needs CC_RUN_REGION) */
+  CC_RUN_IMPORTANCE = 64,       /* weight windows from the learned importance map */
+  CC_RUN_SOURCE_BIAS = 128      /* source cells drawn by learned importance (cc_set_importance) */
 };
 
 typedef struct {
@@ -118,7 +122,55 @@ typedef struct {
   double edep_sum2[CC_MAX_MAT]; /* sum over batches of S_b^2 / n_b */
   double kernel_ms[8];          /* source, flight, photo, compton, rayleigh, end_batch, other, spare */
   uint64_t kernel_launches[8];  /* launches timed per kernel kind */
+  /* region tally (CC_RUN_REGION), pSv per history summed: per-history sum and sum of squares,
+   * and the batch form sum over batches of S_b^2 / n_b */
+  double region_sum, region_sum2, region_batch_sum2;
+  uint64_t n_split;             /* copies made by splitting */
+  uint64_t n_roulette;          /* photons ended by Russian roulette */
+  uint64_t n_records;           /* contribution records held after the run */
 } cc_run_out;
+
+/* Learned importance map: f(x) = MLP 7 -> 64 -> 64 -> 1 (BF16 MFMA, FP32 accumulate).
+ * Flat parameters: W1 [16][64] (rows 0..6 features, row 7 bias, rows 8..15 zero),
+ * W2 [64][64], b2 [64], w3 [64], b3. */
+enum { CC_MLP_IN = 16, CC_MLP_HIDDEN = 64, CC_MLP_PARAMS = 16 * 64 + 64 * 64 + 64 + 64 + 1 };
+enum { CC_RECORD_FLOATS = 12 };   /* x y z dx dy dz E score_before target slot batch kind */
+/* Two maps of the same shape: 0 for photons about to fly (flight starts, the source),
+ * 1 for photons about to interact (collision point, incoming direction and energy). */
+enum { CC_MAP_FLIGHT = 0, CC_MAP_COLLISION = 1 };
+
+typedef struct {
+  uint32_t steps, batch;        /* Adam steps, records per step */
+  float lr, beta1, beta2, eps;
+  float yscale;                 /* target scale; <= 0: n_records / sum of targets */
+  float lr_end;                 /* cosine decay of the step size from lr to lr_end */
+  uint64_t seed;                /* minibatch permutation */
+  uint32_t target_power;        /* 1: fit the mean of the target; 2: its second moment */
+  uint32_t pad;
+} cc_train_cfg;
+
+typedef struct {
+  double loss_first, loss_last; /* mean Poisson deviance of the first and last 1% of steps */
+  double ms;                    /* device time of the training loop */
+  float yscale;
+  uint32_t n_records, n_positive;
+  uint32_t pad;
+} cc_train_out;
+
+typedef struct {
+  float cw;                     /* target weight = clamp(cw / exp(f), w_lo, w_hi); <= 0: mean source importance */
+  float w_lo, w_hi;
+  int32_t split_max;            /* at most this many copies per split */
+  float pool_factor;            /* particle pool = batch * pool_factor */
+  int32_t source_bias;          /* 1: build the source cells for CC_RUN_SOURCE_BIAS */
+  float bias_mix;               /* defensive share of the analog cell probabilities */
+  int32_t samples_per_cell;     /* source photons per cell for the mean importance */
+  int32_t use_post;             /* window at flight starts (map 0; source photons only with use_pre) */
+  int32_t use_pre;              /* split at collisions before the outcome is drawn, roulette the outcomes */
+  float split_gain;             /* collision split count gain (<= 0: 1) */
+  uint32_t pad;
+  uint64_t seed;
+} cc_importance_cfg;
 
 const char* cc_last_error(void);
 int cc_version(int32_t* major, int32_t* minor);
@@ -172,6 +224,31 @@ int cc_test_isotropic(int64_t n, const float* u1, const float* u2, float* out);
 int cc_test_dda(void* ctx, int64_t n, const float* o, const float* d, const float* len,
                 const double* w_kerma, const double* w_h10);
 int cc_test_philox(int64_t n, const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4);
+/* Learned importance (no counterpart in the reference; its transport is analog only).
+ * Scoring region: sphere of H*(10) track-length scoring; radius <= 0 clears it.  The MLP
+ * feature frame is centred on the region with length scale scale_cm (<= 0: 100 cm). */
+int cc_set_region(void* ctx, const double* center, double radius_cm, double scale_cm);
+/* Contribution records of analog runs (half the capacity per kind) and the probability
+ * that a photon is recorded at a flight start or a collision. */
+int cc_records_alloc(void* ctx, uint32_t capacity, float keep_prob);
+int cc_records_read(void* ctx, int32_t kind, uint32_t first, uint32_t n, float* out, uint32_t* n_held);
+int cc_records_clear(void* ctx);
+int cc_mlp_set_params(void* ctx, int32_t map, const float* w);   /* CC_MLP_PARAMS floats; resets Adam */
+int cc_mlp_get_params(void* ctx, int32_t map, float* w);
+/* Feature rows (n x 16) of states in the frame of the scoring region (without one: the
+ * world-box centre, half the largest extent). */
+int cc_mlp_features(void* ctx, int64_t n, const float* pos, const float* dir, const float* energy, float* feat);
+int cc_mlp_forward(void* ctx, int32_t map, int64_t n, const float* feat, float* out);
+/* Gradient of sum_r exp(f_r) - yscale y_r f_r over the rows, and that loss. */
+int cc_mlp_grad(void* ctx, int32_t map, int64_t n, const float* feat, const float* y, float yscale, float* grad,
+                double* loss);
+/* Adam on the held records of the map's kind; loss_curve (steps doubles) may be NULL. */
+int cc_mlp_train(void* ctx, int32_t map, const cc_train_cfg* cfg, cc_train_out* out, double* loss_curve);
+/* Weight-window parameters and, when cfg->source_bias or cfg->cw <= 0, the source cells of src.
+ * cell_prob (512 floats) may be NULL; cw_out receives the cw in use. */
+int cc_set_importance(void* ctx, const cc_importance_cfg* cfg, const cc_source* src, float* cw_out,
+                      float* cell_prob);
+
 /* Source sampling for histories [first, first + n): pos, dir (n x 3), energy (n). */
 int cc_test_source(void* ctx, const cc_source* src, int64_t n, uint64_t seed, uint64_t first_history,
                    float* pos, float* dir, float* energy);

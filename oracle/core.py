@@ -19,6 +19,8 @@ Pinning of each function (tests/test_oracle_*.py):
   with the reference's stratified estimator (chatcarlo/tally.py:248-293) in
   expectation.
 * batch statistics: brute force at batch size 1 (reference tests/test_uncertainty.py:22-50).
+* sphere_chord: hand-computed chords and Cauchy's mean chord 4r/3 for isotropic
+  uniform lines (tests/test_oracle_mlp.py).
 """
 from __future__ import annotations
 
@@ -560,3 +562,21 @@ def relative_error(T, Q, m, n):
     mean = T / n
     with np.errstate(divide="ignore", invalid="ignore"):
         return np.where(mean != 0, sem / mean, np.nan)
+
+
+# ---------------------------------------------------------------------------
+# region tally (the operator sphere of the learned-importance workload; the
+# reference scores H*(10) only on its voxel grid, chatcarlo/tally.py:248-293)
+# ---------------------------------------------------------------------------
+def sphere_chord(o, d, length, c, r):
+    """Length of each segment [o, o + d length] inside the sphere (c, r)."""
+    o = np.atleast_2d(np.asarray(o, dtype=float))
+    d = np.atleast_2d(np.asarray(d, dtype=float))
+    oc = o - np.asarray(c, dtype=float)
+    b = np.einsum("ij,ij->i", oc, d)
+    cc = np.einsum("ij,ij->i", oc, oc) - r * r
+    disc = b * b - cc
+    sq = np.sqrt(np.maximum(disc, 0.0))
+    lo = np.maximum(-b - sq, 0.0)
+    hi = np.minimum(-b + sq, np.asarray(length, dtype=float))
+    return np.where((disc > 0) & (hi > lo), hi - lo, 0.0)

@@ -59,8 +59,11 @@ def _draw_gen(rng):
 
 
 def transport_photons(pos, dirv, energy, geom, pack, rng, grid=None, fluorescence=True,
-                      exact_tally=True, tally_rng=None, k_trials=100000):
-    """chatcarlo/transport.py:98-233 on the oracle's geometry and pack."""
+                      exact_tally=True, tally_rng=None, k_trials=100000, region=None):
+    """chatcarlo/transport.py:98-233 on the oracle's geometry and pack.
+
+    region = (centre, radius): adds the per-history H*(10) track-length score
+    in that sphere, h*(10)(E) x chord / volume, as result["region_score"]."""
     if grid is not None and not exact_tally and tally_rng is None:
         tally_rng = rng.spawn(1)[0]
     n = pos.shape[0]
@@ -73,6 +76,10 @@ def transport_photons(pos, dirv, energy, geom, pack, rng, grid=None, fluorescenc
     n_fl = 0
     n_mat = len(pack.materials)
     rounds = 0
+    rscore = np.zeros(n)
+    if region is not None:
+        rc, rr = np.asarray(region[0], dtype=float), float(region[1])
+        inv_vol = 1.0 / (4.0 / 3.0 * np.pi * rr ** 3)
     while np.any(alive):
         rounds += 1
         idx = np.where(alive)[0]
@@ -86,6 +93,8 @@ def transport_photons(pos, dirv, energy, geom, pack, rng, grid=None, fluorescenc
         inter = tau[idx] < tau_b
         ds = np.where(inter, tau[idx] / mu_safe, t_b)
         ends = o + d * ds[:, None]
+        if region is not None:
+            rscore[idx] += core.h10_coeff(pack, e) * core.sphere_chord(o, d, ds, rc, rr) * inv_vol
         if grid is not None:
             wk = e * xs["mu_en"]
             wh = core.h10_coeff(pack, e)
@@ -150,24 +159,28 @@ def transport_photons(pos, dirv, energy, geom, pack, rng, grid=None, fluorescenc
             tau[ridx] = -np.log(rng.random(len(ridx)))
             n_scatter[ridx] += 1
     return dict(n_scatter=n_scatter, absorbed=absorbed, escaped=escaped, final_energy=energy,
-                energy_deposited=edep, n_fluorescence=n_fl, rounds=rounds, n_mat=n_mat)
+                energy_deposited=edep, n_fluorescence=n_fl, rounds=rounds, n_mat=n_mat,
+                region_score=rscore)
 
 
 def run(src, geom, pack, spec_e, spec_cdf, n_histories, seed, batch_size=200_000, grid=None,
-        fluorescence=True, exact_tally=True):
+        fluorescence=True, exact_tally=True, region=None):
     """chatcarlo/transport.py:257-312 + the serial branch of :358-474."""
     rng = np.random.default_rng(seed)
     edep = {}
     sums, sum2 = {}, {}
     n_abs = n_esc = scat = n_fl = 0
     n_b = 0
+    r_sum = r_sum2 = 0.0
     remaining = n_histories
     while remaining > 0:
         nb = min(batch_size, remaining)
         remaining -= nb
         pos, dirv, en = sample_source(src, nb, rng, spec_e, spec_cdf)
         r = transport_photons(pos, dirv, en, geom, pack, rng, grid=grid,
-                              fluorescence=fluorescence, exact_tally=exact_tally)
+                              fluorescence=fluorescence, exact_tally=exact_tally, region=region)
+        r_sum += float(r["region_score"].sum())
+        r_sum2 += float((r["region_score"] ** 2).sum())
         for k, v in r["energy_deposited"].items():
             edep[k] = edep.get(k, 0.0) + v
         for k in set(sums) | set(r["energy_deposited"]):
@@ -183,4 +196,4 @@ def run(src, geom, pack, spec_e, spec_cdf, n_histories, seed, batch_size=200_000
         scat += int(r["n_scatter"].sum())
     return dict(energy_deposited_keV=edep, sums=sums, sum2=sum2, n_batches=n_b,
                 n_absorbed=n_abs, n_escaped=n_esc, scatter_sum=scat, n_fluorescence=n_fl,
-                n_histories=n_histories)
+                n_histories=n_histories, region_sum=r_sum, region_sum2=r_sum2)

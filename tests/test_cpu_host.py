@@ -34,17 +34,31 @@ def test_library_exports_every_declared_symbol():
 
 def test_ctypes_layouts_match_the_header(tmp_path):
     from chatcarlo_amd import _native as N
+    checks = [("sizeof(cc_prim)", C.sizeof(N.CcPrim)), ("sizeof(cc_world)", C.sizeof(N.CcWorld)),
+              ("sizeof(cc_tables)", C.sizeof(N.CcTables)), ("sizeof(cc_source)", C.sizeof(N.CcSource)),
+              ("sizeof(cc_run_out)", C.sizeof(N.CcRunOut)),
+              ("offsetof(cc_run_out, edep_keV)", N.CcRunOut.edep_keV.offset),
+              ("offsetof(cc_run_out, region_sum)", N.CcRunOut.region_sum.offset),
+              ("offsetof(cc_run_out, n_records)", N.CcRunOut.n_records.offset),
+              ("sizeof(cc_train_cfg)", C.sizeof(N.CcTrainCfg)),
+              ("offsetof(cc_train_cfg, seed)", N.CcTrainCfg.seed.offset),
+              ("offsetof(cc_train_cfg, target_power)", N.CcTrainCfg.target_power.offset),
+              ("offsetof(cc_importance_cfg, split_gain)", N.CcImportanceCfg.split_gain.offset),
+              ("sizeof(cc_train_out)", C.sizeof(N.CcTrainOut)),
+              ("offsetof(cc_train_out, n_positive)", N.CcTrainOut.n_positive.offset),
+              ("sizeof(cc_importance_cfg)", C.sizeof(N.CcImportanceCfg)),
+              ("offsetof(cc_importance_cfg, seed)", N.CcImportanceCfg.seed.offset),
+              ("CC_MLP_PARAMS", N.MLP_PARAMS), ("CC_RECORD_FLOATS", N.RECORD_FLOATS),
+              ("CC_RUN_SOURCE_BIAS", N.RUN_SOURCE_BIAS), ("CC_RUN_IMPORTANCE", N.RUN_IMPORTANCE),
+              ("CC_RUN_RECORD", N.RUN_RECORD), ("CC_RUN_REGION", N.RUN_REGION)]
+    body = "".join(f'printf("%zu\\n", (size_t)({expr}));' for expr, _ in checks)
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "chatcarlo_hip.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(cc_prim), sizeof(cc_world),'
-                   ' sizeof(cc_tables), sizeof(cc_source), sizeof(cc_run_out), offsetof(cc_run_out, edep_keV));'
-                   'return 0;}\n')
+                   f'int main(void){{{body}return 0;}}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    want = [C.sizeof(N.CcPrim), C.sizeof(N.CcWorld), C.sizeof(N.CcTables), C.sizeof(N.CcSource),
-            C.sizeof(N.CcRunOut), N.CcRunOut.edep_keV.offset]
-    assert got == want
+    assert got == [v for _, v in checks]
     from chatcarlo_amd.geometry import PRIM_DTYPE, WORLD_DTYPE
     assert PRIM_DTYPE.itemsize == C.sizeof(N.CcPrim) == 48
     assert WORLD_DTYPE.itemsize == C.sizeof(N.CcWorld)

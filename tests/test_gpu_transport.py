@@ -171,3 +171,35 @@ def test_run_transport_api(engine):
                        batch_size=10_000)
     np.testing.assert_allclose(r3.grid.kerma_keV, r1.grid.kerma_keV, rtol=1e-12, atol=1e-300)
     assert r3.fraction_absorbed == r1.fraction_absorbed
+
+
+def test_grazing_flights_do_not_creep(engine):
+    """Photons on a cylinder's side moving almost along its axis, and photons leaving a sphere:
+    with boundary roots measured from the flight origin a crossed surface is never found again,
+    so a flight crosses each surface at most twice (regression: measured from the rounded
+    current point the same surface reappeared a few ulps ahead and such flights crept along it
+    for more than 1e5 steps)."""
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.workloads import angio_room, chest_room
+    rng = np.random.default_rng(17)
+    n = 4096
+    for fn, centre, radius, axis in ((chest_room, np.array([0.0, 5.0, 130.0]), 1.5, 2),
+                                     (angio_room, np.array([60.0, 0.0, 150.0]), 10.0, None)):
+        raw, _ = fn()
+        geo = Geometry(raw["geometry"])
+        engine.set_geometry(geo)
+        if axis is not None:
+            phi = rng.uniform(0, 2 * np.pi, n)
+            pos = np.stack([centre[0] + radius * np.cos(phi), centre[1] + radius * np.sin(phi),
+                            centre[2] + rng.uniform(-15, 15, n)], axis=1)
+            d = np.stack([rng.normal(0, 1e-3, n), rng.normal(0, 1e-3, n), np.sign(rng.normal(size=n))], axis=1)
+        else:
+            v = rng.normal(size=(n, 3))
+            v /= np.linalg.norm(v, axis=1, keepdims=True)
+            pos = centre + radius * v
+            d = rng.normal(size=(n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        e = np.full(n, 120.0)
+        *_, out = engine.transport(pos, d, e, seed=3)
+        assert out.error_bits == 0
+        assert out.n_events / out.n_flights < 2 * 8 + 2
