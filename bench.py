@@ -85,6 +85,32 @@ def cpu_baseline(raw, all_cores):
     return out
 
 
+def learned_fom(eng, target=0.02):
+    """Angio room, operator sphere: contribution records from analog GPU runs, the MLP trained on
+    the GPU, then analog and learned runs to the same relative error of the operator H*(10)."""
+    from chatcarlo_amd.importance import LearnedImportance
+    from chatcarlo_amd.workloads import angio_room, angio_region
+    raw, _ = angio_room()
+    c, r = angio_region()
+    li = LearnedImportance(eng, raw, c, r)
+    t0 = time.perf_counter()
+    n_rec, _ = li.generate_records(10_000_000, 32_000_000, keep=0.3, seed=61)
+    t1 = time.perf_counter()
+    tr = li.train(0, steps=6000, batch=8192, lr=3e-3, lr_end=1e-4)
+    t2 = time.perf_counter()
+    li.configure()
+    ana, lrn, chosen, z = li.compare(target_rel_err=target)
+    return {"workload": "angio_room_90kVp_operator_sphere_r10cm", "target_rel_err": target,
+            "analog": {"histories": ana.n_histories, "rel_err": ana.rel_err, "seconds": ana.seconds,
+                       "fom": ana.fom, "h10_pSv_per_history": ana.mean},
+            "learned": {"histories": lrn.n_histories, "rel_err": lrn.rel_err, "seconds": lrn.seconds,
+                        "fom": lrn.fom, "h10_pSv_per_history": lrn.mean, "splits": lrn.n_split,
+                        "roulette_kills": lrn.n_roulette},
+            "fom_ratio": lrn.fom / ana.fom, "z_learned_vs_analog": z, "learned_chosen": chosen is lrn,
+            "records": n_rec, "record_seconds": t1 - t0, "train_seconds": t2 - t1,
+            "train_device_ms": tr.ms, "train_loss_first_last": [tr.loss_first, tr.loss_last]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +120,7 @@ def main():
     ap.add_argument("--batch", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-all-cores", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-importance FOM comparison")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -178,6 +205,8 @@ def main():
         "physics": {"fraction_absorbed": o.n_absorbed / n, "fraction_escaped": o.n_escaped / n,
                     "mean_interactions": o.scatter_sum / n, "n_fluorescence": int(o.n_fluorescence)},
     }
+    if not args.no_learned:
+        line["learned_importance"] = learned_fom(eng)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(raw, args.cpu_all_cores)
         line["cpu_baseline"]["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]

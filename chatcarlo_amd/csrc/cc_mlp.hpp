@@ -149,32 +149,34 @@ __device__ __forceinline__ void mlp_forward_tile(int R, MlpTile& T, const float*
 // Every product is computed transposed, weights as the A operand and photons as
 // the B operand: Z1^T = W1^T X^T, then Z2^T = W2^T relu(Z1)^T.  The D layout of a
 // 16x16x16 MFMA (row 4 (l / 16) + e, column l % 16) is then exactly the B layout
-// the next MFMA wants, so the hidden layer never leaves registers; weights are
-// held in registers for the whole kernel.  No LDS and no block barrier.
+// the next MFMA wants, so the hidden layer never leaves registers.  W1 and W2 are
+// held in registers for the whole kernel, b2 and w3 in a 512-byte LDS block
+// (lanes of one g read one address: broadcast).
 // ---------------------------------------------------------------------------
 struct MlpWave {
   bf16x4 a1[4];        // a1[tm][e] = W1[4g + e][16 tm + r]          (r = lane % 16, g = lane / 16)
   bf16x4 a2[4][4];     // a2[tm][kb][e] = W2[16 kb + 4g + e][16 tm + r]
-  float b2[4][4];      // b2[tm][e] = b2[16 tm + 4g + e]
-  float w3[4][4];
+  const float* bw;     // LDS: b2 [64], w3 [64]
   float b3;
 };
 
-__device__ __forceinline__ void mlp_wave_load(const uint16_t* wb, const float* w, MlpWave& M) {
+// Block-wide (call before any divergence): registers of every wave and the LDS block sbw[128].
+__device__ __forceinline__ void mlp_wave_load(const uint16_t* wb, const float* w, MlpWave& M, float* sbw) {
   const int lane = threadIdx.x & 63;
   const int r = lane & 15, g = lane >> 4;
+  for (int k = threadIdx.x; k < 2 * kMlpH; k += blockDim.x) sbw[k] = w[kOffB2 + k];   // b2 then w3
 #pragma unroll
   for (int tm = 0; tm < 4; ++tm) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       M.a1[tm][e] = (short)wb[(4 * g + e) * kMlpH + 16 * tm + r];
-      M.b2[tm][e] = w[kOffB2 + 16 * tm + 4 * g + e];
-      M.w3[tm][e] = w[kOffW3 + 16 * tm + 4 * g + e];
 #pragma unroll
       for (int kb = 0; kb < 4; ++kb) M.a2[tm][kb][e] = (short)wb[kOffW2 + (16 * kb + 4 * g + e) * kMlpH + 16 * tm + r];
     }
   }
+  M.bw = sbw;
   M.b3 = w[kOffB3];
+  __syncthreads();
 }
 
 // f of the photon whose 16 features this lane holds.  Wave-uniform call.
@@ -212,7 +214,10 @@ __device__ __forceinline__ float mlp_wave_eval(const MlpWave& M, const float x[k
 #pragma unroll
       for (int kb = 0; kb < 4; ++kb) z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(M.a2[tm][kb], h[kb], z, 0, 0, 0);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) acc += fmaxf(z[e] + M.b2[tm][e], 0.0f) * M.w3[tm][e];
+      for (int e = 0; e < 4; ++e) {
+        const int h2 = 16 * tm + 4 * g + e;
+        acc += fmaxf(z[e] + M.bw[h2], 0.0f) * M.bw[kMlpH + h2];
+      }
     }
     acc += __shfl_xor(acc, 16);
     acc += __shfl_xor(acc, 32);

@@ -54,7 +54,8 @@ constexpr int kTrialCap = 100000;
 
 enum { F_ABSORBED = 1, F_ESCAPED = 2, F_DEAD = 4 };
 // device counter slots (uint32)
-enum { C_ACT_A = 0, C_ACT_B = 1, C_QP = 2, C_QC = 3, C_QR = 4, C_IMP = 5, C_POOL = 6, C_COLL = 7, C_NCNT = 8 };
+enum { C_ACT_A = 0, C_ACT_B = 1, C_QP = 2, C_QC = 3, C_QR = 4, C_IMP = 5, C_POOL = 6, C_COLL = 7, C_RESV = 8,
+       C_NCNT = 16 };
 // device stats slots (u64)
 enum { S_ABS = 0, S_ESC = 1, S_SCAT = 2, S_FLUOR = 3, S_EVT = 4, S_FLIGHT = 5, S_VOX = 6, S_SPLIT = 7, S_ROUL = 8,
        S_NSTAT = 10 };
@@ -97,6 +98,9 @@ struct Importance {        // weight windows from the learned importance map
   MlpFrame frame;
   float cw, w_lo, w_hi;    // target weight = clamp(cw / exp(fscale f), w_lo, w_hi)
   float split_gain;        // collision split count = split_gain w exp(f(region direction)) / cw
+  float region_r;          // radius of the scoring region (its centre is frame.c)
+  uint32_t* stage;         // per-wave staging of new photons (collision splitting), kStageFields x slots
+  uint32_t stage_slots;    // staged photons per wave (2 split_max 64)
   int split_max;
   uint32_t pool_cap;       // particle slots; the next free one is counted in cnt[C_POOL]
   int on;
@@ -357,7 +361,8 @@ __global__ __launch_bounds__(kBlock) void k_init_given(Launch L, int n, int* act
 
 __global__ void k_round_reset(uint32_t* cnt, int next_slot) {
   if (threadIdx.x == 0) {
-    cnt[C_QP] = 0; cnt[C_QC] = 0; cnt[C_QR] = 0; cnt[C_IMP] = 0; cnt[C_COLL] = 0; cnt[next_slot] = 0;
+    cnt[C_QP] = 0; cnt[C_QC] = 0; cnt[C_QR] = 0; cnt[C_IMP] = 0; cnt[C_COLL] = 0; cnt[C_RESV] = 0;
+    cnt[next_slot] = 0;
   }
 }
 
@@ -811,8 +816,9 @@ __device__ __forceinline__ void lane_features(const Launch& L, const Importance&
 // wave evaluates the map for its own 64 photons.
 __global__ __launch_bounds__(kBlock) void k_importance(Launch L, Importance I, const int* in, const uint32_t* cnt_in,
                                                        int* out, uint32_t* cnt_out) {
+  __shared__ float sbw[2 * kMlpH];
   MlpWave M;
-  mlp_wave_load(I.wb, I.w, M);
+  mlp_wave_load(I.wb, I.w, M, sbw);
   const uint32_t n = *cnt_in;
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
   for (uint32_t base = (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n;
@@ -838,71 +844,116 @@ __global__ __launch_bounds__(kBlock) void k_importance(Launch L, Importance I, c
   }
 }
 
-// One interaction at (material m, energy E) of a photon with incoming direction
-// d: the type, then the outcome, with the draws of the analog interaction
-// kernels.  Returns whether a photon leaves the collision (E_out, d_out); dep is
-// the energy it leaves behind.
-__device__ __forceinline__ bool collide_outcome(const Tables& T, int m, float E, const EPos& ep, const float d[3],
-                                                int fluor, PhiloxRng& rng, int* err, float& E_out, float d_out[3],
-                                                float& dep, int& type) {
-  type = sample_type(T, m, E, ep, rng.next());
+// One interaction at (material m, energy E), type from the part fractions (pp, pc),
+// with the draws of the analog interaction kernels up to the azimuth: returns
+// whether a photon leaves the collision and its polar angle to the incoming
+// direction (cos_t, sin_t) and energy; dep is the energy left behind.
+// Fluorescence is isotropic, i.e. cos_t uniform on [-1, 1] about any axis.
+__device__ __forceinline__ bool collide_polar(const Tables& T, int m, float E, const EPos& ep, float pp, float pc,
+                                              int fluor, PhiloxRng& rng, int* err, float& E_out, float& cos_t,
+                                              float& sin_t, float& dep, int& type) {
+  const float r = rng.next();
+  type = (r < pp) ? 0 : ((r < pp + pc) ? 1 : 2);
   if (type == 0) {
     int el;
     float eline = 0.f;
     const bool emit = fluor && fluor_sample(T, m, E, ep, rng, el, eline);
     dep = emit ? E - eline : E;
     if (!emit) return false;
-    const float u1 = rng.next();
-    const float u2 = rng.next();
-    isotropic_dir(d_out, u1, u2);
+    const float u = rng.next();
+    cos_t = -1.0f + 2.0f * u;
+    sin_t = 2.0f * sqrtf(fmaxf(u * (1.0f - u), 0.0f));
     E_out = eline;
     return true;
   }
-  float cs, sn;
   int el, trials;
   if (type == 1) {
-    const float eps = compton_sample(T, m, E, ep, rng, kTrialCap, cs, sn, el, trials);
+    const float eps = compton_sample(T, m, E, ep, rng, kTrialCap, cos_t, sin_t, el, trials);
     E_out = E * eps;
     dep = E - E_out;
   } else {
     bool qerr;
-    cs = rayleigh_sample(T, m, E, ep, rng, kTrialCap, sn, el, trials, qerr);
+    cos_t = rayleigh_sample(T, m, E, ep, rng, kTrialCap, sin_t, el, trials, qerr);
     if (qerr) raise_err(err, CC_DEV_Q_RANGE);
     E_out = E;
     dep = 0.f;
   }
   if (trials == 0) raise_err(err, CC_DEV_ITER_CAP);
-  d_out[0] = d[0]; d_out[1] = d[1]; d_out[2] = d[2];
-  rotate_dir(d_out, cs, sn, rng.next());
   return true;
 }
 
-// Collisions of a learned-importance run, fused: split, sample, window.
+// The (e1, e2) frame of rotate_dir about d.
+__device__ __forceinline__ void dir_frame(const float d[3], float e1[3], float e2[3]) {
+  float up0 = 0.f, up1 = 0.f, up2 = 1.f;
+  if (!(fabsf(d[2]) < 0.999f)) { up0 = 1.f; up2 = 0.f; }
+  e1[0] = up1 * d[2] - up2 * d[1];
+  e1[1] = up2 * d[0] - up0 * d[2];
+  e1[2] = up0 * d[1] - up1 * d[0];
+  const float inv = 1.0f / sqrtf(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+  e1[0] *= inv; e1[1] *= inv; e1[2] *= inv;
+  e2[0] = d[1] * e1[2] - d[2] * e1[1];
+  e2[1] = d[2] * e1[0] - d[0] * e1[2];
+  e2[2] = d[0] * e1[1] - d[1] * e1[0];
+}
+
+__device__ __forceinline__ void dir_at(const float d[3], const float e1[3], const float e2[3], float cos_t, float sin_t,
+                                       float phi, float out[3]) {
+  float sp, cp;
+  sincosf(phi, &sp, &cp);
+  const float a = sin_t * cp, b = sin_t * sp;
+  float n[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) n[k] = a * e1[k] + b * e2[k] + cos_t * d[k];
+  const float inv = 1.0f / sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) out[k] = n[k] * inv;
+}
+
+constexpr int kStageFields = 14;   // x y z dx dy dz e wt tau ctr tag hid nsc mat
+constexpr int kSplitBlocks = 1024;  // grid of the collision kernel (its staging is per wave)
+
+// Collisions of a learned-importance run, fused: split, sample, stratify, window.
 //
-// A photon of weight w colliding at x is split into n = min(gain w I(x, u) / cw,
-// split_max) copies of weight w / n, where I(x, u) = exp(f) is the map for a
-// photon at x heading straight for the scoring region (u = direction to its
-// centre): a copy that scatters towards the region then arrives at that
-// direction's window centre.  Each copy draws its own interaction (type,
-// energy, direction) from its own Philox lane, deposits its share, and its
-// outgoing photon is judged by the map at once: below the window it survives
-// roulette with probability w' / wt at weight wt.  Only survivors take a pool
-// slot; killed copies never leave registers.  Copies 1 .. n-1 use fresh lanes
-// and fresh slots; copy 0 runs last on the parent's lane and slot.  Before each
-// round of copies a wave checks that the pool has room for one slot per lane of
-// every wave of the grid (each wave has at most one such round between its
-// check and its allocation, so the pool cannot overflow); when it has not, the
-// copies not yet drawn are merged into copy 0 (its weight becomes the sum of
-// theirs), which keeps the split unbiased and needs no new slot.
+// Split.  A photon of weight w colliding at x is split into n = min(gain w I(x, u)
+// / cw, split_max) copies of weight w / n, where I(x, u) = exp(f) is the map for
+// a photon at x heading straight for the scoring region (u = direction to its
+// centre).  Each copy draws its own interaction type, polar angle and energy
+// from its own Philox lane and deposits its share.
+//
+// Stratify.  The azimuth of the outgoing direction is uniform, so for a copy's
+// polar angle t the azimuths that point into the region's cone (half angle a =
+// asin(R / |c - x|)) form one arc of known width D (from cos a = cos t cos p +
+// sin t sin p cos(phi - phi0), p the angle between the incoming direction and u).
+// A copy whose arc is neither empty nor full leaves as two photons, one with phi
+// uniform on the arc and weight share D / 2 pi, one with phi uniform on the rest
+// and share 1 - D / 2 pi: a stratified draw of the azimuth, exact and unbiased.
+//
+// Window.  Each outgoing photon is judged by the map at once: below its window
+// it survives roulette with probability w' / wt at weight wt.  Only survivors
+// take a pool slot; killed photons never leave registers.  Copies 1 .. n-1 use
+// fresh lanes and slots; copy 0 runs last on the parent's lane, and its first
+// survivor takes the parent's slot.  Other survivors are staged in the wave's
+// own region of a staging buffer and moved to pool slots at the end of the tile
+// with one allocation and one append to the next active list per wave: atomics
+// on those shared counters would otherwise serialise every round of copies.
+// Before each tile a wave reserves its worst case (2 split_max slots per lane)
+// in a reservation counter and proceeds only if the pool holds the slots in use
+// plus every outstanding reservation (a reservation is released after the
+// tile's allocation, so the pool cannot overflow); otherwise the tile does not
+// split (n = 1) and copy 0 is not stratified.  Counters and deposits are summed
+// in registers.
 __global__ __launch_bounds__(kBlock) void k_scatter_split(Launch L, Importance I, const int* in,
                                                           const uint32_t* cnt_in, int* act_next, uint32_t* cnt_next) {
+  __shared__ float sbw[2 * kMlpH];
   MlpWave M;
-  mlp_wave_load(I.wb, I.w, M);
+  mlp_wave_load(I.wb, I.w, M, sbw);
   const uint32_t n_in = *cnt_in;
   PState S = L.S;
   const Tables& T = L.T;
   const int lane = __lane_id();
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const float kTwoPiF = 6.283185307179586f;
+  unsigned long long n_abs = 0, n_fluor = 0, n_scat = 0, n_roul = 0, n_split = 0;
   for (uint32_t base = (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n_in;
        base += waves * kWave) {
     const uint32_t j = base + lane;
@@ -918,121 +969,168 @@ __global__ __launch_bounds__(kBlock) void k_scatter_split(Launch L, Importance I
       hid = S.hid[i]; tag = S.tag[i]; ctr0 = S.ctr[i];
     }
     const EPos ep = energy_pos(T, E);
+    float pp = 0.f, pc = 0.f;
+    {
+      const float ph = xs_part(T, m, P_PHOTO, E, ep), co = xs_part(T, m, P_COMPT, E, ep);
+      const float tot = ph + co + xs_part(T, m, P_RAYL, E, ep);
+      if (tot > 0.f) { pp = ph / tot; pc = co / tot; }
+    }
+    // geometry of the region seen from the collision point, in the frame of d
+    float u[3] = {I.frame.c[0] - p[0], I.frame.c[1] - p[1], I.frame.c[2] - p[2]};
+    const float dist = sqrtf(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    if (dist > 0.f) { u[0] /= dist; u[1] /= dist; u[2] /= dist; } else { u[0] = 0.f; u[1] = 0.f; u[2] = 1.f; }
+    const float ratio = dist > I.region_r ? I.region_r / dist : 1.0f;
+    const float cos_a = dist > I.region_r ? sqrtf(fmaxf(1.f - ratio * ratio, 0.f)) : -1.0f;
+    float e1[3], e2[3];
+    dir_frame(d, e1, e2);
+    const float cos_p = fminf(fmaxf(d[0] * u[0] + d[1] * u[1] + d[2] * u[2], -1.f), 1.f);
+    const float sin_p = sqrtf(fmaxf(1.f - cos_p * cos_p, 0.f));
+    const float phi0 = atan2f(u[0] * e2[0] + u[1] * e2[1] + u[2] * e2[2], u[0] * e1[0] + u[1] * e1[1] + u[2] * e1[2]);
     // split count from the map for the direction to the region centre
     float x[kMlpIn];
-    {
-      float u[3] = {I.frame.c[0] - p[0], I.frame.c[1] - p[1], I.frame.c[2] - p[2]};
-      const float ul = sqrtf(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-      if (ul > 0.f) { u[0] /= ul; u[1] /= ul; u[2] /= ul; } else { u[0] = 0.f; u[1] = 0.f; u[2] = 1.f; }
-      mlp_features(I.frame, p, u, E, x);
-    }
+    mlp_features(I.frame, p, u, E, x);
     const float fr = mlp_wave_eval(M, x);
+    const uint32_t worst = 2u * (uint32_t)I.split_max * (uint32_t)__popcll(__ballot(valid));
+    int room = 0;
+    if (lane == 0) {
+      const uint32_t r_all = atomicAdd(L.cnt + C_RESV, worst) + worst;
+      const uint32_t used = atomicAdd(L.cnt + C_POOL, 0u);
+      room = (uint64_t)used + r_all <= I.pool_cap;
+      if (!room) atomicSub(L.cnt + C_RESV, worst);
+    }
+    room = __shfl(room, 0);
     int ncp = 1;
-    if (valid) {
+    if (valid && room) {
       const float want = I.split_gain * w * expf(fminf(fmaxf(fr, -80.f), 80.f)) / I.cw;
       if (want >= 2.f) ncp = (int)fminf(want, (float)I.split_max);
     }
     int most = ncp;
     for (int off = kWave / 2; off > 0; off >>= 1) most = max(most, __shfl_xor(most, off));
-    const float wc = w / (float)ncp;
-    int drawn = 0;   // copies 1 .. drawn already sampled
-    for (int c = 1; c < most; ++c) {
-      const bool act = valid && c < ncp;
-      if (__ballot(act) == 0ull) break;
-      int ok = 0;
-      if (lane == 0) {
-        const uint32_t used = atomicAdd(L.cnt + C_POOL, 0u);
-        ok = (uint64_t)used + (uint64_t)kWave * waves <= I.pool_cap;
-      }
-      if (!__shfl(ok, 0)) break;   // pool nearly full: the rest merge into copy 0
-      bool alive = false;
-      float eo = E, dout[3] = {0.f, 0.f, 1.f}, dep = 0.f, wo = wc;
-      uint32_t tg = 0, ctr = 0;
+    const float wcp = w / (float)ncp;
+    bool parent_free = true, parent_alive = false;
+    double dep_acc = 0.0;
+    uint32_t staged = 0;   // wave-uniform count of staged photons
+    uint32_t* stg = I.stage + (size_t)(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kStageFields *
+                                  I.stage_slots;
+    for (int it = 1; it <= most; ++it) {
+      const bool last = it == most;          // copy 0, the parent's lane and slot
+      const int c = last ? 0 : it;
+      const bool act = valid && (last || c < ncp);
+      if (__ballot(act) == 0ull) continue;
+      const float wc = wcp;
+      const uint32_t tg = last ? tag : mix32(tag, ctr0, (uint32_t)c);
+      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)hid, tg, last ? ctr0 : 0u);
+      bool out_ok = false;
+      float eo = E, ct = 1.f, st = 0.f, dep = 0.f;
       int type = -1;
       if (act) {
-        tg = mix32(tag, ctr0, (uint32_t)c);
-        PhiloxRng rng(L.seed, L.hist0 + (uint64_t)hid, tg, 0u);
-        alive = collide_outcome(T, m, E, ep, d, L.fluor, rng, L.err, eo, dout, dep, type);
-        ctr = rng.ctr;
-        drawn = c;
+        out_ok = collide_polar(T, m, E, ep, pp, pc, L.fluor, rng, L.err, eo, ct, st, dep, type);
+        dep_acc += (double)wc * (double)dep;
+        n_scat += 1;
+        if (type == 0) { if (out_ok) n_fluor += 1; else n_abs += 1; }
       }
-      wave_add_by_key(act, m, (double)wc * (double)dep, L.edep);
-      wave_count(act && type == 0 && !alive, L.stats + S_ABS);
-      wave_count(act && type == 0 && alive, L.stats + S_FLUOR);
-      wave_count(act, L.stats + S_SCAT);
-      // window of the outgoing photon (roulette only)
-      if (alive) mlp_features(I.frame, p, dout, eo, x);
-      else for (int k = 0; k < kMlpIn; ++k) x[k] = 0.f;
-      const float fo = mlp_wave_eval(M, x);
-      float tau = 0.f;
-      bool killed = false;
-      if (alive) {
-        PhiloxRng rng(L.seed, L.hist0 + (uint64_t)hid, tg, ctr);
-        const float wt = fminf(fmaxf(I.cw * expf(-fminf(fmaxf(fo, -80.f), 80.f)), I.w_lo), I.w_hi);
-        if (wo < 0.5f * wt) {
-          if (rng.next() < wo / wt) wo = wt;
-          else { alive = false; killed = true; }
+      // azimuth: one uniform draw, or the two strata of the arc towards the region
+      int nout = out_ok ? 1 : 0;
+      float phi[2] = {0.f, 0.f}, share[2] = {1.f, 0.f};
+      if (out_ok) {
+        const float uphi = rng.next();
+        float half = -1.f;   // half-width of the arc, when it is neither empty nor full
+        if (room && sin_p * st > 1e-6f) {
+          const float k = (cos_a - ct * cos_p) / (st * sin_p);
+          if (k > -1.f && k < 1.f) half = acosf(k);
         }
-        tau = -logf(rng.next());
-        ctr = rng.ctr;
-      }
-      wave_count(killed, L.stats + S_ROUL);
-      // slots for the survivors
-      uint32_t tot = 0;
-      const uint32_t off = wave_excl_scan(alive ? 1u : 0u, tot);
-      uint32_t s0 = 0;
-      if (lane == 0 && tot > 0) s0 = atomicAdd(L.cnt + C_POOL, tot);
-      s0 = __shfl(s0, 0);
-      if (alive) {
-        const int q = (int)(s0 + off);
-        S.x[q] = p[0]; S.y[q] = p[1]; S.z[q] = p[2];
-        S.dx[q] = dout[0]; S.dy[q] = dout[1]; S.dz[q] = dout[2];
-        S.e[q] = eo; S.wt[q] = wo; S.tau[q] = tau; S.score[q] = 0.f;
-        S.ctr[q] = ctr; S.hid[q] = hid; S.tag[q] = tg; S.nsc[q] = nsc + 1; S.mat[q] = m; S.flags[q] = 0;
-      }
-      wave_sum_u64(alive ? 1ull : 0ull, L.stats + S_SPLIT);
-      const int k = wave_append(alive, cnt_next);
-      if (alive) act_next[k] = (int)(s0 + off);
-    }
-    // copy 0: the parent's lane and slot, carrying the weight of every copy not drawn
-    {
-      const float w0 = valid ? w * (float)(ncp - drawn) / (float)ncp : 0.f;
-      bool alive = false;
-      float eo = E, dout[3] = {0.f, 0.f, 1.f}, dep = 0.f, wo = w0, tau = 0.f;
-      int type = -1;
-      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)hid, tag, ctr0);
-      if (valid) alive = collide_outcome(T, m, E, ep, d, L.fluor, rng, L.err, eo, dout, dep, type);
-      wave_add_by_key(valid, m, (double)w0 * (double)dep, L.edep);
-      wave_count(valid && type == 0 && !alive, L.stats + S_ABS);
-      wave_count(valid && type == 0 && alive, L.stats + S_FLUOR);
-      wave_count(valid, L.stats + S_SCAT);
-      if (alive) mlp_features(I.frame, p, dout, eo, x);
-      else for (int k = 0; k < kMlpIn; ++k) x[k] = 0.f;
-      const float fo = mlp_wave_eval(M, x);
-      bool killed = false;
-      if (alive) {
-        const float wt = fminf(fmaxf(I.cw * expf(-fminf(fmaxf(fo, -80.f), 80.f)), I.w_lo), I.w_hi);
-        if (wo < 0.5f * wt) {
-          if (rng.next() < wo / wt) wo = wt;
-          else { alive = false; killed = true; }
+        if (half > 0.f) {
+          const float frac = half / 3.14159265f;
+          phi[0] = phi0 + (2.f * uphi - 1.f) * half;
+          phi[1] = phi0 + half + rng.next() * (kTwoPiF - 2.f * half);
+          share[0] = frac;
+          share[1] = 1.f - frac;
+          nout = 2;
+        } else {
+          phi[0] = kTwoPiF * uphi;
         }
-        tau = -logf(rng.next());
       }
-      wave_count(killed, L.stats + S_ROUL);
-      if (valid) {
-        S.ctr[i] = rng.ctr;
-        S.nsc[i] = nsc + 1;
-        if (alive) {
+      const uint32_t ctr_after = rng.ctr;
+      const int n_o = __ballot(nout == 2) ? 2 : 1;
+      for (int o = 0; o < n_o; ++o) {
+        const bool has = o < nout;
+        float dout[3] = {0.f, 0.f, 1.f};
+        if (has) dir_at(d, e1, e2, ct, st, phi[o], dout);
+        if (has) mlp_features(I.frame, p, dout, eo, x);
+        else for (int k = 0; k < kMlpIn; ++k) x[k] = 0.f;
+        const float fo = mlp_wave_eval(M, x);
+        bool alive = has;
+        float wo = wc * share[o], tau = 0.f;
+        const uint32_t tgo = o == 0 ? tg : mix32(tg, ctr_after, 0x51u);
+        PhiloxRng ro(L.seed, L.hist0 + (uint64_t)hid, tgo, o == 0 ? ctr_after : 0u);
+        if (has) {
+          const float wt = fminf(fmaxf(I.cw * expf(-fminf(fmaxf(fo, -80.f), 80.f)), I.w_lo), I.w_hi);
+          if (wo < 0.5f * wt) {
+            if (ro.next() < wo / wt) wo = wt;
+            else { alive = false; n_roul += 1; }
+          }
+          tau = -logf(ro.next());
+        }
+        const bool to_parent = alive && last && parent_free;
+        if (to_parent) { parent_free = false; parent_alive = true; }
+        const bool need = alive && !to_parent;
+        uint32_t tot = 0;
+        const uint32_t off = wave_excl_scan(need ? 1u : 0u, tot);
+        if (to_parent) {
           S.dx[i] = dout[0]; S.dy[i] = dout[1]; S.dz[i] = dout[2];
           S.e[i] = eo; S.wt[i] = wo; S.tau[i] = tau;
-        } else if (type == 0 && !killed) {
-          S.flags[i] |= F_ABSORBED;
+          S.ctr[i] = ro.ctr; S.tag[i] = tgo; S.nsc[i] = nsc + 1; S.flags[i] = 0;
         }
+        if (need) {
+          const uint32_t k = staged + off, ns = I.stage_slots;
+          const float v[9] = {p[0], p[1], p[2], dout[0], dout[1], dout[2], eo, wo, tau};
+#pragma unroll
+          for (int f = 0; f < 9; ++f) stg[f * ns + k] = __float_as_uint(v[f]);
+          stg[9 * ns + k] = ro.ctr;
+          stg[10 * ns + k] = tgo;
+          stg[11 * ns + k] = hid;
+          stg[12 * ns + k] = (uint32_t)(nsc + 1);
+          stg[13 * ns + k] = (uint32_t)m;
+          n_split += 1;
+        }
+        staged += tot;
       }
-      const int k = wave_append(alive, cnt_next);
-      if (alive) act_next[k] = i;
+      if (last && act && parent_free) {   // nothing left in the parent's slot
+        S.nsc[i] = nsc + 1;
+        if (type == 0 && !out_ok) S.flags[i] |= F_ABSORBED;
+      }
+    }
+    wave_add_by_key(valid, m, dep_acc, L.edep);
+    // move the staged photons to pool slots; append parents and new photons to the next list
+    uint32_t npar = 0;
+    const uint32_t poff = wave_excl_scan(parent_alive ? 1u : 0u, npar);
+    uint32_t s0 = 0, a0 = 0;
+    if (lane == 0) {
+      if (staged > 0) s0 = atomicAdd(L.cnt + C_POOL, staged);
+      if (room) atomicSub(L.cnt + C_RESV, worst);
+      if (staged + npar > 0) a0 = atomicAdd(cnt_next, staged + npar);
+    }
+    s0 = __shfl(s0, 0);
+    a0 = __shfl(a0, 0);
+    if (parent_alive) act_next[a0 + poff] = i;
+    for (uint32_t k = lane; k < staged; k += kWave) {
+      const uint32_t ns = I.stage_slots;
+      const int q = (int)(s0 + k);
+      S.x[q] = __uint_as_float(stg[k]); S.y[q] = __uint_as_float(stg[ns + k]); S.z[q] = __uint_as_float(stg[2 * ns + k]);
+      S.dx[q] = __uint_as_float(stg[3 * ns + k]); S.dy[q] = __uint_as_float(stg[4 * ns + k]);
+      S.dz[q] = __uint_as_float(stg[5 * ns + k]); S.e[q] = __uint_as_float(stg[6 * ns + k]);
+      S.wt[q] = __uint_as_float(stg[7 * ns + k]); S.tau[q] = __uint_as_float(stg[8 * ns + k]);
+      S.ctr[q] = stg[9 * ns + k]; S.tag[q] = stg[10 * ns + k]; S.hid[q] = stg[11 * ns + k];
+      S.nsc[q] = (int)stg[12 * ns + k]; S.mat[q] = (int)stg[13 * ns + k];
+      S.flags[q] = 0; S.score[q] = 0.f;
+      act_next[a0 + npar + k] = q;
     }
   }
+  wave_sum_u64(n_abs, L.stats + S_ABS);
+  wave_sum_u64(n_fluor, L.stats + S_FLUOR);
+  wave_sum_u64(n_scat, L.stats + S_SCAT);
+  wave_sum_u64(n_roul, L.stats + S_ROUL);
+  wave_sum_u64(n_split, L.stats + S_SPLIT);
 }
 
 __global__ __launch_bounds__(kBlock) void k_record(Launch L, const int* act, const uint32_t* cnt, uint32_t batch) {
@@ -1103,8 +1201,9 @@ __global__ __launch_bounds__(kBlock) void k_region_end(double* hscore, uint32_t 
 // f of rows of 16 features with the wave-level inference of the transport kernels.
 __global__ __launch_bounds__(kBlock) void k_mlp_forward(const float* w, const uint16_t* wb, int n, const float* feat,
                                                         float* out) {
+  __shared__ float sbw[2 * kMlpH];
   MlpWave M;
-  mlp_wave_load(wb, w, M);
+  mlp_wave_load(wb, w, M, sbw);
   const int waves = gridDim.x * (blockDim.x / kWave);
   for (int base = (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n; base += waves * kWave) {
     const int row = base + __lane_id();
@@ -1522,6 +1621,8 @@ struct Ctx {
   cc_importance_cfg icfg{};
   float cw = 1.f;
   float *bcdf = nullptr, *bwt = nullptr;
+  uint32_t* stage = nullptr;   // collision-split staging
+  size_t stage_words = 0;
 };
 
 // Feature frame without a scoring region: world-box centre, half the largest extent.
@@ -1694,7 +1795,8 @@ int run_rounds(Ctx* c, const Launch& L, uint32_t n, Timer& tm, uint32_t* rounds_
       tm.end();
       if (o.pre) {
         tm.begin(K_OTHER);
-        hipLaunchKernelGGL(k_scatter_split, dim3(ib), dim3(kBlock), 0, c->stream, L, o.I, (const int*)c->coll,
+        hipLaunchKernelGGL(k_scatter_split, dim3(std::min(ib, kSplitBlocks)), dim3(kBlock), 0, c->stream, L, o.I,
+                           (const int*)c->coll,
                            c->cnt + C_COLL, act_nxt, c->cnt + nxt);
         tm.end();
       } else {
@@ -1878,7 +1980,7 @@ int cc_ctx_destroy(void* ctx) {
   for (void* p : c->tbufs) (void)hipFree(p);
   void* ps[] = {c->prims, c->tk, c->th, c->qk, c->qh, c->hits, c->G.dk, c->G.dh, c->cnt, c->stats, c->edep, c->err,
                 c->spec, c->rec, c->rcnt, c->mw[0], c->mm[0], c->mv[0], c->mgrad[0], c->mwb[0],
-                c->mw[1], c->mm[1], c->mv[1], c->mgrad[1], c->mwb[1], c->bcdf, c->bwt};
+                c->mw[1], c->mm[1], c->mv[1], c->mgrad[1], c->mwb[1], c->bcdf, c->bwt, c->stage};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
@@ -2103,9 +2205,22 @@ int cc_run(void* ctx, const cc_source* src, uint64_t n_hist, uint64_t seed, uint
     I.w_lo = c->icfg.w_lo;
     I.w_hi = c->icfg.w_hi;
     I.split_gain = c->icfg.split_gain > 0.f ? c->icfg.split_gain : 1.0f;
+    I.region_r = c->reg.on ? c->reg.r : 0.f;
+    I.stage_slots = 2u * (uint32_t)c->icfg.split_max * kWave;
     I.split_max = c->icfg.split_max;
     I.pool_cap = (uint32_t)c->cap;
     I.on = 1;
+  }
+  if (ro.pre) {
+    const size_t words = (size_t)kSplitBlocks * (kBlock / kWave) * kStageFields * ro.I.stage_slots;
+    if (words > c->stage_words) {
+      if (c->stage) (void)hipFree(c->stage);
+      c->stage = nullptr;
+      c->stage_words = 0;
+      if ((r = dalloc(&c->stage, words))) return r;
+      c->stage_words = words;
+    }
+    ro.I.stage = c->stage;
   }
   int err_bits = 0;
   uint64_t remaining = n_hist, h0 = first;

@@ -218,10 +218,15 @@ class LearnedImportance:
         return out
 
     # -- importance runs ----------------------------------------------------------------
-    def configure(self, w_lo: float = 1e-3, w_hi: float = 1e3, split_max: int = 16, pool_factor: float = 32.0,
+    def configure(self, w_lo: float = 1e-3, w_hi: float = 3.0, split_max: int = 4, pool_factor: float = 32.0,
                   source_bias: bool = True, bias_mix: float = 0.1, samples_per_cell: int = 64,
-                  cw: float = 0.0, use_post: bool = True, use_pre: bool = False, split_gain: float = 1.0,
+                  cw: float = 0.0, use_post: bool = True, use_pre: bool = True, split_gain: float = 1.0,
                   seed: int = 7):
+        """Weight windows from the trained map.  w_lo, w_hi bound the window centre (weights
+        relative to a unit source photon; a low w_hi keeps an underestimated importance from
+        growing heavy photons); split_max caps the copies per split; use_pre splits at
+        collisions before their outcome is drawn (k_scatter_split), use_post windows the source
+        photons (and every flight start when use_pre is off)."""
         self._bind()
         cfg = N.CcImportanceCfg(cw=float(cw), w_lo=float(w_lo), w_hi=float(w_hi), split_max=int(split_max),
                                 pool_factor=float(pool_factor), source_bias=1 if source_bias else 0,
@@ -238,7 +243,7 @@ class LearnedImportance:
         return self.cw
 
     def run(self, learned: bool, target_rel_err: float | None = 0.02, n_histories: int | None = None,
-            seed: int = 11, batch: int = 1_000_000, chunk_batches: int = 4, min_histories: int = 2_000_000,
+            seed: int = 11, batch: int = 1_000_000, chunk_batches: int = 1, min_histories: int = 500_000,
             max_histories: int = 2_000_000_000, max_seconds: float = 120.0, timing: bool = False) -> RegionResult:
         """Run until the region's relative error reaches target_rel_err (or exactly n_histories)."""
         self._bind()
@@ -272,12 +277,12 @@ class LearnedImportance:
     def compare(self, target_rel_err: float = 0.02, seed_analog: int = 101, seed_learned: int = 202,
                 batch: int = 1_000_000, **kw):
         """Analog and learned runs to the same relative error.  Returns (analog, learned, chosen, z):
-        z is the difference of the means in combined standard errors; the learned result is chosen
-        only if |z| <= 3."""
+        z is the difference of the means in combined standard errors.  The analog run judges the
+        learned one: its result is chosen only if |z| <= 3 and its figure of merit is higher."""
         a = self.run(False, target_rel_err, seed=seed_analog, batch=batch, **kw)
         b = self.run(True, target_rel_err, seed=seed_learned, batch=batch, **kw)
         z = (b.mean - a.mean) / math.sqrt(a.sem ** 2 + b.sem ** 2)
-        chosen = b if abs(z) <= 3.0 else a
+        chosen = b if abs(z) <= 3.0 and b.fom > a.fom else a
         if chosen is a:
             a.fallback = True
         return a, b, chosen, z

@@ -234,14 +234,14 @@ def test_collision_splitting_is_unbiased(engine):
     _constant_map(engine, li, 0.0, 1.0 / 8.0, post=False, pre=True)
     n = 1_000_000
     spl = li.run(True, None, n_histories=n, seed=24, batch=100_000, chunk_batches=5)
-    # only first collisions split (weight 1 / (1/8) = 8 copies); the outgoing copies sit at the
-    # window centre, so none is rouletted and later collisions do not split
-    assert spl.n_split > 0 and spl.n_roulette == 0
+    # first collisions split into 8 copies of weight 1/8 (later ones sit at the window centre);
+    # a copy whose azimuth can reach the region leaves as two strata, the lighter one below the
+    # window, so roulette acts too
+    assert spl.n_split > 0 and spl.n_roulette > 0
     z = (spl.mean - ana.mean) / math.sqrt(spl.sem ** 2 + ana.sem ** 2)
     assert abs(z) < 4.0, (spl.mean, ana.mean, z)
-    # independent outcomes per copy: a larger variance reduction per history than splitting at
-    # the source, where the 8 copies share the source ray
-    assert spl.rel_err * math.sqrt(spl.n_histories) < 0.6 * ana.rel_err * math.sqrt(ana.n_histories)
+    # independent outcomes per copy lower the variance per history
+    assert spl.rel_err * math.sqrt(spl.n_histories) < 0.8 * ana.rel_err * math.sqrt(ana.n_histories)
 
 
 def test_roulette_is_unbiased(engine):
@@ -312,9 +312,9 @@ def test_learned_importance_is_unbiased_and_beats_analog(engine):
     t0 = li.train(0, steps=6000, batch=8192, lr=3e-3, lr_end=1e-4)
     assert t0.loss_last < t0.loss_first
     li.configure()
-    ana, lrn, chosen, z = li.compare(target_rel_err=0.03, seed_analog=71, seed_learned=72)
+    ana, lrn, chosen, z = li.compare(target_rel_err=0.02, seed_analog=71, seed_learned=72)
     print(f"analog R={ana.rel_err:.4f} t={ana.seconds:.3f}s FOM={ana.fom:.1f} | learned R={lrn.rel_err:.4f} "
           f"t={lrn.seconds:.3f}s FOM={lrn.fom:.1f} gain={lrn.fom / ana.fom:.2f} z={z:.2f}")
     assert abs(z) < 3.0
     assert chosen is lrn
-    assert lrn.fom > 1.2 * ana.fom
+    assert lrn.fom > 5.0 * ana.fom
