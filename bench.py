@@ -58,7 +58,8 @@ def _cpu_sample(args):
     done = 0
     t0 = time.perf_counter()
     k = 0
-    while done < n and (time.perf_counter() - t0 < min_seconds or done == 0):
+    # min_seconds > 0: stop at n histories or once min_seconds have passed; 0: run all n
+    while done < n and (min_seconds <= 0 or time.perf_counter() - t0 < min_seconds):
         m = min(10_000, n - done)
         OT.run(raw["source"], og, pack, e, cdf, m, seed + k, batch_size=m, grid=grid, exact_tally=False)
         done += m
@@ -68,20 +69,23 @@ def _cpu_sample(args):
 
 def cpu_baseline(raw, all_cores):
     import multiprocessing as mp
-    done, dt = _cpu_sample((raw, 100_000, 11, 12.0))
+    done, dt = _cpu_sample((raw, 400_000, 11, 12.0))
     out = {"value": done / dt, "unit": "histories/s", "cores": 1, "kind": "port",
            "sample": f"{done} histories of the same workload (oracle NumPy restatement of the "
                      f"reference transport, stratified track-length tally), one process, {dt:.1f} s"}
     if all_cores:
         nproc = max(1, min(os.cpu_count() or 1, 16))
         ctx = mp.get_context("spawn")
+        per = max(10_000, done // 2)
         with ctx.Pool(nproc) as pool:
             t0 = time.perf_counter()
-            res = pool.map(_cpu_sample, [(raw, done, 100 + i, 0.0) for i in range(nproc)])
+            res = pool.map(_cpu_sample, [(raw, per, 100 + i, 0.0) for i in range(nproc)])
             dt2 = time.perf_counter() - t0
         tot = sum(r[0] for r in res)
-        out["all_cores"] = {"value": tot / dt2, "cores": nproc,
-                            "sample": f"{tot} histories over {nproc} processes, {dt2:.1f} s incl. start-up"}
+        slowest = max(r[1] for r in res)
+        out["all_cores"] = {"value": tot / slowest, "cores": nproc,
+                            "sample": f"{tot} histories over {nproc} processes ({per} each), slowest process "
+                                      f"{slowest:.1f} s of transport, {dt2:.1f} s wall incl. start-up"}
     return out
 
 
@@ -107,6 +111,7 @@ def learned_fom(eng, target=0.02):
                         "fom": lrn.fom, "h10_pSv_per_history": lrn.mean, "splits": lrn.n_split,
                         "roulette_kills": lrn.n_roulette},
             "fom_ratio": lrn.fom / ana.fom, "z_learned_vs_analog": z, "learned_chosen": chosen is lrn,
+            "fom_ratio_with_training": (1.0 / (lrn.rel_err ** 2 * (lrn.seconds + (t1 - t0) + (t2 - t1)))) / ana.fom,
             "records": n_rec, "record_seconds": t1 - t0, "train_seconds": t2 - t1,
             "train_device_ms": tr.ms, "train_loss_first_last": [tr.loss_first, tr.loss_last]}
 
@@ -119,8 +124,11 @@ def main():
     ap.add_argument("--histories", type=float, default=1e7)
     ap.add_argument("--batch", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-all-cores", action="store_true")
+    ap.add_argument("--no-cpu-all-cores", action="store_true",
+                    help="skip the all-cores leg of the CPU baseline (one process per core)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-importance FOM comparison")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="only the learned-importance comparison (prints its JSON object)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -138,6 +146,11 @@ def main():
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(histories shard across GPUs only in the design, not in this build)", file=sys.stderr)
         sys.exit(2)
+
+    if args.learned_only:
+        from chatcarlo_amd.transport import default_engine
+        print(json.dumps(learned_fom(default_engine())), flush=True)
+        return
 
     from chatcarlo_amd import _native as N
     from chatcarlo_amd.geometry import Geometry
@@ -208,8 +221,11 @@ def main():
     if not args.no_learned:
         line["learned_importance"] = learned_fom(eng)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(raw, args.cpu_all_cores)
-        line["cpu_baseline"]["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+        line["cpu_baseline"] = cpu_baseline(raw, not args.no_cpu_all_cores)
+        cb = line["cpu_baseline"]
+        cb["gpu_over_cpu"] = value / cb["value"]
+        if "all_cores" in cb:
+            cb["gpu_over_all_cores"] = value / cb["all_cores"]["value"]
     print(json.dumps(line), flush=True)
 
 

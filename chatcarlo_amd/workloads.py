@@ -63,12 +63,9 @@ def ct_helical(kvp=120.0, filtration_mm_al=6.0):
     over the phantom +- 2 cm."""
     geoms = [{"name": "phantom", "shape": "cylinder", "material": "pmma", "center": [0, 0, 0],
               "radius_cm": 16.0, "height_cm": 15.0, "axis": "z"}]
-    for i, (x, y) in enumerate([(0, 0), (15, 0), (-15, 0), (0, 15), (0, -15)]):
-        r = 0.65
-        if i > 0:
-            x, y = x * (15.0 - 1.0) / 15.0, y * (15.0 - 1.0) / 15.0
+    for i, (x, y) in enumerate([(0, 0), (15, 0), (-15, 0), (0, 15), (0, -15)]):   # holes 1 cm below the surface
         geoms.append({"name": f"hole{i}", "shape": "cylinder", "material": "air", "center": [x, y, 0],
-                      "radius_cm": r, "height_cm": 15.0, "axis": "z"})
+                      "radius_cm": 0.65, "height_cm": 15.0, "axis": "z"})
     raw = {
         "source": {"position": [0.0, -60.0, 0.0], "direction": [0.0, 1.0, 0.0],
                    "field": {"shape": "rect", "size_cm": [50.0, 4.0], "sid_cm": 60.0},

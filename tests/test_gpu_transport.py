@@ -203,3 +203,44 @@ def test_grazing_flights_do_not_creep(engine):
         *_, out = engine.transport(pos, d, e, seed=3)
         assert out.error_bits == 0
         assert out.n_events / out.n_flights < 2 * 8 + 2
+
+
+def test_ct_helical_chi2_against_oracle(engine, dpack):
+    """CTDI phantom under a helical 120 kVp scan (rotating, shifting source): per-voxel chi^2 of the
+    kerma grid (2 cm voxels over the phantom) between the GPU run and the FP64 oracle run."""
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.scene import validate_scene
+    from chatcarlo_amd.source import SourceRecord
+    from chatcarlo_amd.workloads import ct_helical
+    raw, _ = ct_helical()
+    sc = validate_scene(raw)
+    assert sc.ok, sc.errors
+    geo = Geometry(sc.raw["geometry"])
+    origin, shape, vox = np.array([-18.0, -18.0, -10.0]), (18, 18, 10), 2.0
+    engine.set_geometry(geo)
+    engine.bind_grid(origin, shape, vox, True)
+    rec = SourceRecord(sc.raw["source"])
+    n_gpu = 1_000_000
+    out = engine.run(rec, n_gpu, 4242, 0, 10_000, N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY)
+    g = engine.read_grid(shape)
+    prims, world = geo.packed(dpack.mat_id)
+    og = core.OGeom.from_packed(prims, world)
+    grid = OT.Grid(origin, shape, vox)
+    n_or = 60_000
+    OT.run(sc.raw["source"], og, dpack, rec.spec_e, rec.spec_cdf, n_or, 17, batch_size=1_200, grid=grid)
+
+    def mean_var(T, Q, M, Nh):
+        return T / Nh, core.batch_variance(T, Q, M, Nh) / Nh
+
+    m1, v1 = mean_var(g["kerma"], g["kerma_sum2"], g["n_batches"], g["n_histories"])
+    m2, v2 = mean_var(grid.kerma, grid.kerma_q, grid.n_batches, grid.n_histories)
+    r1 = np.sqrt(v1) / np.where(m1 > 0, m1, np.inf)
+    r2 = np.sqrt(v2) / np.where(m2 > 0, m2, np.inf)
+    sel = (m1 > 0) & (m2 > 0) & (r1 < 0.1) & (r2 < 0.1)
+    assert sel.sum() > 300
+    z = (m1[sel] - m2[sel]) / np.sqrt(v1[sel] + v2[sel])
+    chi2, dof = float(np.sum(z ** 2)), int(sel.sum())
+    assert abs(chi2 / dof - 1.0) < 3.0 * np.sqrt(2.0 / dof) + 0.05, (chi2, dof)
+    assert np.max(np.abs(z)) < 5.5
+    assert out.error_bits == 0
