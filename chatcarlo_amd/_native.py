@@ -29,6 +29,7 @@ ip = C.POINTER(C.c_int32)
 i64p = C.POINTER(C.c_int64)
 u32p = C.POINTER(C.c_uint32)
 u8p = C.POINTER(C.c_uint8)
+u64p = C.POINTER(C.c_uint64)
 
 
 class CcPrim(C.Structure):
@@ -94,6 +95,18 @@ class CcImportanceCfg(C.Structure):
                 ("split_gain", C.c_float), ("pad", C.c_uint32), ("seed", C.c_uint64)]
 
 
+class CcBatchTuning(C.Structure):
+    _fields_ = [("block", C.c_int32), ("rounds_per_sync", C.c_int32), ("pool", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class CcBatchOut(C.Structure):
+    _fields_ = [("n_histories", C.c_uint64), ("n_absorbed", C.c_uint64), ("n_escaped", C.c_uint64),
+                ("scatter_sum", C.c_uint64), ("n_fluorescence", C.c_uint64), ("n_events", C.c_uint64),
+                ("n_flights", C.c_uint64), ("n_voxel_visits", C.c_uint64), ("rounds", C.c_uint64),
+                ("graph_launches", C.c_uint64), ("device_ms", C.c_double), ("error_bits", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
 _SIGS = {
     "cc_last_error": (C.c_char_p, []),
     "cc_version": (C.c_int, [ip, ip]),
@@ -133,6 +146,13 @@ _SIGS = {
     "cc_mlp_grad": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, fp, fp, C.c_float, fp, dp]),
     "cc_mlp_train": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(CcTrainCfg), C.POINTER(CcTrainOut), dp]),
     "cc_set_importance": (C.c_int, [C.c_void_p, C.POINTER(CcImportanceCfg), C.POINTER(CcSource), fp, fp]),
+    "cc_batch_upload": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(CcPrim), ip, C.POINTER(CcWorld),
+                                  C.POINTER(CcSource), u64p, u64p, dp, ip, dp]),
+    "cc_batch_run": (C.c_int, [C.c_void_p, C.POINTER(CcBatchTuning), C.c_double, C.c_uint32,
+                               C.POINTER(CcBatchOut)]),
+    "cc_batch_read": (C.c_int, [C.c_void_p, C.c_int32, dp, dp, dp]),
+    "cc_batch_autotune": (C.c_int, [C.c_void_p, C.c_double, C.c_uint32, C.POINTER(CcBatchTuning), dp,
+                                    C.c_int32, ip]),
 }
 EXPORTS = tuple(_SIGS)
 

@@ -561,13 +561,17 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 // the focal spot -- issues one pair of atomics per voxel instead of one per
 // lane.  Every lane of the wave must call this (converged control flow).
 constexpr int kAggRounds = 4;
-__device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t v, double ak, double ah) {
+// kScene: lanes may score different grids (batched runs); only lanes of one
+// scene (key) and one voxel are combined.
+template <bool kScene>
+__device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t v, double ak, double ah, uint32_t key) {
   const int lane = __lane_id();
   unsigned long long pend = __ballot(has);
   for (int r = 0; r < kAggRounds && pend; ++r) {
     const int leader = __ffsll((long long)pend) - 1;
     const uint32_t vl = __shfl(v, leader);
-    const bool mine = has && v == vl;
+    const uint32_t kl = kScene ? (uint32_t)__shfl((int)key, leader) : 0u;
+    const bool mine = has && v == vl && (!kScene || key == kl);
     const unsigned long long m = __ballot(mine);
     if (m == (1ull << leader)) break;          // no partner: the rest go direct
     const double sk = wave_sum_f64(mine ? ak : 0.0);
@@ -589,8 +593,9 @@ __device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t 
 // o + t d, t in [0, L], overlaps, for lanes with seg = true.  Exact 3-D DDA
 // (Amanatides & Woo 1987); all lanes of the wave must call it.  Returns the
 // number of voxel visits of the calling lane.
+template <bool kScene = false>
 __device__ __forceinline__ int dda_score(const GridDev& G, const float o[3], const float d[3], float L,
-                                         double wk, double wh, bool seg) {
+                                         double wk, double wh, bool seg, uint32_t key = 0) {
   const int n[3] = {G.nx, G.ny, G.nz};
   float t0 = 0.0f, t1 = L;
   if (seg) {
@@ -652,7 +657,7 @@ __device__ __forceinline__ int dda_score(const GridDev& G, const float o[3], con
         if (idx[a] < 0 || idx[a] >= n[a]) act = false;
       }
     }
-    wave_tally(G, has, v, ak, ah);
+    wave_tally<kScene>(G, has, v, ak, ah, key);
   }
   return visits;
 }

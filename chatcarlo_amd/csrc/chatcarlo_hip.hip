@@ -64,6 +64,7 @@ struct PState {
   float *x, *y, *z, *dx, *dy, *dz, *e, *tau, *wt, *score;
   uint32_t *ctr, *hid, *tag;   // Philox event counter, history offset, lane tag (split copies)
   int *nsc, *mat, *flags;
+  int* sc;                     // scene of the photon (batched runs)
 };
 
 struct Src {
@@ -128,6 +129,24 @@ __device__ __forceinline__ void put_record(const Records& R, int kind, uint32_t 
   r[9] = __int_as_float(slot); r[10] = __uint_as_float(batch); r[11] = (float)kind;
 }
 
+// Batched runs: many scenes in one particle pool.  Histories of all scenes form
+// one index space [0, total) (scene s owns [hist_start[s], hist_start[s+1]));
+// a device cursor hands them out in order to free pool slots.
+struct MultiScene {
+  const Prim* prims;          // all scenes' primitives, concatenated
+  const int* prim_off;        // [n + 1]
+  const World* worlds;        // [n]
+  const GridDev* grids;       // [n]
+  const Src* srcs;            // [n]
+  const uint64_t* seeds;      // [n]
+  const uint64_t* hist_start; // [n + 1]
+  int n;
+  int* free_stack;            // free pool slots
+  uint32_t* free_cnt;
+  unsigned long long* cursor; // next history of the index space
+  unsigned long long total;
+};
+
 struct Launch {
   Tables T;
   const Prim* prims;
@@ -144,7 +163,14 @@ struct Launch {
   Region R;
   Records Rec;
   uint32_t rec_batch;      // batch index written into contribution records
+  int multi;               // batched run: scene data from MS, per-scene deposits (edep[scene][mat])
+  MultiScene MS;
 };
+
+// Philox key of a photon's history: the run seed, or its scene's seed in a batched run.
+__device__ __forceinline__ uint64_t photon_seed(const Launch& L, const PState& S, int i) {
+  return L.multi ? L.MS.seeds[S.sc[i]] : L.seed;
+}
 
 // ---------------------------------------------------------------------------
 // wave helpers
@@ -346,6 +372,82 @@ __global__ void k_features(MlpFrame F, int n, const float* pos, const float* dir
   for (int k = 0; k < kMlpIn; ++k) feat[(size_t)t * kMlpIn + k] = x[k];
 }
 
+// ---------------------------------------------------------------------------
+// batched runs: refill free slots from the device work queue of histories
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t refill_count(const MultiScene& M, uint32_t limit) {
+  const unsigned long long left = M.total - *M.cursor;
+  const uint32_t fc = *M.free_cnt;
+  return (uint32_t)min((unsigned long long)min(fc, limit), left);
+}
+
+// Starts the next histories of the index space in free pool slots and appends
+// them to the active list.  Scene s's history h draws from the stream keyed by
+// (seed_s, h), exactly as history h of a single-scene run with seed_s.
+__global__ __launch_bounds__(kBlock) void k_refill(Launch L, int* act, uint32_t* cnt_act, uint32_t limit) {
+  const MultiScene& M = L.MS;
+  const unsigned long long cur = *M.cursor;
+  const uint32_t fc = *M.free_cnt;
+  const uint32_t n_new = refill_count(M, limit);
+  PState S = L.S;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n_new; base += gridDim.x * blockDim.x) {
+    const uint32_t t = base + threadIdx.x;
+    const bool valid = t < n_new;
+    int q = 0;
+    if (valid) {
+      const unsigned long long g = cur + t;
+      int lo = 0, hi = M.n - 1;   // last scene whose first history is <= g
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (M.hist_start[mid] <= g) lo = mid; else hi = mid - 1;
+      }
+      const unsigned long long h = g - M.hist_start[lo];
+      q = M.free_stack[fc - 1 - t];
+      PhiloxRng rng(M.seeds[lo], (uint64_t)h, 0u, 0u);
+      float p[3], d[3], E;
+      sample_source(M.srcs[lo], rng, p, d, E);
+      const float tau = -logf(rng.next());
+      S.x[q] = p[0]; S.y[q] = p[1]; S.z[q] = p[2];
+      S.dx[q] = d[0]; S.dy[q] = d[1]; S.dz[q] = d[2];
+      S.e[q] = E; S.tau[q] = tau; S.wt[q] = 1.0f; S.score[q] = 0.f;
+      S.ctr[q] = rng.ctr; S.hid[q] = (uint32_t)h; S.tag[q] = 0u; S.nsc[q] = 0; S.mat[q] = -1; S.flags[q] = 0;
+      S.sc[q] = lo;
+    }
+    const int k = wave_append(valid, cnt_act);
+    if (k >= 0) act[k] = q;
+  }
+}
+
+// Advances the queue past the histories k_refill started (same count: nothing
+// else touches the counters between the two launches).
+__global__ void k_refill_commit(Launch L, uint32_t limit) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const MultiScene& M = L.MS;
+    const uint32_t n_new = refill_count(M, limit);
+    *M.free_cnt -= n_new;
+    *M.cursor += n_new;
+  }
+}
+
+// Queue position and active count for the host's termination check.
+__global__ void k_batch_status(Launch L, const uint32_t* cnt_act, unsigned long long* out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    out[0] = *L.MS.cursor;
+    out[1] = *cnt_act;
+  }
+}
+
+__global__ void k_batch_init(uint32_t* free_cnt, uint32_t n_free, unsigned long long* cursor) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *free_cnt = n_free;
+    *cursor = 0ull;
+  }
+}
+
+__global__ void k_iota(int* p, uint32_t n) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = (int)k;
+}
+
 // caller-supplied photons: only tau is drawn (chatcarlo/transport.py:124)
 __global__ __launch_bounds__(kBlock) void k_init_given(Launch L, int n, int* act) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -452,11 +554,13 @@ __device__ float region_expect(const Prim* sp, const World& W, const Tables& T, 
 // drawn after the collision's weight window); otherwise to the type queues.
 // kRec: contribution-record runs (expected flight scores, collision records), a
 // separate instantiation so the plain transport keeps its register budget.
-template <bool kRec>
+// kMulti: batched run; every photon reads its scene's primitives (from global
+// memory, cached), world box, grid and seed, and a photon that leaves frees its slot.
+template <bool kRec, bool kMulti>
 __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, const uint32_t* cnt_in, int* qp,
                                                    int* qc, int* qr, int* coll) {
-  __shared__ Prim sp[CC_MAX_PRIMS];
-  stage_prims(L.prims, sp, L.W.n_prim);
+  __shared__ Prim sp_lds[kMulti ? 1 : CC_MAX_PRIMS];
+  if (!kMulti) stage_prims(L.prims, sp_lds, L.W.n_prim);
   const uint32_t n = *cnt_in;
   const Tables& T = L.T;
   PState S = L.S;
@@ -472,10 +576,19 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     float p[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 1.f};
     float E = 1.0f, tau = 0.f, w = 0.f, mu = 0.f, muen = 0.f, hreg = 0.f, sreg = 0.f;
     double wh = 0.0;
-    int mc = -1, it = 0;
+    int mc = -1, it = 0, scene = 0;
     EPos ep{0, 0.f, 0.f};
+    const Prim* sp = sp_lds;
+    World Wd = L.W;
+    GridDev G = L.G;
     if (valid) {
       i = act[j];
+      if (kMulti) {
+        scene = S.sc[i];
+        sp = L.MS.prims + L.MS.prim_off[scene];
+        Wd = L.MS.worlds[scene];
+        G = L.MS.grids[scene];
+      }
       p[0] = S.x[i]; p[1] = S.y[i]; p[2] = S.z[i];
       d[0] = S.dx[i]; d[1] = S.dy[i]; d[2] = S.dz[i];
       E = S.e[i];
@@ -501,14 +614,14 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     const float eps0 = 1e-6f + 1.2e-7f * fmaxf(fabsf(p[0]), fmaxf(fabsf(p[1]), fabsf(p[2])));
     float tc = 0.f;
     float sexp = 0.f;   // expected region score of this flight (contribution records)
-    if (kRec && alive) sexp = hreg * region_expect(sp, L.W, T, L.R, o0, d, E, ep, eps0);
+    if (kRec && alive) sexp = hreg * region_expect(sp, Wd, T, L.R, o0, d, E, ep, eps0);
     while (__ballot(alive)) {
       float ds = 0.f, taub = 0.f, t_next = 0.f;
       bool inter = false, esc = false, seg = false;
       int m = 0;
       if (alive) {
         ++events;
-        t_next = next_boundary(sp, L.W.n_prim, L.W, o0, d, it == 0 ? eps0 : tc, esc);
+        t_next = next_boundary(sp, Wd.n_prim, Wd, o0, d, it == 0 ? eps0 : tc, esc);
         if (!(t_next < INFINITY) || t_next < 0.f) {   // outside the world box
           escaped = true;
           alive = false;
@@ -518,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
 #pragma unroll
           for (int k = 0; k < 3; ++k) p[k] = o0[k] + d[k] * tc;
           const float mid[3] = {o0[0] + d[0] * th, o0[1] + d[1] * th, o0[2] + d[2] * th};
-          m = locate(sp, L.W.n_prim, L.W.background, mid);
+          m = locate(sp, Wd.n_prim, Wd.background, mid);
           if (m != mc) {
             mc = m;
             const float rho = T.density[m];
@@ -531,7 +644,8 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
           seg = L.grid_on != 0;
         }
       }
-      if (L.grid_on) visits += dda_score(L.G, p, d, ds, (double)w * (double)E * (double)muen, wh, seg);
+      if (L.grid_on)
+        visits += dda_score<kMulti>(G, p, d, ds, (double)w * (double)E * (double)muen, wh, seg, (uint32_t)scene);
       if (alive && L.R.on) {
         // chord of the segment [0, ds] through the scoring sphere
         const float ox = p[0] - L.R.c[0], oy = p[1] - L.R.c[1], oz = p[2] - L.R.c[2];
@@ -550,7 +664,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
           if (coll) {
             type = 3;
           } else {
-            PhiloxRng rng(L.seed, L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
+            PhiloxRng rng(photon_seed(L, S, i), L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
             type = sample_type(T, m, E, ep, rng.next());
             S.ctr[i] = rng.ctr;
           }
@@ -585,6 +699,10 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
       if (take) put_record(L.Rec, 1, (uint32_t)k, p, d, E, S.score[i], i, L.rec_batch);
     }
     if (sreg > 0.f) atomic_add_f64(L.hscore + S.hid[i], (double)sreg);
+    if (kMulti) {   // a photon that does not interact (escaped, or stopped by an error) frees its slot
+      const int k = wave_append(valid && type < 0, L.MS.free_cnt);
+      if (k >= 0) L.MS.free_stack[k] = i;
+    }
     wave_count(valid && escaped, L.stats + S_ESC);
     wave_count(valid, L.stats + S_FLIGHT);
     wave_sum_u64(events, L.stats + S_EVT);
@@ -620,7 +738,7 @@ __global__ __launch_bounds__(kBlock) void k_photo(Launch L, const int* q, const 
       m = S.mat[i];
       const float E = S.e[i];
       const float w = S.wt[i];
-      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
+      PhiloxRng rng(photon_seed(L, S, i), L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
       float eline = 0.f;
       if (L.fluor) {
         const EPos ep = energy_pos(T, E);
@@ -640,12 +758,16 @@ __global__ __launch_bounds__(kBlock) void k_photo(Launch L, const int* q, const 
       }
       S.ctr[i] = rng.ctr;
     }
-    wave_add_by_key(valid, m, dep, L.edep);
+    wave_add_by_key(valid, L.multi && valid ? S.sc[i] * CC_MAX_MAT + m : m, dep, L.edep);
     wave_count(valid && !emit, L.stats + S_ABS);
     wave_count(valid && emit, L.stats + S_FLUOR);
     wave_count(valid, L.stats + S_SCAT);
     const int slot = wave_append(valid && emit, cnt_next);
     if (slot >= 0) act_next[slot] = i;
+    if (L.multi) {   // absorbed photons free their slot
+      const int k = wave_append(valid && !emit, L.MS.free_cnt);
+      if (k >= 0) L.MS.free_stack[k] = i;
+    }
   }
 }
 
@@ -664,7 +786,7 @@ __global__ __launch_bounds__(kBlock) void k_compton(Launch L, const int* q, cons
       m = S.mat[i];
       const float E = S.e[i];
       const float w = S.wt[i];
-      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
+      PhiloxRng rng(photon_seed(L, S, i), L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
       const EPos ep = energy_pos(T, E);
       float cs, sn;
       int el, trials;
@@ -680,7 +802,7 @@ __global__ __launch_bounds__(kBlock) void k_compton(Launch L, const int* q, cons
       S.nsc[i] += 1;
       S.ctr[i] = rng.ctr;
     }
-    wave_add_by_key(valid, m, dep, L.edep);
+    wave_add_by_key(valid, L.multi && valid ? S.sc[i] * CC_MAX_MAT + m : m, dep, L.edep);
     wave_count(valid, L.stats + S_SCAT);
     const int slot = wave_append(valid, cnt_next);
     if (slot >= 0) act_next[slot] = i;
@@ -700,7 +822,7 @@ __global__ __launch_bounds__(kBlock) void k_rayleigh(Launch L, const int* q, con
       i = q[j];
       const int m = S.mat[i];
       const float E = S.e[i];
-      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
+      PhiloxRng rng(photon_seed(L, S, i), L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
       const EPos ep = energy_pos(T, E);
       int el, trials;
       bool qerr;
@@ -1623,7 +1745,44 @@ struct Ctx {
   float *bcdf = nullptr, *bwt = nullptr;
   uint32_t* stage = nullptr;   // collision-split staging
   size_t stage_words = 0;
+  // batched runs (many scenes in one pool)
+  int nscene = 0;
+  std::vector<uint64_t> scene_hist;   // histories per scene at fraction 1
+  Prim* b_prims = nullptr;
+  int* b_prim_off = nullptr;
+  World* b_worlds = nullptr;
+  GridDev* b_grids = nullptr;
+  Src* b_srcs = nullptr;
+  uint64_t* b_seeds = nullptr;
+  uint64_t* b_hist_start = nullptr;
+  float* b_spec = nullptr;
+  double* b_tally = nullptr;           // all scenes' kerma then H*(10) totals
+  size_t b_nvox = 0;
+  std::vector<size_t> b_vox_off;       // first voxel of each scene
+  std::vector<GridDev> b_grid_host;
+  double* b_edep = nullptr;            // [scene][CC_MAX_MAT]
+  int* b_free = nullptr;
+  size_t b_free_cap = 0;
+  uint32_t* b_free_cnt = nullptr;
+  unsigned long long* b_cursor = nullptr;
+  unsigned long long* b_status = nullptr;
+  unsigned long long* h_status = nullptr;   // pinned
 };
+
+void free_batch(Ctx* c) {
+  void* ps[] = {c->b_prims, c->b_prim_off, c->b_worlds, c->b_grids, c->b_srcs, c->b_seeds, c->b_hist_start,
+                c->b_spec, c->b_tally, c->b_edep};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  c->b_prims = nullptr; c->b_prim_off = nullptr; c->b_worlds = nullptr; c->b_grids = nullptr;
+  c->b_srcs = nullptr; c->b_seeds = nullptr; c->b_hist_start = nullptr; c->b_spec = nullptr;
+  c->b_tally = nullptr; c->b_edep = nullptr;
+  c->nscene = 0;
+  c->b_nvox = 0;
+  c->b_vox_off.clear();
+  c->b_grid_host.clear();
+  c->scene_hist.clear();
+}
 
 // Feature frame without a scoring region: world-box centre, half the largest extent.
 void set_default_frame(Ctx* c) {
@@ -1638,7 +1797,7 @@ void set_default_frame(Ctx* c) {
 void free_pool(Ctx* c) {
   void* ps[] = {c->S.x, c->S.y, c->S.z, c->S.dx, c->S.dy, c->S.dz, c->S.e, c->S.tau, c->S.wt, c->S.score,
                 c->S.ctr, c->S.hid, c->S.tag, c->S.nsc, c->S.mat, c->S.flags, c->act0, c->act1, c->act2,
-                c->qp, c->qc, c->qr, c->hscore, c->coll};
+                c->qp, c->qc, c->qr, c->hscore, c->coll, c->S.sc};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   c->S = PState{};
@@ -1658,7 +1817,8 @@ int ensure_pool(Ctx* c, size_t n) {
       (r = dalloc(&c->S.flags, n)) || (r = dalloc(&c->act0, n)) || (r = dalloc(&c->act1, n)) ||
       (r = dalloc(&c->qp, n)) || (r = dalloc(&c->qc, n)) || (r = dalloc(&c->qr, n)) ||
       (r = dalloc(&c->S.score, n)) || (r = dalloc(&c->S.hid, n)) || (r = dalloc(&c->S.tag, n)) ||
-      (r = dalloc(&c->act2, n)) || (r = dalloc(&c->hscore, n)) || (r = dalloc(&c->coll, n)))
+      (r = dalloc(&c->act2, n)) || (r = dalloc(&c->hscore, n)) || (r = dalloc(&c->coll, n)) ||
+      (r = dalloc(&c->S.sc, n)))
     return r;
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(n)), dim3(kBlock), 0, c->stream, c->hscore, n, 0.0);
   HIP_TRY(hipGetLastError());
@@ -1676,7 +1836,7 @@ int upload(Ctx* c, const T* host, size_t n, T** dev) {
 }
 
 Launch make_launch(Ctx* c, uint64_t seed, uint64_t hist0, int grid_on, int fluor) {
-  Launch L;
+  Launch L{};   // value-initialised: multi = 0, no region, no records unless set below
   L.T = c->T;
   L.prims = c->prims;
   L.W = c->W;
@@ -1787,11 +1947,11 @@ int run_rounds(Ctx* c, const Launch& L, uint32_t n, Timer& tm, uint32_t* rounds_
       }
       tm.begin(K_FLIGHT);
       if (o.record)
-        hipLaunchKernelGGL(k_flight<true>, dim3(nb), dim3(kBlock), 0, c->stream, L, fl, fc, c->qp, c->qc, c->qr,
-                           (int*)nullptr);
+        hipLaunchKernelGGL((k_flight<true, false>), dim3(nb), dim3(kBlock), 0, c->stream, L, fl, fc, c->qp, c->qc,
+                           c->qr, (int*)nullptr);
       else
-        hipLaunchKernelGGL(k_flight<false>, dim3(nb), dim3(kBlock), 0, c->stream, L, fl, fc, c->qp, c->qc, c->qr,
-                           o.pre ? c->coll : (int*)nullptr);
+        hipLaunchKernelGGL((k_flight<false, false>), dim3(nb), dim3(kBlock), 0, c->stream, L, fl, fc, c->qp, c->qc,
+                           c->qr, o.pre ? c->coll : (int*)nullptr);
       tm.end();
       if (o.pre) {
         tm.begin(K_OTHER);
@@ -1890,27 +2050,23 @@ struct DevBufs {
   }
 };
 
-int make_src_impl(Ctx* c, const cc_source* s, Src* d) {
+int check_src(const cc_source* s) {
   if (s->n_spec <= 0 || !s->spec_e || !s->spec_cdf) return fail(CC_ERR_ARG, "empty spectrum");
   for (int i = 0; i < s->n_spec; ++i)
     if (!(s->spec_e[i] >= 1.0 && s->spec_e[i] <= 150.0))
       return fail(CC_ERR_ARG, "spectrum energy %g keV outside [1, 150] keV", s->spec_e[i]);
   if (s->field < 0 || s->field > 2) return fail(CC_ERR_ARG, "field kind %d", s->field);
   if (s->rotation == 2 && s->n_angles < 2) return fail(CC_ERR_ARG, "n_angles must be >= 2");
-  if (2 * s->n_spec > c->spec_cap) {
-    if (c->spec) (void)hipFree(c->spec);
-    c->spec = nullptr;
-    int r = dalloc(&c->spec, 2 * (size_t)s->n_spec);
-    if (r) return r;
-    c->spec_cap = 2 * s->n_spec;
-  }
-  std::vector<float> h(2 * (size_t)s->n_spec);
+  return CC_OK;
+}
+
+// Source fields; the spectrum (energies then CDF, last CDF value 1) goes to h at [off, off + 2 n).
+void fill_src(const cc_source* s, Src* d, std::vector<float>& h, size_t off) {
   for (int i = 0; i < s->n_spec; ++i) {
-    h[i] = (float)s->spec_e[i];
-    h[s->n_spec + i] = (float)s->spec_cdf[i];
+    h[off + i] = (float)s->spec_e[i];
+    h[off + s->n_spec + i] = (float)s->spec_cdf[i];
   }
-  h[2 * s->n_spec - 1] = 1.0f;
-  HIP_TRY(hipMemcpy(c->spec, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  h[off + 2 * s->n_spec - 1] = 1.0f;
   d->field = s->field; d->n_spec = s->n_spec; d->n_angles = s->n_angles; d->rotation = s->rotation;
   for (int k = 0; k < 3; ++k) {
     d->pos[k] = (float)s->pos[k]; d->dir[k] = (float)s->dir[k];
@@ -1920,6 +2076,20 @@ int make_src_impl(Ctx* c, const cc_source* s, Src* d) {
   d->sid = (float)s->sid; d->width = (float)s->width; d->height = (float)s->height;
   d->cos_half = (float)s->cos_half; d->omch = (float)(1.0 - s->cos_half);
   d->scan = (float)s->scan_length;
+}
+
+int make_src_impl(Ctx* c, const cc_source* s, Src* d) {
+  int r = check_src(s);
+  if (r) return r;
+  if (2 * s->n_spec > c->spec_cap) {
+    if (c->spec) (void)hipFree(c->spec);
+    c->spec = nullptr;
+    if ((r = dalloc(&c->spec, 2 * (size_t)s->n_spec))) return r;
+    c->spec_cap = 2 * s->n_spec;
+  }
+  std::vector<float> h(2 * (size_t)s->n_spec);
+  fill_src(s, d, h, 0);
+  HIP_TRY(hipMemcpy(c->spec, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
   d->spec_e = c->spec;
   d->spec_cdf = c->spec + s->n_spec;
   return CC_OK;
@@ -1983,6 +2153,11 @@ int cc_ctx_destroy(void* ctx) {
                 c->mw[1], c->mm[1], c->mv[1], c->mgrad[1], c->mwb[1], c->bcdf, c->bwt, c->stage};
   for (void* p : ps)
     if (p) (void)hipFree(p);
+  free_batch(c);
+  void* bs[] = {c->b_free, c->b_free_cnt, c->b_cursor, c->b_status};
+  for (void* p : bs)
+    if (p) (void)hipFree(p);
+  if (c->h_status) (void)hipHostFree(c->h_status);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
   for (auto e : c->ev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -2878,6 +3053,268 @@ int cc_set_importance(void* ctx, const cc_importance_cfg* cfg, const cc_source* 
   c->have_imp = true;
   c->have_bias = cfg->source_bias != 0;
   if (cw_out) *cw_out = cw;
+  return CC_OK;
+}
+
+// ---- batched runs -----------------------------------------------------------
+int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int32_t* prim_off,
+                    const cc_world* worlds, const cc_source* sources, const uint64_t* n_histories,
+                    const uint64_t* seeds, const double* grid_origin, const int32_t* grid_shape,
+                    const double* voxel_cm) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c || !prim_off || !worlds || !sources || !n_histories || !seeds || !grid_origin || !grid_shape || !voxel_cm)
+    return fail(CC_ERR_ARG, "null argument");
+  if (n_scenes <= 0 || n_scenes > 4096) return fail(CC_ERR_ARG, "n_scenes %d out of range", n_scenes);
+  if (!c->have_tables) return fail(CC_ERR_STATE, "upload tables before the scenes");
+  if (prim_off[0] != 0) return fail(CC_ERR_ARG, "prim_off[0] must be 0");
+  size_t nvox = 0;
+  std::vector<size_t> voff(n_scenes);
+  for (int s = 0; s < n_scenes; ++s) {
+    const int np = prim_off[s + 1] - prim_off[s];
+    if (np < 0 || np > CC_MAX_PRIMS) return fail(CC_ERR_ARG, "scene %d: %d primitives", s, np);
+    if (np > 0 && !prims) return fail(CC_ERR_ARG, "null primitives");
+    for (int i = prim_off[s]; i < prim_off[s + 1]; ++i) {
+      if (prims[i].material < 0 || prims[i].material >= c->T.n_mat)
+        return fail(CC_ERR_ARG, "scene %d primitive %d: material id %d", s, i, prims[i].material);
+      if (prims[i].kind < 0 || prims[i].kind > 2 || prims[i].axis < 0 || prims[i].axis > 2)
+        return fail(CC_ERR_ARG, "scene %d primitive %d: kind/axis", s, i);
+    }
+    if (worlds[s].background < 0 || worlds[s].background >= c->T.n_mat)
+      return fail(CC_ERR_ARG, "scene %d: background id", s);
+    if (n_histories[s] == 0 || n_histories[s] > (1ull << 32)) return fail(CC_ERR_ARG, "scene %d: histories", s);
+    if (grid_shape[3 * s] <= 0 || grid_shape[3 * s + 1] <= 0 || grid_shape[3 * s + 2] <= 0 || !(voxel_cm[s] > 0))
+      return fail(CC_ERR_ARG, "scene %d: grid", s);
+    int r = check_src(&sources[s]);
+    if (r) return r;
+    voff[s] = nvox;
+    nvox += (size_t)grid_shape[3 * s] * grid_shape[3 * s + 1] * grid_shape[3 * s + 2];
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  free_batch(c);
+  const int total_prims = prim_off[n_scenes];
+  size_t nspec = 0;
+  for (int s = 0; s < n_scenes; ++s) nspec += 2 * (size_t)sources[s].n_spec;
+  int r;
+  if ((r = dalloc(&c->b_prims, std::max(total_prims, 1))) || (r = dalloc(&c->b_prim_off, n_scenes + 1)) ||
+      (r = dalloc(&c->b_worlds, n_scenes)) || (r = dalloc(&c->b_grids, n_scenes)) ||
+      (r = dalloc(&c->b_srcs, n_scenes)) || (r = dalloc(&c->b_seeds, n_scenes)) ||
+      (r = dalloc(&c->b_hist_start, n_scenes + 1)) || (r = dalloc(&c->b_spec, nspec)) ||
+      (r = dalloc(&c->b_tally, 2 * nvox)) || (r = dalloc(&c->b_edep, (size_t)n_scenes * CC_MAX_MAT)))
+    return r;
+  if (total_prims > 0)
+    HIP_TRY(hipMemcpy(c->b_prims, prims, sizeof(Prim) * total_prims, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->b_prim_off, prim_off, sizeof(int) * (n_scenes + 1), hipMemcpyHostToDevice));
+  std::vector<World> w(n_scenes);
+  for (int s = 0; s < n_scenes; ++s) {
+    std::memcpy(&w[s], &worlds[s], sizeof(World));
+    w[s].n_prim = prim_off[s + 1] - prim_off[s];
+  }
+  HIP_TRY(hipMemcpy(c->b_worlds, w.data(), sizeof(World) * n_scenes, hipMemcpyHostToDevice));
+  std::vector<float> hs(std::max<size_t>(nspec, 1));
+  std::vector<Src> src(n_scenes);
+  size_t so = 0;
+  for (int s = 0; s < n_scenes; ++s) {
+    fill_src(&sources[s], &src[s], hs, so);
+    src[s].spec_e = c->b_spec + so;
+    src[s].spec_cdf = c->b_spec + so + sources[s].n_spec;
+    so += 2 * (size_t)sources[s].n_spec;
+  }
+  HIP_TRY(hipMemcpy(c->b_spec, hs.data(), sizeof(float) * nspec, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->b_srcs, src.data(), sizeof(Src) * n_scenes, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->b_seeds, seeds, sizeof(uint64_t) * n_scenes, hipMemcpyHostToDevice));
+  c->b_grid_host.resize(n_scenes);
+  for (int s = 0; s < n_scenes; ++s) {
+    GridDev& G = c->b_grid_host[s];
+    G.dk = c->b_tally + voff[s];
+    G.dh = c->b_tally + nvox + voff[s];
+    G.nx = grid_shape[3 * s]; G.ny = grid_shape[3 * s + 1]; G.nz = grid_shape[3 * s + 2];
+    for (int k = 0; k < 3; ++k) G.org[k] = (float)grid_origin[3 * s + k];
+    G.h = (float)voxel_cm[s];
+    G.inv_h = (float)(1.0 / voxel_cm[s]);
+  }
+  HIP_TRY(hipMemcpy(c->b_grids, c->b_grid_host.data(), sizeof(GridDev) * n_scenes, hipMemcpyHostToDevice));
+  c->scene_hist.assign(n_histories, n_histories + n_scenes);
+  c->b_vox_off = voff;
+  c->b_nvox = nvox;
+  c->nscene = n_scenes;
+  return CC_OK;
+}
+
+int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32_t flags, cc_batch_out* out) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c || !tune || !out) return fail(CC_ERR_ARG, "null argument");
+  if (c->nscene <= 0) return fail(CC_ERR_STATE, "no scenes uploaded (cc_batch_upload)");
+  if (tune->block < 64 || tune->block > kBlock || tune->block % kWave)
+    return fail(CC_ERR_ARG, "block %d must be a multiple of 64 in [64, %d]", tune->block, kBlock);
+  if (tune->rounds_per_sync < 2 || tune->rounds_per_sync > 256 || tune->rounds_per_sync % 2)
+    return fail(CC_ERR_ARG, "rounds_per_sync must be even, 2..256");
+  if (tune->pool < 1024 || tune->pool > (1u << 28)) return fail(CC_ERR_ARG, "pool %u out of range", tune->pool);
+  if (!(fraction > 0.0 && fraction <= 1.0)) return fail(CC_ERR_ARG, "fraction must be in (0, 1]");
+  HIP_TRY(hipSetDevice(c->device));
+  std::memset(out, 0, sizeof(*out));
+  const int n = c->nscene;
+  std::vector<uint64_t> start(n + 1, 0);
+  for (int s = 0; s < n; ++s) {
+    const uint64_t m = std::max<uint64_t>(1, (uint64_t)std::ceil((double)c->scene_hist[s] * fraction));
+    start[s + 1] = start[s] + m;
+  }
+  const uint64_t total = start[n];
+  HIP_TRY(hipMemcpy(c->b_hist_start, start.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice));
+  const uint32_t pool = tune->pool;
+  int r = ensure_pool(c, pool);
+  if (r) return r;
+  if (c->b_free_cap < pool) {
+    if (c->b_free) (void)hipFree(c->b_free);
+    c->b_free = nullptr;
+    c->b_free_cap = 0;
+    if ((r = dalloc(&c->b_free, pool))) return r;
+    c->b_free_cap = pool;
+  }
+  if (!c->b_free_cnt && ((r = dalloc(&c->b_free_cnt, 1)) || (r = dalloc(&c->b_cursor, 1)) ||
+                         (r = dalloc(&c->b_status, 2))))
+    return r;
+  if (!c->h_status) HIP_TRY(hipHostMalloc((void**)&c->h_status, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+  // fresh tallies, deposits, counters, free slots and queue
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(2 * c->b_nvox)), dim3(kBlock), 0, c->stream, c->b_tally,
+                     2 * c->b_nvox, 0.0);
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks((size_t)n * CC_MAX_MAT)), dim3(kBlock), 0, c->stream, c->b_edep,
+                     (size_t)n * CC_MAX_MAT, 0.0);
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, c->stats, c->edep, (int)C_ACT_A, 0u);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->err, (size_t)1, 0);
+  hipLaunchKernelGGL(k_iota, dim3(grid_blocks(pool)), dim3(kBlock), 0, c->stream, c->b_free, pool);
+  hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, c->stream, c->b_free_cnt, pool, c->b_cursor);
+  HIP_TRY(hipGetLastError());
+  Launch L = make_launch(c, 0, 0, 1, (flags & CC_RUN_FLUORESCENCE) ? 1 : 0);
+  L.edep = c->b_edep;
+  L.multi = 1;
+  L.MS.prims = c->b_prims; L.MS.prim_off = c->b_prim_off; L.MS.worlds = c->b_worlds; L.MS.grids = c->b_grids;
+  L.MS.srcs = c->b_srcs; L.MS.seeds = c->b_seeds; L.MS.hist_start = c->b_hist_start; L.MS.n = n;
+  L.MS.free_stack = c->b_free; L.MS.free_cnt = c->b_free_cnt; L.MS.cursor = c->b_cursor;
+  L.MS.total = (unsigned long long)total;
+  const int bs = tune->block;
+  const int nb = (int)std::min<size_t>((pool + bs - 1) / bs, 8192);
+  // two rounds (list A -> B -> A) and the status copy, captured once and replayed
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  for (int half = 0; half < 2; ++half) {
+    const int cur = half == 0 ? C_ACT_A : C_ACT_B, nxt = half == 0 ? C_ACT_B : C_ACT_A;
+    int* act_cur = half == 0 ? c->act0 : c->act1;
+    int* act_nxt = half == 0 ? c->act1 : c->act0;
+    hipLaunchKernelGGL(k_round_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, nxt);
+    hipLaunchKernelGGL(k_refill, dim3(nb), dim3(bs), 0, c->stream, L, act_cur, c->cnt + cur, pool);
+    hipLaunchKernelGGL(k_refill_commit, dim3(1), dim3(64), 0, c->stream, L, pool);
+    hipLaunchKernelGGL((k_flight<false, true>), dim3(nb), dim3(bs), 0, c->stream, L, (const int*)act_cur,
+                       (const uint32_t*)(c->cnt + cur), c->qp, c->qc, c->qr, (int*)nullptr);
+    hipLaunchKernelGGL(k_photo, dim3(nb), dim3(bs), 0, c->stream, L, c->qp, c->cnt + C_QP, act_nxt, c->cnt + nxt);
+    hipLaunchKernelGGL(k_compton, dim3(nb), dim3(bs), 0, c->stream, L, c->qc, c->cnt + C_QC, act_nxt, c->cnt + nxt);
+    hipLaunchKernelGGL(k_rayleigh, dim3(nb), dim3(bs), 0, c->stream, L, c->qr, c->cnt + C_QR, act_nxt, c->cnt + nxt);
+  }
+  hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(64), 0, c->stream, L, (const uint32_t*)(c->cnt + C_ACT_A),
+                     c->b_status);
+  hipError_t ce = hipMemcpyAsync(c->h_status, c->b_status, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                 c->stream);
+  hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+  if (ce != hipSuccess || ee != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(CC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce != hipSuccess ? ce : ee));
+  }
+  hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return fail(CC_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, c->stream));
+  uint64_t launches = 0;
+  int rc = CC_OK;
+  const uint64_t max_launches = (uint64_t)kMaxRounds;
+  for (;;) {
+    for (int k = 0; k < tune->rounds_per_sync / 2; ++k) {
+      hipError_t le = hipGraphLaunch(exec, c->stream);
+      if (le != hipSuccess) { rc = fail(CC_ERR_HIP, "graph launch failed: %s", hipGetErrorString(le)); break; }
+      ++launches;
+    }
+    if (rc) break;
+    hipError_t se = hipStreamSynchronize(c->stream);
+    if (se != hipSuccess) { rc = fail(CC_ERR_HIP, "stream sync: %s", hipGetErrorString(se)); break; }
+    if (c->h_status[0] >= total && c->h_status[1] == 0) break;
+    if (launches >= max_launches) { rc = fail(CC_ERR_DEVICE, "batched run did not drain"); break; }
+  }
+  (void)hipEventRecord(e1, c->stream);
+  (void)hipStreamSynchronize(c->stream);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  if (rc) return rc;
+  unsigned long long st[S_NSTAT];
+  int e = 0;
+  HIP_TRY(hipMemcpy(st, c->stats, sizeof(st), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(&e, c->err, sizeof(int), hipMemcpyDeviceToHost));
+  out->n_histories = total;
+  out->n_absorbed = st[S_ABS];
+  out->n_escaped = st[S_ESC];
+  out->scatter_sum = st[S_SCAT];
+  out->n_fluorescence = st[S_FLUOR];
+  out->n_events = st[S_EVT];
+  out->n_flights = st[S_FLIGHT];
+  out->n_voxel_visits = st[S_VOX];
+  out->rounds = 2 * launches;
+  out->graph_launches = launches;
+  out->device_ms = ms;
+  out->error_bits = (uint32_t)e;
+  if (e) return fail(CC_ERR_DEVICE, "device error bits 0x%x", e);
+  return CC_OK;
+}
+
+int cc_batch_read(void* ctx, int32_t scene, double* kerma_keV, double* h10_pSv_cm3, double* edep_keV) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return fail(CC_ERR_ARG, "null ctx");
+  if (scene < 0 || scene >= c->nscene) return fail(CC_ERR_ARG, "scene %d out of range", scene);
+  HIP_TRY(hipSetDevice(c->device));
+  const GridDev& G = c->b_grid_host[scene];
+  const size_t nv = (size_t)G.nx * G.ny * G.nz;
+  if (kerma_keV) HIP_TRY(hipMemcpy(kerma_keV, G.dk, nv * sizeof(double), hipMemcpyDeviceToHost));
+  if (h10_pSv_cm3) HIP_TRY(hipMemcpy(h10_pSv_cm3, G.dh, nv * sizeof(double), hipMemcpyDeviceToHost));
+  if (edep_keV)
+    HIP_TRY(hipMemcpy(edep_keV, c->b_edep + (size_t)scene * CC_MAX_MAT, CC_MAX_MAT * sizeof(double),
+                      hipMemcpyDeviceToHost));
+  return CC_OK;
+}
+
+int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tuning* best, double* table,
+                      int32_t max_rows, int32_t* n_rows) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c || !best) return fail(CC_ERR_ARG, "null argument");
+  const int blocks[] = {128, 256};
+  const uint32_t pools[] = {1u << 18, 1u << 19, 1u << 20, 1u << 21};
+  const int rps[] = {4, 8, 16};
+  double best_rate = -1.0;
+  int rows = 0;
+  for (int b : blocks)
+    for (uint32_t p : pools)
+      for (int q : rps) {
+        cc_batch_tuning t{b, q, p, 0u};
+        cc_batch_out o;
+        int r = cc_batch_run(ctx, &t, fraction, flags, &o);
+        if (r) return r;
+        const double rate = o.device_ms > 0 ? (double)o.n_histories / (o.device_ms * 1e-3) : 0.0;
+        if (table && rows < max_rows) {
+          table[4 * rows] = b; table[4 * rows + 1] = q; table[4 * rows + 2] = p; table[4 * rows + 3] = rate;
+        }
+        ++rows;
+        if (rate > best_rate) {
+          best_rate = rate;
+          *best = t;
+        }
+      }
+  if (n_rows) *n_rows = rows;
   return CC_OK;
 }
 

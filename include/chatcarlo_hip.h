@@ -249,6 +249,41 @@ int cc_mlp_train(void* ctx, int32_t map, const cc_train_cfg* cfg, cc_train_out* 
 int cc_set_importance(void* ctx, const cc_importance_cfg* cfg, const cc_source* src, float* cw_out,
                       float* cell_prob);
 
+/* Batched runs (no counterpart in the reference, which runs one scene per call): many scenes in
+ * one particle pool.  Scene s's histories 0 .. n_s - 1 draw from the streams keyed by (seed_s, h),
+ * so each scene's tallies are those of cc_run(seed_s, first 0) up to FP64 summation order.  A
+ * device cursor hands histories to free pool slots; two transport rounds and the status copy
+ * are captured in one hipGraph and replayed until the queue drains.  Tallies: kerma and H*(10)
+ * totals per scene and per-material deposits (no batch moments). */
+typedef struct {
+  int32_t block;                /* threads per block of the transport kernels (64, 128, 192 or 256) */
+  int32_t rounds_per_sync;      /* rounds between host checks of the queue (even) */
+  uint32_t pool;                /* photons in flight */
+  uint32_t pad;
+} cc_batch_tuning;
+
+typedef struct {
+  uint64_t n_histories, n_absorbed, n_escaped, scatter_sum, n_fluorescence;
+  uint64_t n_events, n_flights, n_voxel_visits;
+  uint64_t rounds, graph_launches;
+  double device_ms;             /* from the first replay to the drained queue */
+  uint32_t error_bits, pad;
+} cc_batch_out;
+
+/* prims of all scenes concatenated, scene s owning [prim_off[s], prim_off[s+1]); grid_origin and
+ * grid_shape are n_scenes x 3. */
+int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int32_t* prim_off,
+                    const cc_world* worlds, const cc_source* sources, const uint64_t* n_histories,
+                    const uint64_t* seeds, const double* grid_origin, const int32_t* grid_shape,
+                    const double* voxel_cm);
+/* Runs ceil(fraction n_s) histories of every scene (fraction in (0, 1]); tallies start from zero. */
+int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32_t flags, cc_batch_out* out);
+int cc_batch_read(void* ctx, int32_t scene, double* kerma_keV, double* h10_pSv_cm3, double* edep_keV);
+/* Times every (block, rounds_per_sync, pool) candidate on a fraction of the histories and returns
+ * the fastest; table (4 doubles per row: block, rounds_per_sync, pool, histories/s) may be NULL. */
+int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tuning* best, double* table,
+                      int32_t max_rows, int32_t* n_rows);
+
 /* Source sampling for histories [first, first + n): pos, dir (n x 3), energy (n). */
 int cc_test_source(void* ctx, const cc_source* src, int64_t n, uint64_t seed, uint64_t first_history,
                    float* pos, float* dir, float* energy);

@@ -48,6 +48,11 @@ def test_ctypes_layouts_match_the_header(tmp_path):
               ("offsetof(cc_train_out, n_positive)", N.CcTrainOut.n_positive.offset),
               ("sizeof(cc_importance_cfg)", C.sizeof(N.CcImportanceCfg)),
               ("offsetof(cc_importance_cfg, seed)", N.CcImportanceCfg.seed.offset),
+              ("sizeof(cc_batch_tuning)", C.sizeof(N.CcBatchTuning)),
+              ("offsetof(cc_batch_tuning, pool)", N.CcBatchTuning.pool.offset),
+              ("sizeof(cc_batch_out)", C.sizeof(N.CcBatchOut)),
+              ("offsetof(cc_batch_out, device_ms)", N.CcBatchOut.device_ms.offset),
+              ("offsetof(cc_batch_out, error_bits)", N.CcBatchOut.error_bits.offset),
               ("CC_MLP_PARAMS", N.MLP_PARAMS), ("CC_RECORD_FLOATS", N.RECORD_FLOATS),
               ("CC_RUN_SOURCE_BIAS", N.RUN_SOURCE_BIAS), ("CC_RUN_IMPORTANCE", N.RUN_IMPORTANCE),
               ("CC_RUN_RECORD", N.RUN_RECORD), ("CC_RUN_REGION", N.RUN_REGION)]

@@ -1,0 +1,106 @@
+"""Batched transport (many scenes in one pool, hipGraph replay, device work queue).
+
+Every scene's history h uses the Philox stream keyed by (seed_s, h), exactly as
+history h of a single-scene run with seed_s, so a batched run must reproduce,
+scene by scene, the kerma and H*(10) grids and the deposits of the single-scene
+runs up to the order of FP64 additions, and the event counters exactly.  The
+launch shapes below include pools far smaller than the history count, so slots
+are recycled through the free-slot stack many times and the queue crosses scene
+boundaries inside one refill.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes(n_scenes=5, seed=11):
+    from chatcarlo_amd.batch import SceneSpec
+    from chatcarlo_amd.workloads import batched_scenes
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, (raw, grid, _, s) in enumerate(batched_scenes(n_scenes, master_seed=2026 + seed)):
+        vox = [10.0, 6.0, 15.0, 10.0, 12.0][k % 5]
+        shape = tuple(int(300 // vox) for _ in range(3))
+        out.append(SceneSpec(raw, (np.array([-150.0, -150.0, -150.0]), shape, vox),
+                             int(rng.integers(2000, 20000)), s))
+    return out
+
+
+@pytest.fixture(scope="module")
+def single(engine):
+    """Per-scene single-scene runs: grids, deposits, counters."""
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.source import SourceRecord
+    scenes = _scenes()
+    res = []
+    for s in scenes:
+        engine.set_geometry(Geometry(s.raw["geometry"]))
+        engine.bind_grid(s.grid[0], s.grid[1], s.grid[2], False)
+        out = engine.run(SourceRecord(s.raw["source"]), s.n_histories, s.seed, 0, 4096,
+                         N.RUN_FLUORESCENCE | N.RUN_GRID)
+        g = engine.read_grid(s.grid[1], moments=False)
+        res.append(dict(kerma=g["kerma"], h10=g["h10"], edep=np.array(out.edep_keV[:]),
+                        cnt=np.array([out.n_absorbed, out.n_escaped, out.scatter_sum, out.n_fluorescence,
+                                      out.n_events, out.n_flights, out.n_voxel_visits], dtype=np.int64)))
+    return scenes, res
+
+
+@pytest.mark.parametrize("tune", [(256, 8, 1 << 20), (128, 4, 1024), (64, 2, 4096), (192, 16, 3000)])
+def test_batched_equals_single_scene_runs(engine, single, tune):
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd.batch import BatchedRun, Tuning
+    scenes, ref = single
+    b = BatchedRun(scenes, engine)
+    st = b.run(Tuning(*tune))
+    print(f"tuning {tune}: {st.n_histories} histories, {st.rounds} rounds, {st.device_ms:.2f} ms", flush=True)
+    assert st.n_histories == sum(s.n_histories for s in scenes)
+    got = np.array([st.n_absorbed, st.n_escaped, st.scatter_sum, st.n_fluorescence, st.n_events, st.n_flights,
+                    st.n_voxel_visits], dtype=np.int64)
+    np.testing.assert_array_equal(got, np.sum([r["cnt"] for r in ref], axis=0))
+    assert st.n_absorbed + st.n_escaped == st.n_histories
+    names = engine.pack.materials
+    for k, r in enumerate(ref):
+        kk, hh, edep = b.read(k)
+        np.testing.assert_allclose(kk, r["kerma"], rtol=1e-9, atol=1e-12 * r["kerma"].max())
+        np.testing.assert_allclose(hh, r["h10"], rtol=1e-9, atol=1e-12 * r["h10"].max())
+        e = np.zeros(N.CC_MAX_MAT)
+        for m, v in edep.items():
+            e[names.index(m)] = v
+        np.testing.assert_allclose(e, r["edep"], rtol=1e-9, atol=1e-9)
+
+
+def test_fraction_and_rerun_start_from_zero(engine, single):
+    from chatcarlo_amd.batch import BatchedRun, Tuning
+    scenes, ref = single
+    b = BatchedRun(scenes, engine)
+    st = b.run(Tuning(256, 4, 8192), fraction=0.25)
+    assert st.n_histories == sum(max(1, int(np.ceil(s.n_histories * 0.25))) for s in scenes)
+    st = b.run(Tuning(256, 4, 8192))
+    for k, r in enumerate(ref):
+        kk, _, _ = b.read(k)
+        np.testing.assert_allclose(kk, r["kerma"], rtol=1e-9, atol=1e-12 * r["kerma"].max())
+    res = b.results()
+    assert [x.n_histories for x in res] == [s.n_histories for s in scenes]
+    np.testing.assert_allclose(res[0].grid.kerma_keV, ref[0]["kerma"], rtol=1e-9, atol=1e-12 * ref[0]["kerma"].max())
+
+
+def test_autotune_returns_a_candidate(engine):
+    from chatcarlo_amd.batch import BatchedRun
+    b = BatchedRun(_scenes(3, seed=5), engine)
+    t, table = b.autotune(fraction=0.2)
+    assert len(table) == 24 and np.all(table[:, 3] > 0)
+    assert (t.block, t.rounds_per_sync, t.pool) in {(int(a), int(q), int(p)) for a, q, p, _ in table}
+    best = table[np.argmax(table[:, 3])]
+    assert (t.block, t.rounds_per_sync, t.pool) == (int(best[0]), int(best[1]), int(best[2]))
+
+
+def test_bad_tuning_is_rejected(engine):
+    from chatcarlo_amd.batch import BatchedRun, Tuning
+    b = BatchedRun(_scenes(2, seed=3), engine)
+    for t in (Tuning(100, 8, 4096), Tuning(256, 3, 4096), Tuning(256, 8, 10), Tuning(512, 8, 4096)):
+        with pytest.raises(ValueError):
+            b.run(t)
+    with pytest.raises(ValueError):
+        b.run(Tuning(), fraction=0.0)
