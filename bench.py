@@ -116,6 +116,53 @@ def learned_fom(eng, target=0.02):
             "train_device_ms": tr.ms, "train_loss_first_last": [tr.loss_first, tr.loss_last]}
 
 
+def batched_sweep(eng, autotune_fraction=0.02, sequential=True):
+    """64 random scenes (1-40 primitives each, 50-150 kVp, 1e5-1e7 histories, 2 cm grids) in one
+    particle pool: launch shape autotuned on a fraction of the histories, then the whole sweep;
+    beside it the same sweep scene by scene (one single-scene run per scene, as a caller of
+    run_transport would loop), whose tallies it must equal."""
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd.batch import BatchedRun, SceneSpec
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.source import SourceRecord
+    from chatcarlo_amd.workloads import batched_scenes
+    scenes = [SceneSpec(raw, grid, n, s) for raw, grid, n, s in batched_scenes(64)]
+    t0 = time.perf_counter()
+    b = BatchedRun(scenes, eng)
+    t_up = time.perf_counter() - t0
+    b.run(fraction=0.005)   # warm-up (code objects, pool allocation)
+    t0 = time.perf_counter()
+    tune, table = b.autotune(autotune_fraction)
+    t_tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    st = b.run(tune)
+    wall = time.perf_counter() - t0
+    out = {"workload": "64_random_scenes_300cm_air_room_2cm_grids", "scenes": len(scenes),
+           "histories": st.n_histories, "tuning": {"block": tune.block, "rounds_per_sync": tune.rounds_per_sync,
+                                                   "pool": tune.pool},
+           "histories_per_s": st.n_histories / wall, "device_ms": st.device_ms, "wall_s": wall,
+           "rounds": st.rounds, "graph_launches": st.graph_launches,
+           "voxel_visits_per_history": st.n_voxel_visits / st.n_histories,
+           "flights_per_history": st.n_flights / st.n_histories,
+           "upload_s": t_up, "autotune_s": t_tune, "autotune_fraction": autotune_fraction,
+           "autotune_table": [[int(r[0]), int(r[1]), int(r[2]), round(float(r[3]))] for r in table]}
+    if sequential:
+        kerma_b = [float(b.read(k)[0].sum()) for k in range(len(scenes))]
+        t0 = time.perf_counter()
+        kerma_s = []
+        for s in scenes:
+            eng.set_geometry(Geometry(s.raw["geometry"]))
+            eng.bind_grid(s.grid[0], s.grid[1], s.grid[2], False)
+            eng.run(SourceRecord(s.raw["source"]), s.n_histories, s.seed, 0, 1_000_000,
+                    N.RUN_FLUORESCENCE | N.RUN_GRID)
+            kerma_s.append(float(eng.read_grid(s.grid[1], moments=False)["kerma"].sum()))
+        seq = time.perf_counter() - t0
+        out["sequential"] = {"histories_per_s": st.n_histories / seq, "wall_s": seq,
+                             "batched_speedup": seq / wall,
+                             "max_rel_diff_total_kerma": max(abs(a - c) / c for a, c in zip(kerma_b, kerma_s) if c)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +176,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-importance FOM comparison")
     ap.add_argument("--learned-only", action="store_true",
                     help="only the learned-importance comparison (prints its JSON object)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched sweep")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the 64-scene batched sweep (prints its JSON object)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -147,6 +197,10 @@ def main():
               "(histories shard across GPUs only in the design, not in this build)", file=sys.stderr)
         sys.exit(2)
 
+    if args.batched_only:
+        from chatcarlo_amd.transport import default_engine
+        print(json.dumps(batched_sweep(default_engine())), flush=True)
+        return
     if args.learned_only:
         from chatcarlo_amd.transport import default_engine
         print(json.dumps(learned_fom(default_engine())), flush=True)
@@ -218,6 +272,8 @@ def main():
         "physics": {"fraction_absorbed": o.n_absorbed / n, "fraction_escaped": o.n_escaped / n,
                     "mean_interactions": o.scatter_sum / n, "n_fluorescence": int(o.n_fluorescence)},
     }
+    if not args.no_batched:
+        line["batched_sweep"] = batched_sweep(eng)
     if not args.no_learned:
         line["learned_importance"] = learned_fom(eng)
     if not args.no_cpu_baseline:
