@@ -116,6 +116,7 @@ _SIGS = {
     "cc_upload_scene": (C.c_int, [C.c_void_p, C.POINTER(CcPrim), C.c_int32, C.POINTER(CcWorld)]),
     "cc_bind_grid": (C.c_int, [C.c_void_p, dp, ip, C.c_double, C.c_int32]),
     "cc_unbind_grid": (C.c_int, [C.c_void_p]),
+    "cc_set_grid_copies": (C.c_int, [C.c_void_p, C.c_int32]),
     "cc_reset_grid": (C.c_int, [C.c_void_p]),
     "cc_read_grid": (C.c_int, [C.c_void_p, dp, dp, dp, dp, ip, i64p, i64p]),
     "cc_load_grid": (C.c_int, [C.c_void_p, dp, dp, dp, dp, ip, C.c_int64, C.c_int64]),

@@ -536,13 +536,20 @@ __device__ __forceinline__ void isotropic_dir(float d[3], float u1, float u2) {
 // ---------------------------------------------------------------------------
 // track-length tally: exact 3-D DDA (Amanatides & Woo 1987) with FP64 atomics
 // ---------------------------------------------------------------------------
+// Batch tallies are held in cmask + 1 copies, cstride elements apart: workgroup b
+// adds into copy b & cmask.  Workgroups are dealt to the 8 XCDs round-robin, so with
+// 8 copies each XCD adds into its own copy and the hot voxels of a beam are not
+// contended across XCDs; the batch end sums the copies.
 struct GridDev {
-  double* dk;     // batch kerma [keV]
-  double* dh;     // batch H*(10) track integral [pSv cm^3]
+  double* dk;     // batch kerma [keV], copy 0
+  double* dh;     // batch H*(10) track integral [pSv cm^3], copy 0
   int nx, ny, nz;
   float org[3];
   float h;
   float inv_h;
+  uint32_t cmask; // copies - 1 (copies a power of two)
+  uint32_t pad;
+  size_t cstride; // elements between copies
 };
 
 __device__ __forceinline__ void atomic_add_f64(double* p, double v) {
@@ -566,6 +573,9 @@ constexpr int kAggRounds = 4;
 template <bool kScene>
 __device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t v, double ak, double ah, uint32_t key) {
   const int lane = __lane_id();
+  const size_t copy = (size_t)(blockIdx.x & G.cmask) * G.cstride;
+  double* const dk = G.dk + copy;
+  double* const dh = G.dh + copy;
   unsigned long long pend = __ballot(has);
   for (int r = 0; r < kAggRounds && pend; ++r) {
     const int leader = __ffsll((long long)pend) - 1;
@@ -577,15 +587,15 @@ __device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t 
     const double sk = wave_sum_f64(mine ? ak : 0.0);
     const double sh = wave_sum_f64(mine ? ah : 0.0);
     if (lane == leader) {
-      atomic_add_f64(G.dk + vl, sk);
-      atomic_add_f64(G.dh + vl, sh);
+      atomic_add_f64(dk + vl, sk);
+      atomic_add_f64(dh + vl, sh);
     }
     has = has && !mine;
     pend &= ~m;
   }
   if (has) {
-    atomic_add_f64(G.dk + v, ak);
-    atomic_add_f64(G.dh + v, ah);
+    atomic_add_f64(dk + v, ak);
+    atomic_add_f64(dh + v, ah);
   }
 }
 

@@ -50,6 +50,7 @@ constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kMaxRounds = 100000;
 constexpr int kMaxCrossings = 1 << 14;
+constexpr int kGridCopies = 8;        // default batch-tally copies (one per XCD)
 constexpr int kTrialCap = 100000;
 
 enum { F_ABSORBED = 1, F_ESCAPED = 2, F_DEAD = 4 };
@@ -577,6 +578,8 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     float E = 1.0f, tau = 0.f, w = 0.f, mu = 0.f, muen = 0.f, hreg = 0.f, sreg = 0.f;
     double wh = 0.0;
     int mc = -1, it = 0, scene = 0;
+    uint64_t key = 0;
+    uint32_t hid = 0, tag = 0, ctr = 0;
     EPos ep{0, 0.f, 0.f};
     const Prim* sp = sp_lds;
     World Wd = L.W;
@@ -594,6 +597,8 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
       E = S.e[i];
       tau = S.tau[i];
       w = S.wt[i];
+      key = kMulti ? L.MS.seeds[scene] : L.seed;
+      hid = S.hid[i]; tag = S.tag[i]; ctr = S.ctr[i];
       if (!energy_ok(E)) {
         raise_err(L.err, CC_DEV_ENERGY_RANGE);
         S.flags[i] |= F_DEAD;
@@ -615,6 +620,12 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     float tc = 0.f;
     float sexp = 0.f;   // expected region score of this flight (contribution records)
     if (kRec && alive) sexp = hreg * region_expect(sp, Wd, T, L.R, o0, d, E, ep, eps0);
+    // The tally of each segment is issued one step late, after the next step's table
+    // loads: a load waits for every older vector-memory operation of the wave, atomics
+    // included, so loads issued behind a segment's atomics would wait for all of them.
+    float pend_p[3] = {0.f, 0.f, 0.f}, pend_len = 0.f;
+    double pend_wk = 0.0;
+    bool pend = false;
     while (__ballot(alive)) {
       float ds = 0.f, taub = 0.f, t_next = 0.f;
       bool inter = false, esc = false, seg = false;
@@ -644,8 +655,13 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
           seg = L.grid_on != 0;
         }
       }
-      if (L.grid_on)
-        visits += dda_score<kMulti>(G, p, d, ds, (double)w * (double)E * (double)muen, wh, seg, (uint32_t)scene);
+      if (L.grid_on) {
+        visits += dda_score<kMulti>(G, pend_p, d, pend_len, pend_wk, wh, pend, (uint32_t)scene);
+        pend_p[0] = p[0]; pend_p[1] = p[1]; pend_p[2] = p[2];
+        pend_len = ds;
+        pend_wk = (double)w * (double)E * (double)muen;
+        pend = seg;
+      }
       if (alive && L.R.on) {
         // chord of the segment [0, ds] through the scoring sphere
         const float ox = p[0] - L.R.c[0], oy = p[1] - L.R.c[1], oz = p[2] - L.R.c[2];
@@ -664,9 +680,10 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
           if (coll) {
             type = 3;
           } else {
-            PhiloxRng rng(photon_seed(L, S, i), L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
+            PhiloxRng rng(key, L.hist0 + (uint64_t)hid, tag, ctr);
             type = sample_type(T, m, E, ep, rng.next());
-            S.ctr[i] = rng.ctr;
+            ctr = rng.ctr;
+            S.ctr[i] = ctr;
           }
           S.mat[i] = m;
           tau = 0.f;
@@ -685,6 +702,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
         }
       }
     }
+    if (L.grid_on) visits += dda_score<kMulti>(G, pend_p, d, pend_len, pend_wk, wh, pend, (uint32_t)scene);
     if (valid && energy_ok(E)) {
       if (escaped) S.flags[i] |= F_ESCAPED;
 #pragma unroll
@@ -1507,10 +1525,22 @@ __global__ void k_mlp_refresh(const float* w, uint16_t* wb) {
 // ---------------------------------------------------------------------------
 // batch end: fold the batch buffers into totals and moments (chatcarlo/tally.py:59-94)
 // ---------------------------------------------------------------------------
+// Sums the batch copies of every voxel into the totals (and batch moments), clearing them.
 __global__ __launch_bounds__(kBlock) void k_end_batch(double* dk, double* dh, double* tk, double* th, double* qk,
-                                                      double* qh, int* hits, size_t nvox, double inv_nb, int track) {
+                                                      double* qh, int* hits, size_t nvox, double inv_nb, int track,
+                                                      int ncopy, size_t cstride) {
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (size_t)gridDim.x * blockDim.x) {
-    const double a = dk[v], b = dh[v];
+    double a = dk[v], b = dh[v];
+    if (a != 0.0 || b != 0.0) dk[v] = dh[v] = 0.0;
+    for (int k = 1; k < ncopy; ++k) {
+      const size_t u = (size_t)k * cstride + v;
+      const double ak = dk[u], bk = dh[u];
+      if (ak != 0.0 || bk != 0.0) {
+        a += ak;
+        b += bk;
+        dk[u] = dh[u] = 0.0;
+      }
+    }
     if (a == 0.0 && b == 0.0) continue;
     tk[v] += a;
     th[v] += b;
@@ -1519,8 +1549,6 @@ __global__ __launch_bounds__(kBlock) void k_end_batch(double* dk, double* dh, do
       qh[v] += b * b * inv_nb;
       hits[v] += 1;
     }
-    dk[v] = 0.0;
-    dh[v] = 0.0;
   }
 }
 
@@ -1711,6 +1739,7 @@ struct Ctx {
   int* hits = nullptr;
   size_t nvox = 0;
   bool track = false;
+  int grid_copies = kGridCopies;   // batch-tally copies of the next cc_bind_grid
   int64_t g_batches = 0, g_hist = 0;
   // particle pool
   size_t cap = 0;
@@ -2245,11 +2274,15 @@ int cc_bind_grid(void* ctx, const double* origin, const int32_t* shape, double v
   HIP_TRY(hipSetDevice(c->device));
   cc_unbind_grid(ctx);
   const size_t nv = (size_t)shape[0] * shape[1] * shape[2];
+  const size_t cs = (nv + 31) & ~(size_t)31;   // copies start on 256 B boundaries
+  const size_t nc = (size_t)c->grid_copies;
   int r;
-  if ((r = dalloc(&c->G.dk, nv)) || (r = dalloc(&c->G.dh, nv)) || (r = dalloc(&c->tk, nv)) || (r = dalloc(&c->th, nv)) ||
-      (r = dalloc(&c->qk, nv)) || (r = dalloc(&c->qh, nv)) || (r = dalloc(&c->hits, nv)))
+  if ((r = dalloc(&c->G.dk, nc * cs)) || (r = dalloc(&c->G.dh, nc * cs)) || (r = dalloc(&c->tk, nv)) ||
+      (r = dalloc(&c->th, nv)) || (r = dalloc(&c->qk, nv)) || (r = dalloc(&c->qh, nv)) || (r = dalloc(&c->hits, nv)))
     return r;
   c->nvox = nv;
+  c->G.cmask = (uint32_t)(nc - 1);
+  c->G.cstride = cs;
   c->G.nx = shape[0]; c->G.ny = shape[1]; c->G.nz = shape[2];
   for (int k = 0; k < 3; ++k) c->G.org[k] = (float)origin[k];
   c->G.h = (float)voxel_cm;
@@ -2257,6 +2290,15 @@ int cc_bind_grid(void* ctx, const double* origin, const int32_t* shape, double v
   c->track = (flags & 1) != 0;
   c->have_grid = true;
   return cc_reset_grid(ctx);
+}
+
+int cc_set_grid_copies(void* ctx, int32_t copies) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return fail(CC_ERR_ARG, "null ctx");
+  if (copies < 1 || copies > 64 || (copies & (copies - 1)))
+    return fail(CC_ERR_ARG, "copies %d must be a power of two in [1, 64]", copies);
+  c->grid_copies = copies;
+  return CC_OK;
 }
 
 int cc_unbind_grid(void* ctx) {
@@ -2277,7 +2319,10 @@ int cc_reset_grid(void* ctx) {
   Ctx* c = (Ctx*)ctx;
   if (!c || !c->have_grid) return fail(CC_ERR_STATE, "no grid bound");
   const int nb = grid_blocks(c->nvox);
-  double* ds[] = {c->G.dk, c->G.dh, c->tk, c->th, c->qk, c->qh};
+  const size_t nbatch = (size_t)(c->G.cmask + 1) * c->G.cstride;
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(nbatch)), dim3(kBlock), 0, c->stream, c->G.dk, nbatch, 0.0);
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(nbatch)), dim3(kBlock), 0, c->stream, c->G.dh, nbatch, 0.0);
+  double* ds[] = {c->tk, c->th, c->qk, c->qh};
   for (double* d : ds) hipLaunchKernelGGL(k_fill_f64, dim3(nb), dim3(kBlock), 0, c->stream, d, c->nvox, 0.0);
   hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(kBlock), 0, c->stream, c->hits, c->nvox, 0);
   HIP_TRY(hipGetLastError());
@@ -2425,7 +2470,8 @@ int cc_run(void* ctx, const cc_source* src, uint64_t n_hist, uint64_t seed, uint
     if (grid_on) {
       tm.begin(K_END);
       hipLaunchKernelGGL(k_end_batch, dim3(grid_blocks(c->nvox)), dim3(kBlock), 0, c->stream, c->G.dk, c->G.dh, c->tk,
-                         c->th, c->qk, c->qh, c->hits, c->nvox, 1.0 / (double)nb, track ? 1 : 0);
+                         c->th, c->qk, c->qh, c->hits, c->nvox, 1.0 / (double)nb, track ? 1 : 0,
+                         (int)c->G.cmask + 1, c->G.cstride);
       tm.end();
       if (track) c->g_batches += 1;
       c->g_hist += nb;
@@ -2483,7 +2529,8 @@ int cc_transport(void* ctx, int64_t n, double* pos, double* dir, double* energy,
   out->rounds_max = rounds;
   if (grid_on) {
     hipLaunchKernelGGL(k_end_batch, dim3(grid_blocks(c->nvox)), dim3(kBlock), 0, c->stream, c->G.dk, c->G.dh, c->tk,
-                       c->th, c->qk, c->qh, c->hits, c->nvox, 1.0 / (double)n, 0);
+                       c->th, c->qk, c->qh, c->hits, c->nvox, 1.0 / (double)n, 0,
+                       (int)c->G.cmask + 1, c->G.cstride);
     HIP_TRY(hipGetLastError());
   }
   int err_bits = 0;
@@ -2689,7 +2736,7 @@ int cc_test_dda(void* ctx, int64_t n, const float* o, const float* d, const floa
   hipLaunchKernelGGL(k_test_dda, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->G, (int)n, dO, dD, dL, dk,
                      dh);
   hipLaunchKernelGGL(k_end_batch, dim3(grid_blocks(c->nvox)), dim3(kBlock), 0, c->stream, c->G.dk, c->G.dh, c->tk, c->th,
-                     c->qk, c->qh, c->hits, c->nvox, 1.0, 0);
+                     c->qk, c->qh, c->hits, c->nvox, 1.0, 0, (int)c->G.cmask + 1, c->G.cstride);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CC_OK;

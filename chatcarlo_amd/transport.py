@@ -155,6 +155,11 @@ class Engine:
         self._scene_key = key
         return prims, world
 
+    def set_grid_copies(self, copies: int):
+        """Batch-tally copies (power of two) for grids bound from now on."""
+        N.check(self.lib.cc_set_grid_copies(self.ctx, int(copies)))
+        self._grid_key = None
+
     def bind_grid(self, origin, shape, voxel_cm, track_uncertainty):
         key = (tuple(float(x) for x in origin), tuple(int(x) for x in shape), float(voxel_cm),
                bool(track_uncertainty))

@@ -140,3 +140,65 @@ def batched_scenes(n_scenes=64, master_seed=2026, max_prims=40):
 
 def copy_raw(raw):
     return copy.deepcopy(raw)
+
+
+def _plain(x):
+    """numpy scalars and arrays to plain Python values (for YAML)."""
+    if isinstance(x, dict):
+        return {k: _plain(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [_plain(v) for v in x]
+    if isinstance(x, np.ndarray):
+        return [_plain(v) for v in x.tolist()]
+    if isinstance(x, np.generic):
+        return x.item()
+    return x
+
+
+def _grid_doc(gs):
+    if gs is None:
+        return {"over": "world box", "resolution_cm": 5.0}
+    o, s, v = gs
+    return {"origin_cm": _plain(o), "shape": _plain(s), "voxel_cm": float(v)}
+
+
+def fixture_documents():
+    """The workloads as YAML documents: {file name: document}.
+
+    Each single scene is a plain scene.yaml (source, geometry, physics) that the CLI and
+    `load_scene` read. `workloads.yaml` indexes them with each workload's histories, seed and grid.
+    `batched_64.yaml` holds the 64 batched scenes, each with its grid, histories and seed."""
+    docs = {}
+    index = {}
+    singles = [("water_slab_60kev", water_slab_60kev(), 100_000, 42),
+               ("chest_room", chest_room(), 10_000_000, 42),
+               ("ct_helical", ct_helical(), 100_000_000, 42),
+               ("angio_room", angio_room(), None, None)]
+    for name, (raw, gs), n, seed in singles:
+        docs[f"{name}.yaml"] = _plain(raw)
+        index[name] = {"scene": f"{name}.yaml", "grid": _grid_doc(gs), "n_histories": n, "seed": seed}
+    c, r = angio_region()
+    index["angio_room"].update(grid=None, region={"center_cm": _plain(c), "radius_cm": float(r)},
+                               target_rel_err=0.02)
+    docs["batched_64.yaml"] = {"scenes": [
+        {"scene": _plain(raw), "grid": _grid_doc(gs), "n_histories": int(n), "seed": int(s)}
+        for raw, gs, n, s in batched_scenes(64)]}
+    index["batched_64"] = {"scenes": "batched_64.yaml", "n_scenes": 64, "master_seed": 2026}
+    docs["workloads.yaml"] = index
+    return docs
+
+
+def write_fixtures(directory):
+    """Writes `fixture_documents()` as YAML files into `directory`."""
+    import os
+
+    import yaml
+    os.makedirs(directory, exist_ok=True)
+    for name, doc in fixture_documents().items():
+        with open(os.path.join(directory, name), "w") as f:
+            yaml.safe_dump(doc, f, sort_keys=False, default_flow_style=None, width=110)
+
+
+if __name__ == "__main__":
+    import sys
+    write_fixtures(sys.argv[1] if len(sys.argv) > 1 else "fixtures/scenes")

@@ -441,6 +441,7 @@ def interaction_type(p_photo, p_compt, r):
 # ---------------------------------------------------------------------------
 def overlap_tally(target, origin_cm, voxel_cm, o, d, length, weight):
     """Exact segment-voxel overlap scoring: target[v] += weight * overlap(v).
+    Returns the number of (segment, voxel) pieces of positive length.
 
     The expectation of the reference's stratified estimator
     (chatcarlo/tally.py:248-293) is exactly this overlap; the exact form is the
@@ -452,7 +453,7 @@ def overlap_tally(target, origin_cm, voxel_cm, o, d, length, weight):
     weight = np.asarray(weight, dtype=float)
     n = o.shape[0]
     if n == 0:
-        return
+        return 0
     shape = np.array(target.shape)
     org = np.asarray(origin_cm, dtype=float)
     hi = org + shape * voxel_cm
@@ -471,7 +472,7 @@ def overlap_tally(target, origin_cm, voxel_cm, o, d, length, weight):
         t1 = np.where(par, t1, np.minimum(t1, np.maximum(ta, tb)))
     keep = t1 > t0
     if not np.any(keep):
-        return
+        return 0
     o, d, t0, t1, weight = o[keep], d[keep], t0[keep], t1[keep], weight[keep]
     n = o.shape[0]
     ts = [t0, t1]
@@ -507,6 +508,7 @@ def overlap_tally(target, origin_cm, voxel_cm, o, d, length, weight):
     idx = np.clip(idx, 0, shape - 1)
     flat = np.ravel_multi_index((idx[:, 0], idx[:, 1], idx[:, 2]), target.shape)
     np.add.at(target.reshape(-1), flat, weight[sid] * ln)
+    return int(len(ln))
 
 
 def stratified_tally(target, origin_cm, voxel_cm, o, d, length, weight, rng,

@@ -76,6 +76,7 @@ def transport_photons(pos, dirv, energy, geom, pack, rng, grid=None, fluorescenc
     n_fl = 0
     n_mat = len(pack.materials)
     rounds = 0
+    n_seg = n_vox = 0      # flight segments, and (segment, voxel) pieces of the exact tally
     rscore = np.zeros(n)
     if region is not None:
         rc, rr = np.asarray(region[0], dtype=float), float(region[1])
@@ -83,6 +84,7 @@ def transport_photons(pos, dirv, energy, geom, pack, rng, grid=None, fluorescenc
     while np.any(alive):
         rounds += 1
         idx = np.where(alive)[0]
+        n_seg += len(idx)
         o, d, e = pos[idx], dirv[idx], energy[idx]
         mat = geom.material_at(o)
         xs = core.xs_lookup(pack, mat, e)
@@ -99,7 +101,7 @@ def transport_photons(pos, dirv, energy, geom, pack, rng, grid=None, fluorescenc
             wk = e * xs["mu_en"]
             wh = core.h10_coeff(pack, e)
             if exact_tally:
-                core.overlap_tally(grid.kerma, grid.origin, grid.voxel, o, d, ds, wk)
+                n_vox += core.overlap_tally(grid.kerma, grid.origin, grid.voxel, o, d, ds, wk)
                 core.overlap_tally(grid.h10, grid.origin, grid.voxel, o, d, ds, wh)
             else:
                 core.stratified_tally(grid.kerma, grid.origin, grid.voxel, o, d, ds, wk, tally_rng)
@@ -160,7 +162,7 @@ def transport_photons(pos, dirv, energy, geom, pack, rng, grid=None, fluorescenc
             n_scatter[ridx] += 1
     return dict(n_scatter=n_scatter, absorbed=absorbed, escaped=escaped, final_energy=energy,
                 energy_deposited=edep, n_fluorescence=n_fl, rounds=rounds, n_mat=n_mat,
-                region_score=rscore)
+                region_score=rscore, n_segments=n_seg, n_voxel_pieces=n_vox)
 
 
 def run(src, geom, pack, spec_e, spec_cdf, n_histories, seed, batch_size=200_000, grid=None,
@@ -169,7 +171,7 @@ def run(src, geom, pack, spec_e, spec_cdf, n_histories, seed, batch_size=200_000
     rng = np.random.default_rng(seed)
     edep = {}
     sums, sum2 = {}, {}
-    n_abs = n_esc = scat = n_fl = 0
+    n_abs = n_esc = scat = n_fl = n_seg = n_vox = 0
     n_b = 0
     r_sum = r_sum2 = 0.0
     remaining = n_histories
@@ -191,9 +193,12 @@ def run(src, geom, pack, spec_e, spec_cdf, n_histories, seed, batch_size=200_000
         if grid is not None:
             grid.end_batch(nb)
         n_fl += r["n_fluorescence"]
+        n_seg += r["n_segments"]
+        n_vox += r["n_voxel_pieces"]
         n_abs += int(r["absorbed"].sum())
         n_esc += int(r["escaped"].sum())
         scat += int(r["n_scatter"].sum())
     return dict(energy_deposited_keV=edep, sums=sums, sum2=sum2, n_batches=n_b,
                 n_absorbed=n_abs, n_escaped=n_esc, scatter_sum=scat, n_fluorescence=n_fl,
-                n_histories=n_histories, region_sum=r_sum, region_sum2=r_sum2)
+                n_histories=n_histories, region_sum=r_sum, region_sum2=r_sum2,
+                n_segments=n_seg, n_voxel_pieces=n_vox)

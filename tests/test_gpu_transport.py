@@ -244,3 +244,36 @@ def test_ct_helical_chi2_against_oracle(engine, dpack):
     assert abs(chi2 / dof - 1.0) < 3.0 * np.sqrt(2.0 / dof) + 0.05, (chi2, dof)
     assert np.max(np.abs(z)) < 5.5
     assert out.error_bits == 0
+
+
+@pytest.mark.parametrize("workload", ["chest_room", "ct_helical"])
+def test_segment_and_voxel_counts_match_oracle(engine, dpack, workload):
+    """Flight segments and voxel pieces per history: the N_evt and N_vox of the roofline's
+    algorithmic bytes.  The GPU counters (one per boundary step, one per voxel piece of positive
+    length) must have the oracle's means: batch means of 8 oracle batches give the spread."""
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd import workloads as W
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.source import SourceRecord
+    from chatcarlo_amd.tally import VoxelGrid
+    raw, gs = getattr(W, workload)()
+    geo = Geometry(raw["geometry"])
+    if gs is None:
+        vg = VoxelGrid.from_bbox(geo.bbox_min, geo.bbox_max, 5.0)
+        gs = (vg.origin_cm, vg.shape, vg.voxel_size_cm)
+    engine.set_geometry(geo)
+    engine.bind_grid(gs[0], gs[1], gs[2], False)
+    rec = SourceRecord(raw["source"])
+    n_gpu = 1_000_000
+    out = engine.run(rec, n_gpu, 99, 0, 250_000, N.RUN_FLUORESCENCE | N.RUN_GRID)
+    prims, world = geo.packed(dpack.mat_id)
+    og = core.OGeom.from_packed(prims, world)
+    seg, vox = [], []
+    for b in range(8):
+        grid = OT.Grid(gs[0], gs[1], gs[2], track_uncertainty=False)
+        r = OT.run(raw["source"], og, dpack, rec.spec_e, rec.spec_cdf, 4000, 300 + b, batch_size=4000, grid=grid)
+        seg.append(r["n_segments"] / 4000)
+        vox.append(r["n_voxel_pieces"] / 4000)
+    for gpu, orc in ((out.n_events / n_gpu, seg), (out.n_voxel_visits / n_gpu, vox)):
+        m, se = np.mean(orc), np.std(orc, ddof=1) / np.sqrt(len(orc))
+        assert abs(gpu - m) < 5 * se + 0.005 * m, (workload, gpu, m, se)
