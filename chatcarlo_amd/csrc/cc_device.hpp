@@ -12,6 +12,18 @@
 
 namespace cc {
 
+// v[a] for a run-time a, formed arithmetically: a select chain over the elements of
+// one array or struct is folded by the compiler into a run-time index, which moves
+// the whole aggregate from registers to scratch memory.
+__device__ __forceinline__ float sel3(int a, float v0, float v1, float v2) {
+  const uint32_t m0 = 0u - (uint32_t)(a == 0), m1 = 0u - (uint32_t)(a == 1), m2 = 0u - (uint32_t)(a == 2);
+  return __uint_as_float((__float_as_uint(v0) & m0) | (__float_as_uint(v1) & m1) | (__float_as_uint(v2) & m2));
+}
+__device__ __forceinline__ int sel3i(int a, int v0, int v1, int v2) {
+  return v0 * (a == 0) + v1 * (a == 1) + v2 * (a == 2);
+}
+
+
 constexpr float kHC = 12.3984193f;      // chatcarlo/physics.py:35
 constexpr float kMEC2 = 511.0f;         // chatcarlo/physics.py:34
 constexpr float kFluorCut = 5.0f;       // chatcarlo/physics.py:38
@@ -205,9 +217,9 @@ __device__ __forceinline__ bool isect_cylinder(const Prim& P, const float o[3], 
   const int ax = P.axis;
   const int i0 = (ax == 0) ? 1 : 0;
   const int i1 = (ax == 2) ? 1 : 2;
-  ff ox = two_sum(o[i0], -P.p[i0]);
-  ff oy = two_sum(o[i1], -P.p[i1]);
-  float dx = d[i0], dy = d[i1];
+  ff ox = two_sum(sel3(i0, o[0], o[1], o[2]), -P.p[i0]);
+  ff oy = two_sum(sel3(i1, o[0], o[1], o[2]), -P.p[i1]);
+  float dx = sel3(i0, d[0], d[1], d[2]), dy = sel3(i1, d[0], d[1], d[2]);
   ff aa = ff_add(two_prod(dx, dx), two_prod(dy, dy));
   float a = ff_f(aa);
   ff b = ff_add(ff_mul(ox, {dx, 0.f}), ff_mul(oy, {dy, 0.f}));
@@ -226,9 +238,10 @@ __device__ __forceinline__ bool isect_cylinder(const Prim& P, const float o[3], 
     if (!quad_roots(aa, b, cc, ten_r, tex_r)) miss_r = true;
   }
   // caps at c_ax +- half_h, plane offsets formed exactly in float-float
-  float dz = d[ax];
-  ff lo_rel = ff_add(two_sum(P.p[ax], -P.p[4]), {-o[ax], 0.f});
-  ff hi_rel = ff_add(two_sum(P.p[ax], P.p[4]), {-o[ax], 0.f});
+  float dz = sel3(ax, d[0], d[1], d[2]);
+  const float oz = sel3(ax, o[0], o[1], o[2]);
+  ff lo_rel = ff_add(two_sum(P.p[ax], -P.p[4]), {-oz, 0.f});
+  ff hi_rel = ff_add(two_sum(P.p[ax], P.p[4]), {-oz, 0.f});
   float ten_z, tex_z;
   bool miss_z = false;
   if (fabsf(dz) < 1e-12f) {
@@ -275,10 +288,11 @@ __device__ __forceinline__ bool contains(const Prim& P, const float q[3]) {
   const int ax = P.axis;
   const int i0 = (ax == 0) ? 1 : 0;
   const int i1 = (ax == 2) ? 1 : 2;
-  ff s = ff_add(ff_sqr(two_sum(q[i0], -P.p[i0])), ff_sqr(two_sum(q[i1], -P.p[i1])));
+  ff s = ff_add(ff_sqr(two_sum(sel3(i0, q[0], q[1], q[2]), -P.p[i0])),
+                ff_sqr(two_sum(sel3(i1, q[0], q[1], q[2]), -P.p[i1])));
   s = ff_add(s, ff_neg(two_prod(P.p[3], P.p[3])));
   if (!(s.h < 0.f || (s.h == 0.f && s.l <= 0.f))) return false;
-  ff z = two_sum(q[ax], -P.p[ax]);
+  ff z = two_sum(sel3(ax, q[0], q[1], q[2]), -P.p[ax]);
   ff za = (z.h < 0.f || (z.h == 0.f && z.l < 0.f)) ? ff_neg(z) : z;
   ff e = ff_add(za, {-P.p[4], 0.f});
   return e.h < 0.f || (e.h == 0.f && e.l <= 0.f);
@@ -599,44 +613,64 @@ __device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t 
   }
 }
 
-// Scores wk*len into dk and wh*len into dh for every voxel the segment
-// o + t d, t in [0, L], overlaps, for lanes with seg = true.  Exact 3-D DDA
-// (Amanatides & Woo 1987); all lanes of the wave must call it.  Returns the
-// number of voxel visits of the calling lane.
-template <bool kScene = false>
-__device__ __forceinline__ int dda_score(const GridDev& G, const float o[3], const float d[3], float L,
-                                         double wk, double wh, bool seg, uint32_t key = 0) {
+// Clips the segment o + t d, t in [0, L], to the grid box: [t0, t1]; false if
+// they do not overlap.
+__device__ __forceinline__ bool dda_clip(const GridDev& G, const float o[3], const float d[3], float L,
+                                         float& t0, float& t1) {
   const int n[3] = {G.nx, G.ny, G.nz};
-  float t0 = 0.0f, t1 = L;
-  if (seg) {
+  bool seg = true;
+  t0 = 0.0f;
+  t1 = L;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      float lo = G.org[k], hi = G.org[k] + (float)n[k] * G.h;
-      if (fabsf(d[k]) < 1e-30f) {
-        if (o[k] < lo || o[k] > hi) seg = false;
-      } else {
-        float ta = (lo - o[k]) / d[k], tb = (hi - o[k]) / d[k];
-        t0 = fmaxf(t0, fminf(ta, tb));
-        t1 = fminf(t1, fmaxf(ta, tb));
-      }
+  for (int k = 0; k < 3; ++k) {
+    float lo = G.org[k], hi = G.org[k] + (float)n[k] * G.h;
+    if (fabsf(d[k]) < 1e-30f) {
+      if (o[k] < lo || o[k] > hi) seg = false;
+    } else {
+      float ta = (lo - o[k]) / d[k], tb = (hi - o[k]) / d[k];
+      t0 = fmaxf(t0, fminf(ta, tb));
+      t1 = fminf(t1, fmaxf(ta, tb));
     }
-    if (!(t1 > t0)) seg = false;
   }
+  return seg && t1 > t0;
+}
+
+// Distance along o + t d to the plane of index k on axis a.  The walk and the
+// chunk boundaries of dda_balanced use this one expression, so a chunk ends
+// exactly where the walk meets that plane.
+__device__ __forceinline__ float plane_t(float org_a, float h, float o_a, float inv_da, int k) {
+  const float plane = org_a + (float)k * h;
+  return (plane - o_a) * inv_da;
+}
+
+
+
+// Walks o + t d over [t0, t1] voxel by voxel (exact 3-D DDA, Amanatides & Woo
+// 1987) for lanes with act = true, scoring wk*len and wh*len into every voxel
+// overlapped.  fa >= 0 fixes the start index on axis fa to fidx (a walk that
+// starts on a plane of that axis).  All lanes of the wave must call it.
+// Returns the number of voxel visits of the calling lane.
+template <bool kScene>
+__device__ __forceinline__ int dda_walk(const GridDev& G, const float o[3], const float d[3], float t0, float t1,
+                                        double wk, double wh, bool act, uint32_t key, int fa = -1, int fidx = 0) {
+  const int n[3] = {G.nx, G.ny, G.nz};
   int idx[3] = {0, 0, 0}, step[3] = {0, 0, 0};
-  if (seg) {
+  float inv_d[3] = {0.f, 0.f, 0.f};
+  if (act) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      // voxel holding the clipped start point; a start that rounds onto the
-      // wrong side of a plane only moves a zero-length piece
-      int ik = (int)floorf((o[k] + d[k] * t0 - G.org[k]) * G.inv_h);
+      // voxel holding the start point; a start that rounds onto the wrong side
+      // of a plane only moves a zero-length piece
+      int ik = (k == fa) ? fidx : (int)floorf((o[k] + d[k] * t0 - G.org[k]) * G.inv_h);
       idx[k] = min(max(ik, 0), n[k] - 1);
       step[k] = (d[k] > 0.f) ? 1 : ((d[k] < 0.f) ? -1 : 0);
+      // one reciprocal per axis and segment, not a division per voxel
+      inv_d[k] = step[k] != 0 ? 1.0f / d[k] : 0.f;
     }
   }
   float tc = t0;
   int visits = 0, it = 0;
   const int cap = G.nx + G.ny + G.nz + 4;
-  bool act = seg;
   while (__ballot(act)) {
     bool has = false;
     uint32_t v = 0xffffffffu;
@@ -645,12 +679,10 @@ __device__ __forceinline__ int dda_score(const GridDev& G, const float o[3], con
       // parametric distance to the next plane on each axis, recomputed from the index
       float tm[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        float plane = G.org[k] + (float)(idx[k] + (step[k] > 0 ? 1 : 0)) * G.h;
-        tm[k] = step[k] != 0 ? (plane - o[k]) / d[k] : INFINITY;
-      }
+      for (int k = 0; k < 3; ++k)
+        tm[k] = step[k] != 0 ? plane_t(G.org[k], G.h, o[k], inv_d[k], idx[k] + (step[k] > 0 ? 1 : 0)) : INFINITY;
       int a = (tm[0] < tm[1]) ? ((tm[0] < tm[2]) ? 0 : 2) : ((tm[1] < tm[2]) ? 1 : 2);
-      float tn = fminf(fmaxf(tm[a], tc), t1);
+      float tn = fminf(fmaxf(sel3(a, tm[0], tm[1], tm[2]), tc), t1);
       double len = (double)tn - (double)tc;
       if (len > 0.0) {
         v = (uint32_t)(((size_t)idx[0] * G.ny + idx[1]) * G.nz + idx[2]);
@@ -663,11 +695,163 @@ __device__ __forceinline__ int dda_score(const GridDev& G, const float o[3], con
       if (tn >= t1 || ++it >= cap) {
         act = false;
       } else {
-        idx[a] += step[a];
-        if (idx[a] < 0 || idx[a] >= n[a]) act = false;
+        // step on axis a (selects, not a run-time index into register arrays)
+        bool out = false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          if (k == a) {
+            idx[k] += step[k];
+            out = idx[k] < 0 || idx[k] >= n[k];
+          }
+        }
+        if (out) act = false;
       }
     }
     wave_tally<kScene>(G, has, v, ak, ah, key);
+  }
+  return visits;
+}
+
+// Scores wk*len into dk and wh*len into dh for every voxel the segment
+// o + t d, t in [0, L], overlaps, for lanes with seg = true; each lane walks
+// its own segment.  All lanes of the wave must call it.
+template <bool kScene = false>
+__device__ __forceinline__ int dda_score(const GridDev& G, const float o[3], const float d[3], float L,
+                                         double wk, double wh, bool seg, uint32_t key = 0) {
+  float t0 = 0.f, t1 = 0.f;
+  if (seg) seg = dda_clip(G, o, d, L, t0, t1);
+  return dda_walk<kScene>(G, o, d, t0, t1, wk, wh, seg, key);
+}
+
+// Per-wave staging of the segments of a balanced tally.
+constexpr int kLanes = 64;       // wavefront size of CDNA
+constexpr int kChunkSlabs = 4;   // dominant-axis slabs per chunk
+struct SegStage {
+  float o[3][kLanes], d[3][kLanes], t0[kLanes], t1[kLanes];
+  double wk[kLanes], wh[kLanes];
+  uint32_t start[kLanes], key[kLanes];
+  int kf[kLanes], m[kLanes], ax[kLanes];
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The tally of dda_score with the work of the wave's segments spread over its
+// lanes.  Each segment is cut at the planes of its dominant axis (largest |d|)
+// into chunks of kChunkSlabs slabs; between two such planes the other axes
+// cross at most one plane each, so a chunk holds at most 3 kChunkSlabs voxel
+// pieces.  The wave's chunks are dealt to its 64 lanes in passes, so the lanes
+// of one walk have similar trip counts; chunk ends lie on voxel faces, so the
+// pieces (and the sums) are those of dda_score.  grids: per-scene grids when
+// kScene (key = scene index), else unused.  All lanes of the wave must call it.
+template <bool kScene>
+__device__ __forceinline__ int dda_balanced(const GridDev& G0, const GridDev* grids, SegStage& st, const float o[3], const float d[3],
+                            float L, double wk, double wh, bool seg, uint32_t key) {
+  const int lane = __lane_id();
+  float t0 = 0.f, t1 = 0.f;
+  if (seg) seg = dda_clip(G0, o, d, L, t0, t1);
+  // dominant axis and the planes strictly inside (t0, t1)
+  const float ad[3] = {fabsf(d[0]), fabsf(d[1]), fabsf(d[2])};
+  const int a = (ad[0] >= ad[1]) ? ((ad[0] >= ad[2]) ? 0 : 2) : ((ad[1] >= ad[2]) ? 1 : 2);
+  int kf = 0, m = 0;
+  uint32_t nch = 0;
+  if (seg) {
+    const int na = sel3i(a, G0.nx, G0.ny, G0.nz);
+    const float oa = sel3(a, o[0], o[1], o[2]), da = sel3(a, d[0], d[1], d[2]);
+    const float ga = sel3(a, G0.org[0], G0.org[1], G0.org[2]);
+    const float c0 = (oa + da * t0 - ga) * G0.inv_h;
+    const float c1 = (oa + da * t1 - ga) * G0.inv_h;
+    if (da > 0.f) {
+      kf = (int)floorf(c0) + 1;
+      m = (int)ceilf(c1) - kf;
+    } else {
+      kf = (int)ceilf(c0) - 1;
+      m = kf - (int)floorf(c1);
+    }
+    m = min(max(m, 0), na + 1);
+    nch = (uint32_t)((m + kChunkSlabs) / kChunkSlabs);
+  }
+  uint32_t total = 0;
+  uint32_t start = 0;
+  {
+    uint32_t x = nch;
+    for (int off = 1; off < kLanes; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    total = __shfl(x, kLanes - 1);
+    start = x - nch;
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    st.o[k][lane] = o[k];
+    st.d[k][lane] = d[k];
+  }
+  st.t0[lane] = t0; st.t1[lane] = t1; st.wk[lane] = wk; st.wh[lane] = wh;
+  st.start[lane] = start; st.key[lane] = key; st.kf[lane] = kf; st.m[lane] = m; st.ax[lane] = a;
+  wave_lds_sync();
+  int visits = 0;
+  for (uint32_t base = 0; base < total; base += kLanes) {
+    const uint32_t q = base + (uint32_t)lane;
+    bool act = q < total;
+    float so[3] = {0.f, 0.f, 0.f}, sd[3] = {0.f, 0.f, 1.f};
+    float ta = 0.f, tb = 0.f;
+    double swk = 0.0, swh = 0.0;
+    uint32_t skey = 0;
+    int sa = -1, fidx = 0;
+    if (act) {
+      // owner: the last lane whose first chunk is at or before q (lanes without
+      // chunks share the start of the next lane, so the last one has chunks)
+      int lo = 0, hi = kLanes - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (st.start[mid] <= q) lo = mid; else hi = mid - 1;
+      }
+      const int s = lo;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        so[k] = st.o[k][s];
+        sd[k] = st.d[k][s];
+      }
+      const float s0 = st.t0[s], s1 = st.t1[s];
+      swk = st.wk[s]; swh = st.wh[s]; skey = st.key[s];
+      sa = st.ax[s];
+      const int skf = st.kf[s], sm = st.m[s];
+      const int j = (int)(q - st.start[s]);
+      const float da = sel3(sa, sd[0], sd[1], sd[2]), oa = sel3(sa, so[0], so[1], so[2]);
+      const int sg = da > 0.f ? 1 : -1;
+      const float inv_da = 1.0f / da;
+      float ga, gh;
+      if (kScene) {
+        const GridDev& Gs = grids[skey];
+        ga = sel3(sa, Gs.org[0], Gs.org[1], Gs.org[2]);
+        gh = Gs.h;
+      } else {
+        ga = sel3(sa, G0.org[0], G0.org[1], G0.org[2]);
+        gh = G0.h;
+      }
+      // chunk j: slabs [j C, min((j + 1) C, m + 1)); plane P_i (i = 1..m) has index kf + sg (i - 1)
+      const int i0 = j * kChunkSlabs, i1 = i0 + kChunkSlabs;
+      ta = i0 == 0 ? s0 : fmaxf(plane_t(ga, gh, oa, inv_da, skf + sg * (i0 - 1)), s0);
+      tb = i1 >= sm + 1 ? s1 : fminf(plane_t(ga, gh, oa, inv_da, skf + sg * (i1 - 1)), s1);
+      if (i0 == 0) {
+        sa = -1;
+      } else {
+        const int kp = skf + sg * (i0 - 1);   // the chunk starts on this plane
+        fidx = sg > 0 ? kp : kp - 1;
+      }
+      act = tb > ta;
+    }
+    if (kScene) {
+      const GridDev Gs = grids[act ? skey : 0];
+      visits += dda_walk<true>(Gs, so, sd, ta, tb, swk, swh, act, skey, sa, fidx);
+    } else {
+      visits += dda_walk<false>(G0, so, sd, ta, tb, swk, swh, act, skey, sa, fidx);
+    }
   }
   return visits;
 }

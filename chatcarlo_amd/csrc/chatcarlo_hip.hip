@@ -561,6 +561,8 @@ template <bool kRec, bool kMulti>
 __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, const uint32_t* cnt_in, int* qp,
                                                    int* qc, int* qr, int* coll) {
   __shared__ Prim sp_lds[kMulti ? 1 : CC_MAX_PRIMS];
+  __shared__ SegStage seg_stage[kBlock / kWave];
+  SegStage& sst = seg_stage[threadIdx.x / kWave];
   if (!kMulti) stage_prims(L.prims, sp_lds, L.W.n_prim);
   const uint32_t n = *cnt_in;
   const Tables& T = L.T;
@@ -626,7 +628,8 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     float pend_p[3] = {0.f, 0.f, 0.f}, pend_len = 0.f;
     double pend_wk = 0.0;
     bool pend = false;
-    while (__ballot(alive)) {
+    // one more pass after the last lane stops flushes the pending tallies
+    while (__ballot(alive || pend)) {
       float ds = 0.f, taub = 0.f, t_next = 0.f;
       bool inter = false, esc = false, seg = false;
       int m = 0;
@@ -656,7 +659,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
         }
       }
       if (L.grid_on) {
-        visits += dda_score<kMulti>(G, pend_p, d, pend_len, pend_wk, wh, pend, (uint32_t)scene);
+        visits += dda_balanced<kMulti>(G, L.MS.grids, sst, pend_p, d, pend_len, pend_wk, wh, pend, (uint32_t)scene);
         pend_p[0] = p[0]; pend_p[1] = p[1]; pend_p[2] = p[2];
         pend_len = ds;
         pend_wk = (double)w * (double)E * (double)muen;
@@ -702,7 +705,6 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
         }
       }
     }
-    if (L.grid_on) visits += dda_score<kMulti>(G, pend_p, d, pend_len, pend_wk, wh, pend, (uint32_t)scene);
     if (valid && energy_ok(E)) {
       if (escaped) S.flags[i] |= F_ESCAPED;
 #pragma unroll
