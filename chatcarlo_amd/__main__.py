@@ -3,8 +3,10 @@
 Mirrors chatcarlo/__main__.py:131-253 and the `run` parser at :327-349 (flags
 -n/--n-histories, --seed, --dose-grid, --resolution, --dose-out,
 --no-uncertainty, --workers) and writes the same npz keys.  `validate` is kept
-for the scene front end; the visualisation sub-commands are not part of this
-package.
+for the scene front end.  `trace` (chatcarlo/__main__.py:48-85) records the
+trajectories of a few photons on the GPU and writes them as JSON (the
+reference renders them into an HTML preview; rendering is not part of this
+package).
 """
 from __future__ import annotations
 
@@ -79,6 +81,48 @@ def cmd_run(args) -> int:
     return 0
 
 
+_TRACE_MAX_N = 2000
+
+
+def cmd_trace(args) -> int:
+    """Trajectories of n source photons (at most 2000, as the reference clamps) as JSON."""
+    import json
+
+    from .geometry import Geometry
+    from .scene import load_scene
+    from .source import sample_source_photons
+    from .trajectory import TrajectoryRecorder, trajectories_to_json
+    from .transport import transport_photons
+    scene = load_scene(args.scene)
+    if not scene.ok:
+        for e in scene.errors:
+            print(f"[error] {e}", file=sys.stderr)
+        return 1
+    for w in scene.warnings:
+        print(f"[warning] {w}")
+    n = int(args.n)
+    if n > _TRACE_MAX_N:
+        print(f"[warning] -n={n} is too large; clamped to {_TRACE_MAX_N}")
+        n = _TRACE_MAX_N
+    rng = np.random.default_rng(args.seed)
+    geometry = Geometry(scene.raw["geometry"])
+    pos, dirv, energy = sample_source_photons(scene.raw["source"], n, rng)
+    rec = TrajectoryRecorder()
+    transport_photons(pos, dirv, energy, geometry, rng, recorder=rec,
+                      fluorescence_enabled=scene.raw.get("physics", {}).get("fluorescence", True))
+    trajs = trajectories_to_json(rec)
+    counts = {}
+    for t in trajs:
+        c = t["summary"]["category"]
+        counts[c] = counts.get(c, 0) + 1
+    out = args.out or args.scene.rsplit(".", 1)[0] + "_trace.json"
+    with open(out, "w") as f:
+        json.dump({"scene": args.scene, "n": n, "seed": args.seed, "categories": counts,
+                   "trajectories": trajs}, f)
+    print(f"wrote the trajectories of {n} photons to {out}")
+    return 0
+
+
 def main(argv=None) -> int:
     p = argparse.ArgumentParser(prog="chatcarlo_amd")
     sub = p.add_subparsers(dest="cmd", required=True)
@@ -96,6 +140,12 @@ def main(argv=None) -> int:
     pr.add_argument("--workers", type=int, default=1, help="accepted for compatibility; one GPU is used")
     pr.add_argument("--batch-size", type=int, default=200_000)
     pr.set_defaults(func=cmd_run)
+    pt = sub.add_parser("trace", help="record photon trajectories on the GPU (JSON)")
+    pt.add_argument("scene")
+    pt.add_argument("-n", type=int, default=100)
+    pt.add_argument("--seed", type=int, default=None)
+    pt.add_argument("--out")
+    pt.set_defaults(func=cmd_trace)
     args = p.parse_args(argv)
     return args.func(args)
 

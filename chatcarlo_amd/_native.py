@@ -16,6 +16,11 @@ CC_OK, CC_ERR_ARG, CC_ERR_HIP, CC_ERR_STATE, CC_ERR_DEVICE = 0, -1, -2, -3, -4
 CC_MAX_PRIMS, CC_MAX_MAT = 64, 32
 RUN_FLUORESCENCE, RUN_GRID, RUN_TRACK_UNCERTAINTY, RUN_TIMING = 1, 2, 4, 8
 RUN_REGION, RUN_RECORD, RUN_IMPORTANCE, RUN_SOURCE_BIAS = 16, 32, 64, 128
+RUN_TRACE = 256
+EV_BOUNDARY, EV_ESCAPE, EV_PHOTOELECTRIC, EV_FLUORESCENCE, EV_COMPTON, EV_RAYLEIGH = range(6)
+TRACE_DTYPE = np.dtype([("photon", np.uint32), ("flight", np.int32), ("step", np.int32), ("material", np.int32),
+                        ("start", np.float32, 3), ("end", np.float32, 3), ("energy_keV", np.float32),
+                        ("event", np.int32)])
 MLP_IN, MLP_HIDDEN = 16, 64
 MLP_PARAMS = MLP_IN * MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN + 1
 RECORD_FLOATS = 12
@@ -95,6 +100,11 @@ class CcImportanceCfg(C.Structure):
                 ("split_gain", C.c_float), ("pad", C.c_uint32), ("seed", C.c_uint64)]
 
 
+class CcTraceRecord(C.Structure):
+    _fields_ = [("photon", C.c_uint32), ("flight", C.c_int32), ("step", C.c_int32), ("material", C.c_int32),
+                ("start", C.c_float * 3), ("end", C.c_float * 3), ("energy_keV", C.c_float), ("event", C.c_int32)]
+
+
 class CcBatchTuning(C.Structure):
     _fields_ = [("block", C.c_int32), ("rounds_per_sync", C.c_int32), ("pool", C.c_uint32), ("pad", C.c_uint32)]
 
@@ -117,6 +127,8 @@ _SIGS = {
     "cc_bind_grid": (C.c_int, [C.c_void_p, dp, ip, C.c_double, C.c_int32]),
     "cc_unbind_grid": (C.c_int, [C.c_void_p]),
     "cc_set_grid_copies": (C.c_int, [C.c_void_p, C.c_int32]),
+    "cc_trace_alloc": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "cc_trace_read": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, u64p]),
     "cc_reset_grid": (C.c_int, [C.c_void_p]),
     "cc_read_grid": (C.c_int, [C.c_void_p, dp, dp, dp, dp, ip, i64p, i64p]),
     "cc_load_grid": (C.c_int, [C.c_void_p, dp, dp, dp, dp, ip, C.c_int64, C.c_int64]),

@@ -130,6 +130,18 @@ __device__ __forceinline__ void put_record(const Records& R, int kind, uint32_t 
   r[9] = __int_as_float(slot); r[10] = __uint_as_float(batch); r[11] = (float)kind;
 }
 
+// Segment log of traced transport runs: one record per flight segment, in the
+// layout of cc_trace_record (photon, flight, step, material, start[3], end[3],
+// energy, event).  A photoelectric segment's record index is kept per pool slot
+// so that the photo kernel can relabel it when a fluorescence photon leaves.
+struct Trace {
+  float* rec;              // [cap][12]
+  uint32_t* cnt;           // records appended (those past cap are dropped)
+  int* last;               // [pool] record of the slot's last photoelectric segment
+  uint32_t cap;
+  int on;
+};
+
 // Batched runs: many scenes in one particle pool.  Histories of all scenes form
 // one index space [0, total) (scene s owns [hist_start[s], hist_start[s+1]));
 // a device cursor hands them out in order to free pool slots.
@@ -166,6 +178,7 @@ struct Launch {
   uint32_t rec_batch;      // batch index written into contribution records
   int multi;               // batched run: scene data from MS, per-scene deposits (edep[scene][mat])
   MultiScene MS;
+  Trace Tr;
 };
 
 // Philox key of a photon's history: the run seed, or its scene's seed in a batched run.
@@ -557,7 +570,7 @@ __device__ float region_expect(const Prim* sp, const World& W, const Tables& T, 
 // separate instantiation so the plain transport keeps its register budget.
 // kMulti: batched run; every photon reads its scene's primitives (from global
 // memory, cached), world box, grid and seed, and a photon that leaves frees its slot.
-template <bool kRec, bool kMulti>
+template <bool kRec, bool kMulti, bool kTrace = false>
 __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, const uint32_t* cnt_in, int* qp,
                                                    int* qc, int* qr, int* coll) {
   __shared__ Prim sp_lds[kMulti ? 1 : CC_MAX_PRIMS];
@@ -579,7 +592,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     float p[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 1.f};
     float E = 1.0f, tau = 0.f, w = 0.f, mu = 0.f, muen = 0.f, hreg = 0.f, sreg = 0.f;
     double wh = 0.0;
-    int mc = -1, it = 0, scene = 0;
+    int mc = -1, it = 0, scene = 0, flight = 0;
     uint64_t key = 0;
     uint32_t hid = 0, tag = 0, ctr = 0;
     EPos ep{0, 0.f, 0.f};
@@ -601,6 +614,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
       w = S.wt[i];
       key = kMulti ? L.MS.seeds[scene] : L.seed;
       hid = S.hid[i]; tag = S.tag[i]; ctr = S.ctr[i];
+      if (kTrace) flight = S.nsc[i];
       if (!energy_ok(E)) {
         raise_err(L.err, CC_DEV_ENERGY_RANGE);
         S.flags[i] |= F_DEAD;
@@ -632,13 +646,22 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     while (__ballot(alive || pend)) {
       float ds = 0.f, taub = 0.f, t_next = 0.f;
       bool inter = false, esc = false, seg = false;
-      int m = 0;
+      bool logged = false;   // this step ends a segment of the trace log
+      int m = 0, ev = CC_EV_BOUNDARY;
+      float lp0[3] = {0.f, 0.f, 0.f};
       if (alive) {
         ++events;
         t_next = next_boundary(sp, Wd.n_prim, Wd, o0, d, it == 0 ? eps0 : tc, esc);
         if (!(t_next < INFINITY) || t_next < 0.f) {   // outside the world box
           escaped = true;
           alive = false;
+          if (kTrace) {   // a zero-length escape segment at the current point
+#pragma unroll
+            for (int k = 0; k < 3; ++k) lp0[k] = o0[k] + d[k] * tc;
+            m = mc < 0 ? Wd.background : mc;
+            ev = CC_EV_ESCAPE;
+            logged = true;
+          }
         } else {
           const float tb = t_next - tc;
           const float th = tc + 0.5f * tb;
@@ -656,6 +679,10 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
           inter = tau < taub;
           ds = inter ? tau / fmaxf(mu, 1e-30f) : tb;
           seg = L.grid_on != 0;
+          if (kTrace) {
+            lp0[0] = p[0]; lp0[1] = p[1]; lp0[2] = p[2];
+            logged = true;
+          }
         }
       }
       if (L.grid_on) {
@@ -691,18 +718,36 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
           S.mat[i] = m;
           tau = 0.f;
           alive = false;
+          if (kTrace) ev = type == 0 ? CC_EV_PHOTOELECTRIC : (type == 1 ? CC_EV_COMPTON : CC_EV_RAYLEIGH);
         } else {
           tau -= taub;
           tc = t_next;
           if (esc) {
             escaped = true;
             alive = false;
+            if (kTrace) ev = CC_EV_ESCAPE;
           } else if (++it >= kMaxCrossings) {
             raise_err(L.err, CC_DEV_STEP_CAP);
             S.flags[i] |= F_DEAD;
             alive = false;
           }
         }
+      }
+      if (kTrace) {
+        const int k = wave_append(logged, L.Tr.cnt);
+        if (logged && (uint32_t)k < L.Tr.cap) {
+          float* r = L.Tr.rec + (size_t)k * 12;
+          r[0] = __uint_as_float(hid);
+          r[1] = __int_as_float(flight);
+          r[2] = __int_as_float((int)events - 1);
+          r[3] = __int_as_float(m);
+          r[4] = lp0[0]; r[5] = lp0[1]; r[6] = lp0[2];
+#pragma unroll
+          for (int q = 0; q < 3; ++q) r[7 + q] = o0[q] + d[q] * tc;
+          r[10] = E;
+          r[11] = __int_as_float(ev);
+        }
+        if (logged && ev == CC_EV_PHOTOELECTRIC) L.Tr.last[i] = k;
       }
     }
     if (valid && energy_ok(E)) {
@@ -767,6 +812,10 @@ __global__ __launch_bounds__(kBlock) void k_photo(Launch L, const int* q, const 
       }
       dep = (double)w * (double)(emit ? E - eline : E);
       S.nsc[i] += 1;
+      if (emit && L.Tr.on) {
+        const int k = L.Tr.last[i];
+        if (k >= 0 && (uint32_t)k < L.Tr.cap) L.Tr.rec[(size_t)k * 12 + 11] = __int_as_float(CC_EV_FLUORESCENCE);
+      }
       if (emit) {
         float d[3];
         isotropic_dir(d, rng.next(), rng.next());
@@ -1742,6 +1791,12 @@ struct Ctx {
   size_t nvox = 0;
   bool track = false;
   int grid_copies = kGridCopies;   // batch-tally copies of the next cc_bind_grid
+  // segment trace of cc_transport
+  float* tr_rec = nullptr;
+  uint32_t* tr_cnt = nullptr;
+  int* tr_last = nullptr;
+  uint32_t tr_cap = 0;
+  size_t tr_last_cap = 0;
   int64_t g_batches = 0, g_hist = 0;
   // particle pool
   size_t cap = 0;
@@ -1980,6 +2035,9 @@ int run_rounds(Ctx* c, const Launch& L, uint32_t n, Timer& tm, uint32_t* rounds_
       if (o.record)
         hipLaunchKernelGGL((k_flight<true, false>), dim3(nb), dim3(kBlock), 0, c->stream, L, fl, fc, c->qp, c->qc,
                            c->qr, (int*)nullptr);
+      else if (L.Tr.on)
+        hipLaunchKernelGGL((k_flight<false, false, true>), dim3(nb), dim3(kBlock), 0, c->stream, L, fl, fc, c->qp,
+                           c->qc, c->qr, (int*)nullptr);
       else
         hipLaunchKernelGGL((k_flight<false, false>), dim3(nb), dim3(kBlock), 0, c->stream, L, fl, fc, c->qp, c->qc,
                            c->qr, o.pre ? c->coll : (int*)nullptr);
@@ -2179,7 +2237,7 @@ int cc_ctx_destroy(void* ctx) {
   (void)hipStreamSynchronize(c->stream);
   free_pool(c);
   for (void* p : c->tbufs) (void)hipFree(p);
-  void* ps[] = {c->prims, c->tk, c->th, c->qk, c->qh, c->hits, c->G.dk, c->G.dh, c->cnt, c->stats, c->edep, c->err,
+  void* ps[] = {c->tr_rec, c->tr_cnt, c->tr_last, c->prims, c->tk, c->th, c->qk, c->qh, c->hits, c->G.dk, c->G.dh, c->cnt, c->stats, c->edep, c->err,
                 c->spec, c->rec, c->rcnt, c->mw[0], c->mm[0], c->mv[0], c->mgrad[0], c->mwb[0],
                 c->mw[1], c->mm[1], c->mv[1], c->mgrad[1], c->mwb[1], c->bcdf, c->bwt, c->stage};
   for (void* p : ps)
@@ -2292,6 +2350,36 @@ int cc_bind_grid(void* ctx, const double* origin, const int32_t* shape, double v
   c->track = (flags & 1) != 0;
   c->have_grid = true;
   return cc_reset_grid(ctx);
+}
+
+int cc_trace_alloc(void* ctx, uint32_t capacity) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return fail(CC_ERR_ARG, "null ctx");
+  if (capacity == 0 || capacity > (1u << 26)) return fail(CC_ERR_ARG, "trace capacity %u out of range", capacity);
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->tr_rec) (void)hipFree(c->tr_rec);
+  c->tr_rec = nullptr;
+  c->tr_cap = 0;
+  int r;
+  if ((r = dalloc(&c->tr_rec, (size_t)capacity * 12))) return r;
+  if (!c->tr_cnt && (r = dalloc(&c->tr_cnt, 1))) return r;
+  c->tr_cap = capacity;
+  return CC_OK;
+}
+
+int cc_trace_read(void* ctx, uint64_t max_records, cc_trace_record* out, uint64_t* n_total) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c || !n_total) return fail(CC_ERR_ARG, "null argument");
+  if (!c->tr_rec) return fail(CC_ERR_STATE, "no trace buffer (cc_trace_alloc)");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  uint32_t cnt = 0;
+  HIP_TRY(hipMemcpy(&cnt, c->tr_cnt, sizeof(cnt), hipMemcpyDeviceToHost));
+  *n_total = cnt;
+  const uint64_t k = std::min<uint64_t>(std::min<uint64_t>(cnt, c->tr_cap), max_records);
+  if (k && out) HIP_TRY(hipMemcpy(out, c->tr_rec, k * sizeof(cc_trace_record), hipMemcpyDeviceToHost));
+  return CC_OK;
 }
 
 int cc_set_grid_copies(void* ctx, int32_t copies) {
@@ -2522,6 +2610,19 @@ int cc_transport(void* ctx, int64_t n, double* pos, double* dir, double* energy,
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->err, (size_t)1, 0);
   Timer tm{c, false};
   Launch L = make_launch(c, seed, first, grid_on ? 1 : 0, (flags & CC_RUN_FLUORESCENCE) ? 1 : 0);
+  if (flags & CC_RUN_TRACE) {
+    if (!c->tr_rec) return fail(CC_ERR_STATE, "CC_RUN_TRACE without a trace buffer (cc_trace_alloc)");
+    if (c->tr_last_cap < (size_t)n) {
+      if (c->tr_last) (void)hipFree(c->tr_last);
+      c->tr_last = nullptr;
+      c->tr_last_cap = 0;
+      if ((r = dalloc(&c->tr_last, (size_t)n))) return r;
+      c->tr_last_cap = (size_t)n;
+    }
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_blocks((size_t)n)), dim3(kBlock), 0, c->stream, c->tr_last, (size_t)n, -1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, (int*)c->tr_cnt, (size_t)1, 0);
+    L.Tr = Trace{c->tr_rec, c->tr_cnt, c->tr_last, c->tr_cap, 1};
+  }
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, c->stats, c->edep, (int)C_ACT_A, (uint32_t)n);
   hipLaunchKernelGGL(k_init_given, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, L, (int)n, c->act0);
   HIP_TRY(hipGetLastError());

@@ -76,3 +76,23 @@ class SourceRecord:
         s.spec_e = N.ptr(self.spec_e, N.C.c_double)
         s.spec_cdf = N.ptr(self.spec_cdf, N.C.c_double)
         self.struct = s
+
+
+def sample_source_photons(src: dict, n: int, rng: np.random.Generator):
+    """n source photons (pos, dir, energy as float64 arrays) drawn on the GPU
+    (chatcarlo/source.py:67-210 sample_source_photons).  The Philox seed is drawn from `rng`;
+    photon h uses the stream of history h, as in run_transport."""
+    import ctypes as C
+
+    from .transport import default_engine
+    eng = default_engine()
+    rec = SourceRecord(src)
+    n = int(n)
+    pos = np.zeros((n, 3), np.float32)
+    dr = np.zeros((n, 3), np.float32)
+    en = np.zeros(n, np.float32)
+    if n:
+        seed = int(rng.integers(0, 2 ** 63))
+        N.check(eng.lib.cc_test_source(eng.ctx, C.byref(rec.struct), n, seed, 0, N.ptr(pos, C.c_float),
+                                       N.ptr(dr, C.c_float), N.ptr(en, C.c_float)))
+    return pos.astype(np.float64), dr.astype(np.float64), en.astype(np.float64)

@@ -199,20 +199,40 @@ class Engine:
                                 int(batch), int(flags), C.byref(out)))
         return out
 
-    def transport(self, pos, dirv, energy, seed, first=0, flags=0):
+    def transport(self, pos, dirv, energy, seed, first=0, flags=0, trace=False):
+        """cc_transport on copies of the inputs; with trace=True also returns the segment log
+        (TRACE_DTYPE records; the call is repeated with a larger buffer if the first overflowed,
+        which gives the same results since they depend on the seed only)."""
         n = pos.shape[0]
-        p = np.ascontiguousarray(pos, dtype=np.float64)
-        d = np.ascontiguousarray(dirv, dtype=np.float64)
-        e = np.ascontiguousarray(energy, dtype=np.float64)
-        nsc = np.zeros(n, dtype=np.int32)
-        ab = np.zeros(n, dtype=np.uint8)
-        es = np.zeros(n, dtype=np.uint8)
-        out = N.CcRunOut()
-        N.check(self.lib.cc_transport(self.ctx, n, N.ptr(p, C.c_double), N.ptr(d, C.c_double),
-                                      N.ptr(e, C.c_double), int(seed), int(first), int(flags),
-                                      N.ptr(nsc, C.c_int32), N.ptr(ab, C.c_uint8), N.ptr(es, C.c_uint8),
-                                      C.byref(out)))
-        return p, d, e, nsc, ab.astype(bool), es.astype(bool), out
+        cap = max(4096, 64 * n)
+        while True:
+            p = np.array(pos, dtype=np.float64, order="C")
+            d = np.array(dirv, dtype=np.float64, order="C")
+            e = np.array(energy, dtype=np.float64, order="C")
+            nsc = np.zeros(n, dtype=np.int32)
+            ab = np.zeros(n, dtype=np.uint8)
+            es = np.zeros(n, dtype=np.uint8)
+            out = N.CcRunOut()
+            f = int(flags)
+            if trace:
+                N.check(self.lib.cc_trace_alloc(self.ctx, cap))
+                f |= N.RUN_TRACE
+            N.check(self.lib.cc_transport(self.ctx, n, N.ptr(p, C.c_double), N.ptr(d, C.c_double),
+                                          N.ptr(e, C.c_double), int(seed), int(first), f,
+                                          N.ptr(nsc, C.c_int32), N.ptr(ab, C.c_uint8), N.ptr(es, C.c_uint8),
+                                          C.byref(out)))
+            if not trace:
+                return p, d, e, nsc, ab.astype(bool), es.astype(bool), out
+            total = C.c_uint64()
+            N.check(self.lib.cc_trace_read(self.ctx, 0, None, C.byref(total)))
+            if total.value <= cap:
+                rec = np.zeros(total.value, dtype=N.TRACE_DTYPE)
+                N.check(self.lib.cc_trace_read(self.ctx, total.value, rec.ctypes.data_as(C.c_void_p),
+                                               C.byref(total)))
+                return p, d, e, nsc, ab.astype(bool), es.astype(bool), out, rec
+            cap = int(total.value)
+            if flags & N.RUN_GRID:
+                N.check(self.lib.cc_reset_grid(self.ctx))   # the repeated call scores the grid again
 
     # -- queries -------------------------------------------------------------------
     def locate(self, geometry, pts):
@@ -258,10 +278,11 @@ def transport_photons(pos: np.ndarray, dirv: np.ndarray, energy: np.ndarray, geo
 
     The Philox seed is drawn from `rng` (advancing it, as the reference's
     transport does); `tally_rng` is accepted for signature compatibility and
-    unused because the DDA tally draws no random numbers.
+    unused because the DDA tally draws no random numbers.  `recorder` (a
+    trajectory.TrajectoryRecorder) receives every flight segment from the GPU's
+    segment log; logging draws no random numbers, so the results are the same
+    with and without it.
     """
-    if recorder is not None:
-        raise NotImplementedError("trajectory recording is not part of the GPU transport path")
     eng = default_engine()
     eng.set_geometry(geometry)
     flags = N.RUN_FLUORESCENCE if fluorescence_enabled else 0
@@ -269,7 +290,12 @@ def transport_photons(pos: np.ndarray, dirv: np.ndarray, energy: np.ndarray, geo
         eng.bind_grid(grid.origin_cm, grid.shape, grid.voxel_size_cm, False)
         flags |= N.RUN_GRID
     seed = int(rng.integers(0, 2 ** 63))
-    p, d, e, nsc, ab, es, out = eng.transport(pos, dirv, energy, seed, 0, flags)
+    if recorder is not None:
+        from .trajectory import fill_recorder
+        p, d, e, nsc, ab, es, out, rec = eng.transport(pos, dirv, energy, seed, 0, flags, trace=True)
+        fill_recorder(recorder, rec, eng.pack.materials)
+    else:
+        p, d, e, nsc, ab, es, out = eng.transport(pos, dirv, energy, seed, 0, flags)
     pos[...] = p
     dirv[...] = d
     energy[...] = e

@@ -109,7 +109,8 @@ enum {
   CC_RUN_REGION = 16,           /* H*(10) track-length score in the region set by cc_set_region */
   CC_RUN_RECORD = 32,           /* contribution records (analog runs only, needs CC_RUN_REGION) */
   CC_RUN_IMPORTANCE = 64,       /* weight windows from the learned importance map */
-  CC_RUN_SOURCE_BIAS = 128      /* source cells drawn by learned importance (cc_set_importance) */
+  CC_RUN_SOURCE_BIAS = 128,     /* source cells drawn by learned importance (cc_set_importance) */
+  CC_RUN_TRACE = 256            /* cc_transport: log every flight segment (cc_trace_alloc, cc_trace_read) */
 };
 
 typedef struct {
@@ -253,6 +254,29 @@ int cc_mlp_train(void* ctx, int32_t map, const cc_train_cfg* cfg, cc_train_out* 
  * cell_prob (512 floats) may be NULL; cw_out receives the cw in use. */
 int cc_set_importance(void* ctx, const cc_importance_cfg* cfg, const cc_source* src, float* cw_out,
                       float* cell_prob);
+
+/* Segment trace of cc_transport (chatcarlo/trajectory.py:15-43 TrajectoryRecorder, filled at
+ * chatcarlo/transport.py:219-229): with CC_RUN_TRACE, every flight segment of every photon is logged,
+ * from its start to the boundary crossed, the interaction point or the world exit.  Logging draws no
+ * random numbers: results are those of the same call without it.  The records are in no particular
+ * order; sorting by (photon, flight, step) gives each photon's segments in time order. */
+enum { CC_EV_BOUNDARY = 0, CC_EV_ESCAPE = 1, CC_EV_PHOTOELECTRIC = 2, CC_EV_FLUORESCENCE = 3,
+       CC_EV_COMPTON = 4, CC_EV_RAYLEIGH = 5 };
+typedef struct {
+  uint32_t photon;              /* index of the photon in the cc_transport call */
+  int32_t flight;               /* interactions the photon had before this segment */
+  int32_t step;                 /* segment index within its flight */
+  int32_t material;             /* material id of the segment */
+  float start[3], end[3];       /* cm */
+  float energy_keV;             /* photon energy along the segment */
+  int32_t event;                /* CC_EV_*: what ends the segment */
+} cc_trace_record;
+/* Allocates room for `capacity` records (replacing an earlier buffer). */
+int cc_trace_alloc(void* ctx, uint32_t capacity);
+/* Copies min(records logged, capacity, max_records) records of the last traced call into out
+ * (may be NULL) and sets *n_total to the records logged; n_total > capacity means records were
+ * dropped and the call should be repeated with a larger buffer. */
+int cc_trace_read(void* ctx, uint64_t max_records, cc_trace_record* out, uint64_t* n_total);
 
 /* Batched runs (no counterpart in the reference, which runs one scene per call): many scenes in
  * one particle pool.  Scene s's histories 0 .. n_s - 1 draw from the streams keyed by (seed_s, h),

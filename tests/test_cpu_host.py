@@ -48,6 +48,12 @@ def test_ctypes_layouts_match_the_header(tmp_path):
               ("offsetof(cc_train_out, n_positive)", N.CcTrainOut.n_positive.offset),
               ("sizeof(cc_importance_cfg)", C.sizeof(N.CcImportanceCfg)),
               ("offsetof(cc_importance_cfg, seed)", N.CcImportanceCfg.seed.offset),
+              ("sizeof(cc_trace_record)", C.sizeof(N.CcTraceRecord)),
+              ("sizeof(cc_trace_record)", N.TRACE_DTYPE.itemsize),
+              ("offsetof(cc_trace_record, event)", N.CcTraceRecord.event.offset),
+              ("offsetof(cc_trace_record, event)", N.TRACE_DTYPE.fields["event"][1]),
+              ("CC_RUN_TRACE", N.RUN_TRACE), ("CC_EV_FLUORESCENCE", N.EV_FLUORESCENCE),
+              ("CC_EV_RAYLEIGH", N.EV_RAYLEIGH),
               ("sizeof(cc_batch_tuning)", C.sizeof(N.CcBatchTuning)),
               ("offsetof(cc_batch_tuning, pool)", N.CcBatchTuning.pool.offset),
               ("sizeof(cc_batch_out)", C.sizeof(N.CcBatchOut)),
