@@ -16,16 +16,6 @@ import sys
 import numpy as np
 
 
-def dose_map_Gy(grid, geometry):
-    """Kerma -> Gy with the material density at each voxel centre (chatcarlo/diagnostics.py:16-26)."""
-    from .materials import density
-    centers = grid.voxel_centers()
-    mats = geometry.material_at(centers)
-    dens = {m: density(m) for m in set(mats.tolist())}
-    rho = np.array([dens[m] for m in mats]).reshape(grid.shape)
-    return grid.dose_map_Gy(rho)
-
-
 def cmd_validate(args) -> int:
     from .scene import load_scene
     scene = load_scene(args.scene)
@@ -39,6 +29,7 @@ def cmd_validate(args) -> int:
 
 
 def cmd_run(args) -> int:
+    from .diagnostics import background_medium_warning, dose_map_Gy, max_voxel_position_cm, near_source_air_warning
     from .geometry import Geometry
     from .scene import load_scene
     from .transport import run_transport
@@ -62,6 +53,11 @@ def cmd_run(args) -> int:
     for name, e in sorted(result.energy_deposited_MeV.items(), key=lambda kv: -kv[1]):
         rel = result.energy_deposited_rel_err.get(name)
         print(f"  {name}: {e:.6g}" + (f"  (R = {rel:.3g})" if rel is not None and np.isfinite(rel) else ""))
+    src = scene.raw["source"]
+    scale = result.n_photons_real
+    if scale is not None:
+        print(f"dose calibration: CTDIvol = {src['ctdi_vol_mGy']:g} mGy ({src.get('ctdi_phantom', 'body')} "
+              f"phantom), effective photon count = {scale:.6g}")
     if args.dose_grid:
         geometry = Geometry(scene.raw["geometry"])
         grid = result.grid
@@ -73,10 +69,27 @@ def cmd_run(args) -> int:
         print(f"  max absorbed dose [Gy/history]: {dose_ph.max():.6g}")
         print(f"  total kerma: {grid.total_kerma_MeV():.6g} MeV ({grid.total_kerma_MeV() / n * 1000:.6g} keV/history)")
         print(f"  max H*(10) [pSv/history]: {h10_ph.max():.6g}")
+        if scale is not None:
+            print(f"  max absorbed dose [Gy] (CTDIvol-calibrated): {dose_ph.max() * scale:.6g}")
+            print(f"  max H*(10) [pSv] (CTDIvol-calibrated): {h10_ph.max() * scale:.6g}")
+        # is a maximum sitting where it means nothing (chatcarlo/__main__.py:213-226)?
+        src_pos = np.asarray(src["position"], dtype=float)
+        dpos = max_voxel_position_cm(grid, dose_ph)
+        w = background_medium_warning(str(geometry.material_at(dpos[None, :])[0]), geometry.background)
+        if w:
+            print(f"[warning] {w}")
+        hpos = max_voxel_position_cm(grid, h10_ph)
+        w = near_source_air_warning(str(geometry.material_at(hpos[None, :])[0]), geometry.background,
+                                    float(np.linalg.norm(hpos - src_pos)),
+                                    geometry.nearest_object_distance_cm(src_pos))
+        if w:
+            print(f"[warning] {w}")
         if args.dose_out:
+            extra = {} if scale is None else {"dose_Gy": dose_ph * scale, "h10_pSv": h10_ph * scale}
             np.savez(args.dose_out, dose_per_history_Gy=dose_ph, h10_per_history_pSv=h10_ph,
                      kerma_keV=grid.kerma_keV, h10_track_pSv_cm3=grid.h10_track_pSv_cm3,
-                     origin_cm=grid.origin_cm, voxel_size_cm=grid.voxel_size_cm, shape=np.array(grid.shape))
+                     origin_cm=grid.origin_cm, voxel_size_cm=grid.voxel_size_cm, shape=np.array(grid.shape),
+                     **extra)
             print(f"wrote dose grid: {args.dose_out}")
     return 0
 

@@ -327,9 +327,7 @@ def run_transport(scene, n_histories: int = 100_000, seed: int | None = None,
         flags |= N.RUN_GRID
         if track_uncertainty:
             flags |= N.RUN_TRACK_UNCERTAINTY
-    if src.get("ctdi_vol_mGy") is not None:
-        raise NotImplementedError("CTDIvol calibration is not part of the GPU transport path")
-    if src.get("mas") is not None:
+    if src.get("mas") is not None and src.get("ctdi_vol_mGy") is None:
         raise RuntimeError("photon-count calibration (mas) needs spekpy, which is not installed")
     rec = SourceRecord(src)
     out = eng.run(rec, n_histories, _seed_from(seed), 0, batch_size, flags)
@@ -352,6 +350,12 @@ def run_transport(scene, n_histories: int = 100_000, seed: int | None = None,
             grid.n_batches_hit[...] = g["hits"]
             grid.n_batches = int(g["n_batches"])
             grid.n_histories = int(g["n_histories"])
+    n_real = None
+    if src.get("ctdi_vol_mGy") is not None:
+        # after the grid is read: the calibration binds the phantom's own scene and grid
+        # (chatcarlo/transport.py:445-447, same master seed)
+        from .ctdi import effective_histories_from_ctdi
+        n_real = effective_histories_from_ctdi(src, seed=seed, engine=eng)
     n = max(int(n_histories), 1)
     if track_uncertainty:
         sem = {k: v / 1000.0 for k, v in moments.standard_errors().items()}
@@ -367,6 +371,6 @@ def run_transport(scene, n_histories: int = 100_000, seed: int | None = None,
         n_histories=int(n_histories),
         energy_deposited_MeV={k: v / 1000.0 for k, v in edep_keV.items()},
         fraction_absorbed=out.n_absorbed / n, fraction_escaped=out.n_escaped / n,
-        mean_scatter_events=out.scatter_sum / n, grid=grid, n_photons_real=None,
+        mean_scatter_events=out.scatter_sum / n, grid=grid, n_photons_real=n_real,
         n_fluorescence=int(out.n_fluorescence), n_batches=n_batches,
         energy_deposited_sem_MeV=sem, energy_deposited_rel_err=rel, device_stats=stats)
