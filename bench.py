@@ -139,7 +139,8 @@ def batched_sweep(eng, autotune_fraction=0.02, sequential=True):
     wall = time.perf_counter() - t0
     out = {"workload": "64_random_scenes_300cm_air_room_2cm_grids", "scenes": len(scenes),
            "histories": st.n_histories, "tuning": {"block": tune.block, "rounds_per_sync": tune.rounds_per_sync,
-                                                   "pool": tune.pool},
+                                                   "pool": tune.pool,
+                                                   "tally_copies": tune.tally_copies},
            "histories_per_s": st.n_histories / wall, "device_ms": st.device_ms, "wall_s": wall,
            "rounds": st.rounds, "graph_launches": st.graph_launches,
            "voxel_visits_per_history": st.n_voxel_visits / st.n_histories,

@@ -40,13 +40,15 @@ class SceneSpec:
 @dataclass
 class Tuning:
     """Launch shape of a batched run: threads per block of the transport kernels, rounds
-    between host checks of the queue (even), and photons in flight."""
+    between host checks of the queue (even), photons in flight, and copies of the scene
+    tallies (workgroup b adds into copy b mod copies)."""
     block: int = 256
     rounds_per_sync: int = 8
     pool: int = 1 << 20
+    tally_copies: int = 8
 
     def struct(self):
-        return N.CcBatchTuning(int(self.block), int(self.rounds_per_sync), int(self.pool), 0)
+        return N.CcBatchTuning(int(self.block), int(self.rounds_per_sync), int(self.pool), int(self.tally_copies))
 
 
 @dataclass
@@ -136,7 +138,8 @@ class BatchedRun:
         rows = C.c_int32()
         N.check(self.eng.lib.cc_batch_autotune(self.eng.ctx, float(fraction), self.flags, C.byref(best),
                                                N.ptr(table, C.c_double), 64, C.byref(rows)))
-        return Tuning(best.block, best.rounds_per_sync, best.pool), table[:min(rows.value, 64)].copy()
+        return (Tuning(best.block, best.rounds_per_sync, best.pool, best.tally_copies),
+                table[:min(rows.value, 64)].copy())
 
     def histories(self, k: int) -> int:
         """Histories of scene k in the last run: ceil(fraction n_k), at least 1."""

@@ -574,8 +574,8 @@ template <bool kRec, bool kMulti, bool kTrace = false>
 __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, const uint32_t* cnt_in, int* qp,
                                                    int* qc, int* qr, int* coll) {
   __shared__ Prim sp_lds[kMulti ? 1 : CC_MAX_PRIMS];
-  __shared__ SegStage seg_stage[kBlock / kWave];
-  SegStage& sst = seg_stage[threadIdx.x / kWave];
+  __shared__ SegStage seg_stage[kMulti ? 1 : kBlock / kWave];
+  SegStage& sst = seg_stage[kMulti ? 0 : threadIdx.x / kWave];
   if (!kMulti) stage_prims(L.prims, sp_lds, L.W.n_prim);
   const uint32_t n = *cnt_in;
   const Tables& T = L.T;
@@ -686,7 +686,13 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
         }
       }
       if (L.grid_on) {
-        visits += dda_balanced<kMulti>(G, L.MS.grids, sst, pend_p, d, pend_len, pend_wk, wh, pend, (uint32_t)scene);
+        // one scene: the wave's segments are walked work-balanced (long segments on fine grids
+        // leave most lanes idle otherwise); several scenes: each lane walks its own segment, as
+        // the balanced walk's per-chunk scene lookups cost more than balancing gains there
+        if constexpr (kMulti)
+          visits += dda_score<true>(G, pend_p, d, pend_len, pend_wk, wh, pend, (uint32_t)scene);
+        else
+          visits += dda_balanced<false>(G, L.MS.grids, sst, pend_p, d, pend_len, pend_wk, wh, pend, 0u);
         pend_p[0] = p[0]; pend_p[1] = p[1]; pend_p[2] = p[2];
         pend_len = ds;
         pend_wk = (double)w * (double)E * (double)muen;
@@ -1609,6 +1615,15 @@ __global__ void k_fill_f64(double* p, size_t n, double v) {
 __global__ void k_fill_i32(int* p, size_t n, int v) {
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = v;
 }
+// a[v] += a[k n + v] for k = 1 .. ncopy - 1 (tally copies folded into copy 0).
+__global__ void k_fold_copies(double* a, size_t n, int ncopy) {
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int k = 1; k < ncopy; ++k) s += a[(size_t)k * n + v];
+    if (ncopy > 1) a[v] += s;
+  }
+}
+
 __global__ void k_add_f64(double* dst, const double* src, size_t n) {
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) dst[k] += src[k];
 }
@@ -1842,7 +1857,8 @@ struct Ctx {
   uint64_t* b_seeds = nullptr;
   uint64_t* b_hist_start = nullptr;
   float* b_spec = nullptr;
-  double* b_tally = nullptr;           // all scenes' kerma then H*(10) totals
+  double* b_tally = nullptr;           // copies of (all scenes' kerma, then all scenes' H*(10)) totals
+  size_t b_tally_cap = 0;              // doubles allocated in b_tally
   size_t b_nvox = 0;
   std::vector<size_t> b_vox_off;       // first voxel of each scene
   std::vector<GridDev> b_grid_host;
@@ -1863,6 +1879,7 @@ void free_batch(Ctx* c) {
   c->b_prims = nullptr; c->b_prim_off = nullptr; c->b_worlds = nullptr; c->b_grids = nullptr;
   c->b_srcs = nullptr; c->b_seeds = nullptr; c->b_hist_start = nullptr; c->b_spec = nullptr;
   c->b_tally = nullptr; c->b_edep = nullptr;
+  c->b_tally_cap = 0;
   c->nscene = 0;
   c->b_nvox = 0;
   c->b_vox_off.clear();
@@ -3284,6 +3301,7 @@ int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int
     G.inv_h = (float)(1.0 / voxel_cm[s]);
   }
   HIP_TRY(hipMemcpy(c->b_grids, c->b_grid_host.data(), sizeof(GridDev) * n_scenes, hipMemcpyHostToDevice));
+  c->b_tally_cap = 2 * nvox;
   c->scene_hist.assign(n_histories, n_histories + n_scenes);
   c->b_vox_off = voff;
   c->b_nvox = nvox;
@@ -3301,9 +3319,32 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
     return fail(CC_ERR_ARG, "rounds_per_sync must be even, 2..256");
   if (tune->pool < 1024 || tune->pool > (1u << 28)) return fail(CC_ERR_ARG, "pool %u out of range", tune->pool);
   if (!(fraction > 0.0 && fraction <= 1.0)) return fail(CC_ERR_ARG, "fraction must be in (0, 1]");
+  const uint32_t ncopy = tune->tally_copies ? tune->tally_copies : 1u;
+  if (ncopy > 16 || (ncopy & (ncopy - 1))) return fail(CC_ERR_ARG, "tally_copies %u must be 0 or a power of two <= 16", ncopy);
   HIP_TRY(hipSetDevice(c->device));
   std::memset(out, 0, sizeof(*out));
   const int n = c->nscene;
+  // tally copies: copy k of every scene's grid lies k * 2 nvox doubles after copy 0
+  {
+    const size_t need = (size_t)ncopy * 2 * c->b_nvox;
+    if (need > c->b_tally_cap) {
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      (void)hipFree(c->b_tally);
+      c->b_tally = nullptr;
+      c->b_tally_cap = 0;
+      int ra = dalloc(&c->b_tally, need);
+      if (ra) return ra;
+      c->b_tally_cap = need;
+    }
+    for (int s = 0; s < n; ++s) {
+      GridDev& G = c->b_grid_host[s];
+      G.dk = c->b_tally + c->b_vox_off[s];
+      G.dh = c->b_tally + c->b_nvox + c->b_vox_off[s];
+      G.cmask = ncopy - 1;
+      G.cstride = 2 * c->b_nvox;
+    }
+    HIP_TRY(hipMemcpy(c->b_grids, c->b_grid_host.data(), sizeof(GridDev) * n, hipMemcpyHostToDevice));
+  }
   std::vector<uint64_t> start(n + 1, 0);
   for (int s = 0; s < n; ++s) {
     const uint64_t m = std::max<uint64_t>(1, (uint64_t)std::ceil((double)c->scene_hist[s] * fraction));
@@ -3326,8 +3367,8 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
     return r;
   if (!c->h_status) HIP_TRY(hipHostMalloc((void**)&c->h_status, 2 * sizeof(unsigned long long), hipHostMallocDefault));
   // fresh tallies, deposits, counters, free slots and queue
-  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(2 * c->b_nvox)), dim3(kBlock), 0, c->stream, c->b_tally,
-                     2 * c->b_nvox, 0.0);
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks((size_t)ncopy * 2 * c->b_nvox)), dim3(kBlock), 0, c->stream, c->b_tally,
+                     (size_t)ncopy * 2 * c->b_nvox, 0.0);
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks((size_t)n * CC_MAX_MAT)), dim3(kBlock), 0, c->stream, c->b_edep,
                      (size_t)n * CC_MAX_MAT, 0.0);
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, c->stats, c->edep, (int)C_ACT_A, 0u);
@@ -3403,6 +3444,11 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
   (void)hipGraphExecDestroy(exec);
   (void)hipGraphDestroy(graph);
   if (rc) return rc;
+  if (ncopy > 1) {
+    hipLaunchKernelGGL(k_fold_copies, dim3(grid_blocks(2 * c->b_nvox)), dim3(kBlock), 0, c->stream, c->b_tally,
+                       2 * c->b_nvox, (int)ncopy);
+    HIP_TRY(hipGetLastError());
+  }
   unsigned long long st[S_NSTAT];
   int e = 0;
   HIP_TRY(hipMemcpy(st, c->stats, sizeof(st), hipMemcpyDeviceToHost));
@@ -3450,7 +3496,7 @@ int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tunin
   for (int b : blocks)
     for (uint32_t p : pools)
       for (int q : rps) {
-        cc_batch_tuning t{b, q, p, 0u};
+        cc_batch_tuning t{b, q, p, 8u};
         cc_batch_out o;
         int r = cc_batch_run(ctx, &t, fraction, flags, &o);
         if (r) return r;

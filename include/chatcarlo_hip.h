@@ -288,7 +288,8 @@ typedef struct {
   int32_t block;                /* threads per block of the transport kernels (64, 128, 192 or 256) */
   int32_t rounds_per_sync;      /* rounds between host checks of the queue (even) */
   uint32_t pool;                /* photons in flight */
-  uint32_t pad;
+  uint32_t tally_copies;        /* copies of the scene tallies (0 or a power of two <= 16; 0 = 1): workgroup
+                                   b adds into copy b mod copies, folded once the queue has drained */
 } cc_batch_tuning;
 
 typedef struct {
@@ -308,8 +309,9 @@ int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int
 /* Runs ceil(fraction n_s) histories of every scene (fraction in (0, 1]); tallies start from zero. */
 int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32_t flags, cc_batch_out* out);
 int cc_batch_read(void* ctx, int32_t scene, double* kerma_keV, double* h10_pSv_cm3, double* edep_keV);
-/* Times every (block, rounds_per_sync, pool) candidate on a fraction of the histories and returns
- * the fastest; table (4 doubles per row: block, rounds_per_sync, pool, histories/s) may be NULL. */
+/* Times every (block, rounds_per_sync, pool) candidate, with 8 tally copies, on a fraction of the
+ * histories and returns the fastest; table (4 doubles per row: block, rounds_per_sync, pool,
+ * histories/s) may be NULL. */
 int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tuning* best, double* table,
                       int32_t max_rows, int32_t* n_rows);
 
