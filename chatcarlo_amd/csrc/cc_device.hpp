@@ -570,6 +570,40 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Block-private cache of tally contributions in LDS: a direct-mapped table of
+// (voxel, kerma, H*(10)) sums.  The photons of one block share the hot voxels of a beam
+// across waves and steps, which the wave-level aggregation cannot see; adds to a voxel
+// that owns its slot cost LDS atomics instead of L2 atomics.  A voxel whose slot another
+// voxel owns goes straight to memory; the table is added into the grid once, when the
+// block ends.  (Probing further slots was measured slower: when the table misses, the
+// voxels are not shared anyway.)
+constexpr int kCacheSlots = 2048;
+constexpr int kCacheLog2 = 11;
+constexpr uint32_t kCacheEmpty = 0xffffffffu;
+struct TallyCache {
+  uint32_t key[kCacheSlots];
+  double val[kCacheSlots][2];
+};
+
+__device__ __forceinline__ void cache_init(TallyCache* c) {
+  for (int k = threadIdx.x; k < kCacheSlots; k += blockDim.x) {
+    c->key[k] = kCacheEmpty;
+    c->val[k][0] = 0.0;
+    c->val[k][1] = 0.0;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool cache_add(TallyCache* c, uint32_t v, double a, double b) {
+  const uint32_t s = (v * 2654435761u) >> (32 - kCacheLog2);
+  uint32_t cur = c->key[s];
+  if (cur == kCacheEmpty) cur = atomicCAS(&c->key[s], kCacheEmpty, v);
+  if (cur != kCacheEmpty && cur != v) return false;
+  atomicAdd(&c->val[s][0], a);
+  atomicAdd(&c->val[s][1], b);
+  return true;
+}
+
 // Sum of a double over all 64 lanes of the wave; every lane must be active.
 __device__ __forceinline__ double wave_sum_f64(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -585,7 +619,8 @@ constexpr int kAggRounds = 4;
 // kScene: lanes may score different grids (batched runs); only lanes of one
 // scene (key) and one voxel are combined.
 template <bool kScene>
-__device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t v, double ak, double ah, uint32_t key) {
+__device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t v, double ak, double ah, uint32_t key,
+                                           TallyCache* tc = nullptr) {
   const int lane = __lane_id();
   const size_t copy = (size_t)(blockIdx.x & G.cmask) * G.cstride;
   double* const dk = G.dk + copy;
@@ -600,14 +635,14 @@ __device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t 
     if (m == (1ull << leader)) break;          // no partner: the rest go direct
     const double sk = wave_sum_f64(mine ? ak : 0.0);
     const double sh = wave_sum_f64(mine ? ah : 0.0);
-    if (lane == leader) {
+    if (lane == leader && !(tc && cache_add(tc, vl, sk, sh))) {
       atomic_add_f64(dk + vl, sk);
       atomic_add_f64(dh + vl, sh);
     }
     has = has && !mine;
     pend &= ~m;
   }
-  if (has) {
+  if (has && !(tc && cache_add(tc, v, ak, ah))) {
     atomic_add_f64(dk + v, ak);
     atomic_add_f64(dh + v, ah);
   }
@@ -652,7 +687,8 @@ __device__ __forceinline__ float plane_t(float org_a, float h, float o_a, float 
 // Returns the number of voxel visits of the calling lane.
 template <bool kScene>
 __device__ __forceinline__ int dda_walk(const GridDev& G, const float o[3], const float d[3], float t0, float t1,
-                                        double wk, double wh, bool act, uint32_t key, int fa = -1, int fidx = 0) {
+                                        double wk, double wh, bool act, uint32_t key, int fa = -1, int fidx = 0,
+                                        TallyCache* cache = nullptr) {
   const int n[3] = {G.nx, G.ny, G.nz};
   int idx[3] = {0, 0, 0}, step[3] = {0, 0, 0};
   float inv_d[3] = {0.f, 0.f, 0.f};
@@ -707,7 +743,7 @@ __device__ __forceinline__ int dda_walk(const GridDev& G, const float o[3], cons
         if (out) act = false;
       }
     }
-    wave_tally<kScene>(G, has, v, ak, ah, key);
+    wave_tally<kScene>(G, has, v, ak, ah, key, cache);
   }
   return visits;
 }
@@ -749,7 +785,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // kScene (key = scene index), else unused.  All lanes of the wave must call it.
 template <bool kScene>
 __device__ __forceinline__ int dda_balanced(const GridDev& G0, const GridDev* grids, SegStage& st, const float o[3], const float d[3],
-                            float L, double wk, double wh, bool seg, uint32_t key) {
+                            float L, double wk, double wh, bool seg, uint32_t key, TallyCache* cache = nullptr) {
   const int lane = __lane_id();
   float t0 = 0.f, t1 = 0.f;
   if (seg) seg = dda_clip(G0, o, d, L, t0, t1);
@@ -850,7 +886,7 @@ __device__ __forceinline__ int dda_balanced(const GridDev& G0, const GridDev* gr
       const GridDev Gs = grids[act ? skey : 0];
       visits += dda_walk<true>(Gs, so, sd, ta, tb, swk, swh, act, skey, sa, fidx);
     } else {
-      visits += dda_walk<false>(G0, so, sd, ta, tb, swk, swh, act, skey, sa, fidx);
+      visits += dda_walk<false>(G0, so, sd, ta, tb, swk, swh, act, skey, sa, fidx, cache);
     }
   }
   return visits;

@@ -576,6 +576,12 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
   __shared__ Prim sp_lds[kMulti ? 1 : CC_MAX_PRIMS];
   __shared__ SegStage seg_stage[kMulti ? 1 : kBlock / kWave];
   SegStage& sst = seg_stage[kMulti ? 0 : threadIdx.x / kWave];
+  TallyCache* tcache = nullptr;
+  if constexpr (!kMulti) {
+    __shared__ TallyCache tc_lds;
+    tcache = &tc_lds;
+    if (L.grid_on) cache_init(tcache);
+  }
   if (!kMulti) stage_prims(L.prims, sp_lds, L.W.n_prim);
   const uint32_t n = *cnt_in;
   const Tables& T = L.T;
@@ -692,7 +698,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
         if constexpr (kMulti)
           visits += dda_score<true>(G, pend_p, d, pend_len, pend_wk, wh, pend, (uint32_t)scene);
         else
-          visits += dda_balanced<false>(G, L.MS.grids, sst, pend_p, d, pend_len, pend_wk, wh, pend, 0u);
+          visits += dda_balanced<false>(G, L.MS.grids, sst, pend_p, d, pend_len, pend_wk, wh, pend, 0u, tcache);
         pend_p[0] = p[0]; pend_p[1] = p[1]; pend_p[2] = p[2];
         pend_len = ds;
         pend_wk = (double)w * (double)E * (double)muen;
@@ -787,6 +793,19 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     if (slot >= 0) qr[slot] = i;
     slot = wave_append(type == 3, L.cnt + C_COLL);
     if (slot >= 0) coll[slot] = i;
+  }
+  if constexpr (!kMulti) {
+    if (L.grid_on) {   // the block's cached tallies into its copy of the grid
+      __syncthreads();
+      const size_t copy = (size_t)(blockIdx.x & L.G.cmask) * L.G.cstride;
+      for (int k = threadIdx.x; k < kCacheSlots; k += blockDim.x) {
+        const uint32_t v = tcache->key[k];
+        if (v != kCacheEmpty) {
+          atomic_add_f64(L.G.dk + copy + v, tcache->val[k][0]);
+          atomic_add_f64(L.G.dh + copy + v, tcache->val[k][1]);
+        }
+      }
+    }
   }
 }
 
