@@ -562,7 +562,7 @@ struct GridDev {
   float h;
   float inv_h;
   uint32_t cmask; // copies - 1 (copies a power of two)
-  uint32_t pad;
+  uint32_t base;  // voxel 0's index in the tally-cache key space (batched scenes: its scene's offset)
   size_t cstride; // elements between copies
 };
 
@@ -635,14 +635,14 @@ __device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t 
     if (m == (1ull << leader)) break;          // no partner: the rest go direct
     const double sk = wave_sum_f64(mine ? ak : 0.0);
     const double sh = wave_sum_f64(mine ? ah : 0.0);
-    if (lane == leader && !(tc && cache_add(tc, vl, sk, sh))) {
+    if (lane == leader && !(tc && cache_add(tc, G.base + vl, sk, sh))) {
       atomic_add_f64(dk + vl, sk);
       atomic_add_f64(dh + vl, sh);
     }
     has = has && !mine;
     pend &= ~m;
   }
-  if (has && !(tc && cache_add(tc, v, ak, ah))) {
+  if (has && !(tc && cache_add(tc, G.base + v, ak, ah))) {
     atomic_add_f64(dk + v, ak);
     atomic_add_f64(dh + v, ah);
   }
@@ -753,10 +753,11 @@ __device__ __forceinline__ int dda_walk(const GridDev& G, const float o[3], cons
 // its own segment.  All lanes of the wave must call it.
 template <bool kScene = false>
 __device__ __forceinline__ int dda_score(const GridDev& G, const float o[3], const float d[3], float L,
-                                         double wk, double wh, bool seg, uint32_t key = 0) {
+                                         double wk, double wh, bool seg, uint32_t key = 0,
+                                         TallyCache* cache = nullptr) {
   float t0 = 0.f, t1 = 0.f;
   if (seg) seg = dda_clip(G, o, d, L, t0, t1);
-  return dda_walk<kScene>(G, o, d, t0, t1, wk, wh, seg, key);
+  return dda_walk<kScene>(G, o, d, t0, t1, wk, wh, seg, key, -1, 0, cache);
 }
 
 // Per-wave staging of the segments of a balanced tally.

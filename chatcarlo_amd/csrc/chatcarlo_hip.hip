@@ -158,6 +158,10 @@ struct MultiScene {
   uint32_t* free_cnt;
   unsigned long long* cursor; // next history of the index space
   unsigned long long total;
+  double* dk0;                // scene 0's kerma tally: the tally-cache keys index from here
+  double* dh0;                // scene 0's H*(10) tally
+  uint32_t cmask;             // tally copies - 1
+  size_t cstride;             // doubles between tally copies
 };
 
 struct Launch {
@@ -576,12 +580,9 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
   __shared__ Prim sp_lds[kMulti ? 1 : CC_MAX_PRIMS];
   __shared__ SegStage seg_stage[kMulti ? 1 : kBlock / kWave];
   SegStage& sst = seg_stage[kMulti ? 0 : threadIdx.x / kWave];
-  TallyCache* tcache = nullptr;
-  if constexpr (!kMulti) {
-    __shared__ TallyCache tc_lds;
-    tcache = &tc_lds;
-    if (L.grid_on) cache_init(tcache);
-  }
+  __shared__ TallyCache tc_lds;
+  TallyCache* tcache = &tc_lds;
+  if (L.grid_on) cache_init(tcache);
   if (!kMulti) stage_prims(L.prims, sp_lds, L.W.n_prim);
   const uint32_t n = *cnt_in;
   const Tables& T = L.T;
@@ -696,7 +697,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
         // leave most lanes idle otherwise); several scenes: each lane walks its own segment, as
         // the balanced walk's per-chunk scene lookups cost more than balancing gains there
         if constexpr (kMulti)
-          visits += dda_score<true>(G, pend_p, d, pend_len, pend_wk, wh, pend, (uint32_t)scene);
+          visits += dda_score<true>(G, pend_p, d, pend_len, pend_wk, wh, pend, (uint32_t)scene, tcache);
         else
           visits += dda_balanced<false>(G, L.MS.grids, sst, pend_p, d, pend_len, pend_wk, wh, pend, 0u, tcache);
         pend_p[0] = p[0]; pend_p[1] = p[1]; pend_p[2] = p[2];
@@ -794,16 +795,17 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     slot = wave_append(type == 3, L.cnt + C_COLL);
     if (slot >= 0) coll[slot] = i;
   }
-  if constexpr (!kMulti) {
-    if (L.grid_on) {   // the block's cached tallies into its copy of the grid
-      __syncthreads();
-      const size_t copy = (size_t)(blockIdx.x & L.G.cmask) * L.G.cstride;
-      for (int k = threadIdx.x; k < kCacheSlots; k += blockDim.x) {
-        const uint32_t v = tcache->key[k];
-        if (v != kCacheEmpty) {
-          atomic_add_f64(L.G.dk + copy + v, tcache->val[k][0]);
-          atomic_add_f64(L.G.dh + copy + v, tcache->val[k][1]);
-        }
+  if (L.grid_on) {   // the block's cached tallies into its copy of the grid(s)
+    __syncthreads();
+    double* const dk = kMulti ? L.MS.dk0 : L.G.dk;
+    double* const dh = kMulti ? L.MS.dh0 : L.G.dh;
+    const size_t copy = kMulti ? (size_t)(blockIdx.x & L.MS.cmask) * L.MS.cstride
+                               : (size_t)(blockIdx.x & L.G.cmask) * L.G.cstride;
+    for (int k = threadIdx.x; k < kCacheSlots; k += blockDim.x) {
+      const uint32_t v = tcache->key[k];
+      if (v != kCacheEmpty) {
+        atomic_add_f64(dk + copy + v, tcache->val[k][0]);
+        atomic_add_f64(dh + copy + v, tcache->val[k][1]);
       }
     }
   }
@@ -2370,6 +2372,7 @@ int cc_bind_grid(void* ctx, const double* origin, const int32_t* shape, double v
   HIP_TRY(hipSetDevice(c->device));
   cc_unbind_grid(ctx);
   const size_t nv = (size_t)shape[0] * shape[1] * shape[2];
+  if (nv >= 0xffffffffull) return fail(CC_ERR_ARG, "grid of %zu voxels: at most 2^32 - 2", nv);
   const size_t cs = (nv + 31) & ~(size_t)31;   // copies start on 256 B boundaries
   const size_t nc = (size_t)c->grid_copies;
   int r;
@@ -3275,6 +3278,7 @@ int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int
     voff[s] = nvox;
     nvox += (size_t)grid_shape[3 * s] * grid_shape[3 * s + 1] * grid_shape[3 * s + 2];
   }
+  if (nvox >= 0xffffffffull) return fail(CC_ERR_ARG, "%zu voxels over all scenes: at most 2^32 - 2", nvox);
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamSynchronize(c->stream));
   free_batch(c);
@@ -3361,6 +3365,7 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
       G.dh = c->b_tally + c->b_nvox + c->b_vox_off[s];
       G.cmask = ncopy - 1;
       G.cstride = 2 * c->b_nvox;
+      G.base = (uint32_t)c->b_vox_off[s];
     }
     HIP_TRY(hipMemcpy(c->b_grids, c->b_grid_host.data(), sizeof(GridDev) * n, hipMemcpyHostToDevice));
   }
@@ -3402,6 +3407,10 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
   L.MS.srcs = c->b_srcs; L.MS.seeds = c->b_seeds; L.MS.hist_start = c->b_hist_start; L.MS.n = n;
   L.MS.free_stack = c->b_free; L.MS.free_cnt = c->b_free_cnt; L.MS.cursor = c->b_cursor;
   L.MS.total = (unsigned long long)total;
+  L.MS.dk0 = c->b_tally;
+  L.MS.dh0 = c->b_tally + c->b_nvox;
+  L.MS.cmask = ncopy - 1;
+  L.MS.cstride = 2 * c->b_nvox;
   const int bs = tune->block;
   const int nb = (int)std::min<size_t>((pool + bs - 1) / bs, 8192);
   // two rounds (list A -> B -> A) and the status copy, captured once and replayed
