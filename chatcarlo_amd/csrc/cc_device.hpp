@@ -628,9 +628,13 @@ __device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t 
   unsigned long long pend = __ballot(has);
   for (int r = 0; r < kAggRounds && pend; ++r) {
     const int leader = __ffsll((long long)pend) - 1;
+    // lanes match on the voxel's key (G.base + v): equal keys are one voxel of one grid
+    // (one scene, or one batch slot of a streamed run), so the leader's grid is theirs
+    const uint32_t gv = G.base + v;
     const uint32_t vl = __shfl(v, leader);
+    const uint32_t gl = __shfl(gv, leader);
     const uint32_t kl = kScene ? (uint32_t)__shfl((int)key, leader) : 0u;
-    const bool mine = has && v == vl && (!kScene || key == kl);
+    const bool mine = has && gv == gl && (!kScene || key == kl);
     const unsigned long long m = __ballot(mine);
     if (m == (1ull << leader)) break;          // no partner: the rest go direct
     const double sk = wave_sum_f64(mine ? ak : 0.0);
@@ -786,7 +790,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 // kScene (key = scene index), else unused.  All lanes of the wave must call it.
 template <bool kScene>
 __device__ __forceinline__ int dda_balanced(const GridDev& G0, const GridDev* grids, SegStage& st, const float o[3], const float d[3],
-                            float L, double wk, double wh, bool seg, uint32_t key, TallyCache* cache = nullptr) {
+                            float L, double wk, double wh, bool seg, uint32_t key, TallyCache* cache = nullptr,
+                            size_t slot_stride = 0) {
   const int lane = __lane_id();
   float t0 = 0.f, t1 = 0.f;
   if (seg) seg = dda_clip(G0, o, d, L, t0, t1);
@@ -887,7 +892,16 @@ __device__ __forceinline__ int dda_balanced(const GridDev& G0, const GridDev* gr
       const GridDev Gs = grids[act ? skey : 0];
       visits += dda_walk<true>(Gs, so, sd, ta, tb, swk, swh, act, skey, sa, fidx);
     } else {
-      visits += dda_walk<false>(G0, so, sd, ta, tb, swk, swh, act, skey, sa, fidx, cache);
+      if (slot_stride) {   // streamed batches: the owner's tally slot (key = slot)
+        GridDev Gs = G0;
+        const size_t off = (size_t)skey * slot_stride;
+        Gs.dk += off;
+        Gs.dh += off;
+        Gs.base = (uint32_t)off;
+        visits += dda_walk<false>(Gs, so, sd, ta, tb, swk, swh, act, skey, sa, fidx, cache);
+      } else {
+        visits += dda_walk<false>(G0, so, sd, ta, tb, swk, swh, act, skey, sa, fidx, cache);
+      }
     }
   }
   return visits;

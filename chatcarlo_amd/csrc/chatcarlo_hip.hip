@@ -164,6 +164,17 @@ struct MultiScene {
   size_t cstride;             // doubles between tally copies
 };
 
+// Streamed analog runs: the batches of a run share one particle pool.  A device cursor
+// over the run's histories refills the slots that escaped and absorbed photons free, so
+// the pool stays full from the first batch to the last instead of draining at the end of
+// every batch.  A group of batches is in flight at once; each batch of the group keeps its
+// own tally slot (and deposits), so the batch moments are those of batch-by-batch runs.
+struct Stream {
+  uint64_t batch;          // histories per batch
+  uint64_t g0;             // the group's first batch (index in the run)
+  size_t slot_stride;      // doubles between the tallies of consecutive batch slots
+};
+
 struct Launch {
   Tables T;
   const Prim* prims;
@@ -183,6 +194,7 @@ struct Launch {
   int multi;               // batched run: scene data from MS, per-scene deposits (edep[scene][mat])
   MultiScene MS;
   Trace Tr;
+  Stream St;
 };
 
 // Philox key of a photon's history: the run seed, or its scene's seed in a batched run.
@@ -447,6 +459,65 @@ __global__ void k_refill_commit(Launch L, uint32_t limit) {
   }
 }
 
+// Starts the next histories of a streamed run in free pool slots: history h (counted from
+// the run's first) draws from the stream keyed by (seed, first + h), exactly as in the
+// source kernel of a batch-by-batch run, and goes to the tally slot of its batch.
+__global__ __launch_bounds__(kBlock) void k_refill_stream(Launch L, Src s, int* act, uint32_t* cnt_act,
+                                                          uint32_t limit) {
+  const MultiScene& M = L.MS;
+  const unsigned long long cur = *M.cursor;
+  const uint32_t fc = *M.free_cnt;
+  const uint32_t n_new = refill_count(M, limit);
+  PState S = L.S;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n_new; base += gridDim.x * blockDim.x) {
+    const uint32_t t = base + threadIdx.x;
+    const bool valid = t < n_new;
+    int q = 0;
+    if (valid) {
+      const unsigned long long h = cur + t;
+      q = M.free_stack[fc - 1 - t];
+      PhiloxRng rng(L.seed, L.hist0 + (uint64_t)h, 0u, 0u);
+      float p[3], d[3], E;
+      sample_source(s, rng, p, d, E);
+      const float tau = -logf(rng.next());
+      S.x[q] = p[0]; S.y[q] = p[1]; S.z[q] = p[2];
+      S.dx[q] = d[0]; S.dy[q] = d[1]; S.dz[q] = d[2];
+      S.e[q] = E; S.tau[q] = tau; S.wt[q] = 1.0f; S.score[q] = 0.f;
+      S.ctr[q] = rng.ctr; S.hid[q] = (uint32_t)h; S.tag[q] = 0u; S.nsc[q] = 0; S.mat[q] = -1; S.flags[q] = 0;
+      S.sc[q] = (int)(h / L.St.batch - L.St.g0);
+    }
+    const int k = wave_append(valid, cnt_act);
+    if (k >= 0) act[k] = q;
+  }
+}
+
+// Folds the batch slots of a streamed group into the grid totals and batch moments
+// (as k_end_batch does per batch), slot by slot in batch order, and clears them.  Batch
+// slot s holds n_s = min(batch, n_hist - (g0 + s) batch) histories.
+__global__ __launch_bounds__(kBlock) void k_end_group(double* dk, double* dh, size_t slot_stride, int nslot,
+                                                      double* tk, double* th, double* qk, double* qh, int* hits,
+                                                      size_t nvox, uint64_t batch, uint64_t g0, uint64_t n_hist,
+                                                      int track) {
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (size_t)gridDim.x * blockDim.x) {
+    for (int s = 0; s < nslot; ++s) {
+      const size_t u = (size_t)s * slot_stride + v;
+      const double a = dk[u], b = dh[u];
+      if (a == 0.0 && b == 0.0) continue;
+      dk[u] = 0.0;
+      dh[u] = 0.0;
+      tk[v] += a;
+      th[v] += b;
+      if (track) {
+        const uint64_t first = (g0 + (uint64_t)s) * batch;
+        const double inv = 1.0 / (double)min(batch, n_hist - first);
+        qk[v] += a * a * inv;
+        qh[v] += b * b * inv;
+        hits[v] += 1;
+      }
+    }
+  }
+}
+
 // Queue position and active count for the host's termination check.
 __global__ void k_batch_status(Launch L, const uint32_t* cnt_act, unsigned long long* out) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -455,10 +526,11 @@ __global__ void k_batch_status(Launch L, const uint32_t* cnt_act, unsigned long 
   }
 }
 
-__global__ void k_batch_init(uint32_t* free_cnt, uint32_t n_free, unsigned long long* cursor) {
+__global__ void k_batch_init(uint32_t* free_cnt, uint32_t n_free, unsigned long long* cursor,
+                             unsigned long long cursor0) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *free_cnt = n_free;
-    *cursor = 0ull;
+    *cursor = cursor0;
   }
 }
 
@@ -574,7 +646,7 @@ __device__ float region_expect(const Prim* sp, const World& W, const Tables& T, 
 // separate instantiation so the plain transport keeps its register budget.
 // kMulti: batched run; every photon reads its scene's primitives (from global
 // memory, cached), world box, grid and seed, and a photon that leaves frees its slot.
-template <bool kRec, bool kMulti, bool kTrace = false>
+template <bool kRec, bool kMulti, bool kTrace = false, bool kStream = false>
 __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, const uint32_t* cnt_in, int* qp,
                                                    int* qc, int* qr, int* coll) {
   __shared__ Prim sp_lds[kMulti ? 1 : CC_MAX_PRIMS];
@@ -608,6 +680,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     GridDev G = L.G;
     if (valid) {
       i = act[j];
+      if (kStream) scene = S.sc[i];   // the batch slot of the photon's history
       if (kMulti) {
         scene = S.sc[i];
         sp = L.MS.prims + L.MS.prim_off[scene];
@@ -699,7 +772,8 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
         if constexpr (kMulti)
           visits += dda_score<true>(G, pend_p, d, pend_len, pend_wk, wh, pend, (uint32_t)scene, tcache);
         else
-          visits += dda_balanced<false>(G, L.MS.grids, sst, pend_p, d, pend_len, pend_wk, wh, pend, 0u, tcache);
+          visits += dda_balanced<false>(G, L.MS.grids, sst, pend_p, d, pend_len, pend_wk, wh, pend,
+                                        kStream ? (uint32_t)scene : 0u, tcache, kStream ? L.St.slot_stride : 0);
         pend_p[0] = p[0]; pend_p[1] = p[1]; pend_p[2] = p[2];
         pend_len = ds;
         pend_wk = (double)w * (double)E * (double)muen;
@@ -777,7 +851,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
       if (take) put_record(L.Rec, 1, (uint32_t)k, p, d, E, S.score[i], i, L.rec_batch);
     }
     if (sreg > 0.f) atomic_add_f64(L.hscore + S.hid[i], (double)sreg);
-    if (kMulti) {   // a photon that does not interact (escaped, or stopped by an error) frees its slot
+    if (kMulti || kStream) {   // a photon that does not interact (escaped, or stopped by an error) frees its slot
       const int k = wave_append(valid && type < 0, L.MS.free_cnt);
       if (k >= 0) L.MS.free_stack[k] = i;
     }
@@ -1637,10 +1711,14 @@ __global__ void k_fill_i32(int* p, size_t n, int v) {
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = v;
 }
 // a[v] += a[k n + v] for k = 1 .. ncopy - 1 (tally copies folded into copy 0).
-__global__ void k_fold_copies(double* a, size_t n, int ncopy) {
+__global__ void k_fold_copies(double* a, size_t n, int ncopy, int clear) {
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
     double s = 0.0;
-    for (int k = 1; k < ncopy; ++k) s += a[(size_t)k * n + v];
+    for (int k = 1; k < ncopy; ++k) {
+      double& x = a[(size_t)k * n + v];
+      s += x;
+      if (clear && x != 0.0) x = 0.0;
+    }
     if (ncopy > 1) a[v] += s;
   }
 }
@@ -1827,6 +1905,11 @@ struct Ctx {
   size_t nvox = 0;
   bool track = false;
   int grid_copies = kGridCopies;   // batch-tally copies of the next cc_bind_grid
+  // streamed runs: tally slots of a group of batches, their deposits, per-slot seeds
+  double* st_tally = nullptr;
+  size_t st_tally_cap = 0;
+  double* st_edep = nullptr;
+  uint64_t* st_seeds = nullptr;
   // segment trace of cc_transport
   float* tr_rec = nullptr;
   uint32_t* tr_cnt = nullptr;
@@ -2116,6 +2199,24 @@ int run_rounds(Ctx* c, const Launch& L, uint32_t n, Timer& tm, uint32_t* rounds_
   return CC_OK;
 }
 
+// Particle pool, free-slot stack, history cursor and status words of queue-fed runs.
+int ensure_queue(Ctx* c, uint32_t pool) {
+  int r = ensure_pool(c, pool);
+  if (r) return r;
+  if (c->b_free_cap < pool) {
+    if (c->b_free) (void)hipFree(c->b_free);
+    c->b_free = nullptr;
+    c->b_free_cap = 0;
+    if ((r = dalloc(&c->b_free, pool))) return r;
+    c->b_free_cap = pool;
+  }
+  if (!c->b_free_cnt && ((r = dalloc(&c->b_free_cnt, 1)) || (r = dalloc(&c->b_cursor, 1)) ||
+                         (r = dalloc(&c->b_status, 2))))
+    return r;
+  if (!c->h_status) HIP_TRY(hipHostMalloc((void**)&c->h_status, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+  return CC_OK;
+}
+
 int read_batch(Ctx* c, uint32_t nb, cc_run_out* out, int* err_bits) {
   unsigned long long st[S_NSTAT];
   double ed[CC_MAX_MAT + 2];
@@ -2275,7 +2376,7 @@ int cc_ctx_destroy(void* ctx) {
   (void)hipStreamSynchronize(c->stream);
   free_pool(c);
   for (void* p : c->tbufs) (void)hipFree(p);
-  void* ps[] = {c->tr_rec, c->tr_cnt, c->tr_last, c->prims, c->tk, c->th, c->qk, c->qh, c->hits, c->G.dk, c->G.dh, c->cnt, c->stats, c->edep, c->err,
+  void* ps[] = {c->st_tally, c->st_edep, c->st_seeds, c->tr_rec, c->tr_cnt, c->tr_last, c->prims, c->tk, c->th, c->qk, c->qh, c->hits, c->G.dk, c->G.dh, c->cnt, c->stats, c->edep, c->err,
                 c->spec, c->rec, c->rcnt, c->mw[0], c->mm[0], c->mv[0], c->mgrad[0], c->mwb[0],
                 c->mw[1], c->mm[1], c->mv[1], c->mgrad[1], c->mwb[1], c->bcdf, c->bwt, c->stage};
   for (void* p : ps)
@@ -2505,6 +2606,162 @@ int cc_load_grid(void* ctx, const double* kerma, const double* h10, const double
   return CC_OK;
 }
 
+constexpr uint32_t kStreamPool = 1u << 21;      // photons in flight in a streamed run
+constexpr int kStreamSlots = 64;                 // batches in flight at most (one group)
+constexpr size_t kStreamTallyBytes = 8ull << 30; // cap on the group's tally slots (all copies)
+
+// A plain analog run with the pool fed from the device history cursor (see Stream).
+int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t first, uint32_t batch,
+               uint32_t flags, cc_run_out* out) {
+  const bool grid_on = (flags & CC_RUN_GRID) != 0;
+  const bool track = c->track && (flags & CC_RUN_TRACK_UNCERTAINTY);
+  const uint64_t nbt = (n_hist + batch - 1) / batch;
+  const uint32_t pool = (uint32_t)std::max<uint64_t>(1024, std::min<uint64_t>(kStreamPool, n_hist));
+  int r = ensure_queue(c, pool);
+  if (r) return r;
+  const size_t nvp = grid_on ? ((c->nvox + 31) & ~(size_t)31) : 0;
+  const int ncopy = grid_on ? (int)c->G.cmask + 1 : 1;
+  int slots = (int)std::min<uint64_t>(nbt, kStreamSlots);
+  if (grid_on)
+    while (slots > 1 && ((size_t)ncopy * 2 * slots * nvp * sizeof(double) > kStreamTallyBytes ||
+                         (size_t)slots * nvp >= 0xffffffffull))
+      --slots;
+  if (grid_on) {
+    const size_t need = (size_t)ncopy * 2 * slots * nvp;
+    if (need > c->st_tally_cap) {
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      if (c->st_tally) (void)hipFree(c->st_tally);
+      c->st_tally = nullptr;
+      c->st_tally_cap = 0;
+      if ((r = dalloc(&c->st_tally, need))) return r;
+      c->st_tally_cap = need;
+    }
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(need)), dim3(kBlock), 0, c->stream, c->st_tally, need, 0.0);
+  }
+  if (!c->st_edep && ((r = dalloc(&c->st_edep, (size_t)kStreamSlots * CC_MAX_MAT)) ||
+                      (r = dalloc(&c->st_seeds, (size_t)kStreamSlots))))
+    return r;
+  {
+    std::vector<uint64_t> sd(kStreamSlots, seed);
+    HIP_TRY(hipMemcpy(c->st_seeds, sd.data(), sizeof(uint64_t) * kStreamSlots, hipMemcpyHostToDevice));
+  }
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->err, (size_t)1, 0);
+  Timer tm{c, (flags & CC_RUN_TIMING) != 0};
+  const int nbk = grid_blocks(pool);
+  int err_bits = 0;
+  for (uint64_t g0 = 0; g0 < nbt; g0 += (uint64_t)slots) {
+    const int ns = (int)std::min<uint64_t>((uint64_t)slots, nbt - g0);
+    const uint64_t h_begin = g0 * batch, h_end = std::min<uint64_t>(n_hist, (g0 + ns) * (uint64_t)batch);
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, c->stats, c->edep, (int)C_ACT_A, 0u);
+    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(kBlock), 0, c->stream, c->st_edep, (size_t)ns * CC_MAX_MAT, 0.0);
+    hipLaunchKernelGGL(k_iota, dim3(grid_blocks(pool)), dim3(kBlock), 0, c->stream, c->b_free, pool);
+    hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, c->stream, c->b_free_cnt, pool, c->b_cursor,
+                       (unsigned long long)h_begin);
+    Launch L = make_launch(c, seed, first, grid_on ? 1 : 0, (flags & CC_RUN_FLUORESCENCE) ? 1 : 0);
+    L.multi = 1;   // deposits and RNG keys per batch slot, freed slots to the stack
+    L.edep = c->st_edep;
+    L.MS.seeds = c->st_seeds;
+    L.MS.free_stack = c->b_free;
+    L.MS.free_cnt = c->b_free_cnt;
+    L.MS.cursor = c->b_cursor;
+    L.MS.total = (unsigned long long)h_end;
+    L.St.batch = batch;
+    L.St.g0 = g0;
+    L.St.slot_stride = nvp;
+    if (grid_on) {
+      L.G.dk = c->st_tally;
+      L.G.dh = c->st_tally + (size_t)slots * nvp;
+      L.G.cstride = 2 * (size_t)slots * nvp;
+      L.G.base = 0;
+    }
+    HIP_TRY(hipGetLastError());
+    int cur = C_ACT_A, nxt = C_ACT_B;
+    int* act_cur = c->act0;
+    int* act_nxt = c->act1;
+    uint32_t rounds = 0;
+    for (;;) {
+      for (int k = 0; k < kRoundChunk; ++k) {
+        hipLaunchKernelGGL(k_round_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, nxt);
+        tm.begin(K_SOURCE);
+        hipLaunchKernelGGL(k_refill_stream, dim3(nbk), dim3(kBlock), 0, c->stream, L, s, act_cur, c->cnt + cur, pool);
+        hipLaunchKernelGGL(k_refill_commit, dim3(1), dim3(64), 0, c->stream, L, pool);
+        tm.end();
+        tm.begin(K_FLIGHT);
+        hipLaunchKernelGGL((k_flight<false, false, false, true>), dim3(nbk), dim3(kBlock), 0, c->stream, L,
+                           (const int*)act_cur, (const uint32_t*)(c->cnt + cur), c->qp, c->qc, c->qr, (int*)nullptr);
+        tm.end();
+        tm.begin(K_PHOTO);
+        hipLaunchKernelGGL(k_photo, dim3(nbk), dim3(kBlock), 0, c->stream, L, c->qp, c->cnt + C_QP, act_nxt,
+                           c->cnt + nxt);
+        tm.end();
+        tm.begin(K_COMPTON);
+        hipLaunchKernelGGL(k_compton, dim3(nbk), dim3(kBlock), 0, c->stream, L, c->qc, c->cnt + C_QC, act_nxt,
+                           c->cnt + nxt);
+        tm.end();
+        tm.begin(K_RAYLEIGH);
+        hipLaunchKernelGGL(k_rayleigh, dim3(nbk), dim3(kBlock), 0, c->stream, L, c->qr, c->cnt + C_QR, act_nxt,
+                           c->cnt + nxt);
+        tm.end();
+        std::swap(cur, nxt);
+        std::swap(act_cur, act_nxt);
+      }
+      rounds += kRoundChunk;
+      hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(64), 0, c->stream, L, (const uint32_t*)(c->cnt + cur),
+                         c->b_status);
+      HIP_TRY(hipMemcpyAsync(c->h_status, c->b_status, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->stream));
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      tm.collect(out->kernel_ms, out->kernel_launches);
+      if (c->h_status[0] >= h_end && c->h_status[1] == 0) break;
+      if (rounds >= (uint32_t)kMaxRounds) return fail(CC_ERR_DEVICE, "streamed run did not drain");
+    }
+    if (rounds > out->rounds_max) out->rounds_max = rounds;
+    if (grid_on) {
+      tm.begin(K_END);
+      if (ncopy > 1)
+        hipLaunchKernelGGL(k_fold_copies, dim3(grid_blocks(2 * (size_t)slots * nvp)), dim3(kBlock), 0, c->stream,
+                           c->st_tally, 2 * (size_t)slots * nvp, ncopy, 1);
+      hipLaunchKernelGGL(k_end_group, dim3(grid_blocks(c->nvox)), dim3(kBlock), 0, c->stream, L.G.dk, L.G.dh, nvp, ns,
+                         c->tk, c->th, c->qk, c->qh, c->hits, c->nvox, (uint64_t)batch, g0, n_hist, track ? 1 : 0);
+      tm.end();
+      if (track) c->g_batches += ns;
+      c->g_hist += h_end - h_begin;
+    }
+    HIP_TRY(hipGetLastError());
+    // counters of the group, and deposits slot by slot (batch moments of the scalar tallies)
+    unsigned long long st[S_NSTAT];
+    std::vector<double> ed((size_t)ns * CC_MAX_MAT);
+    int e = 0;
+    HIP_TRY(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(ed.data(), c->st_edep, ed.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&e, c->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    tm.collect(out->kernel_ms, out->kernel_launches);
+    out->n_absorbed += st[S_ABS];
+    out->n_escaped += st[S_ESC];
+    out->scatter_sum += st[S_SCAT];
+    out->n_fluorescence += st[S_FLUOR];
+    out->n_events += st[S_EVT];
+    out->n_flights += st[S_FLIGHT];
+    out->n_voxel_visits += st[S_VOX];
+    for (int b = 0; b < ns; ++b) {
+      const double nb = (double)std::min<uint64_t>(batch, n_hist - (g0 + b) * batch);
+      for (int m = 0; m < CC_MAX_MAT; ++m) {
+        const double x = ed[(size_t)b * CC_MAX_MAT + m];
+        out->edep_keV[m] += x;
+        out->edep_sum2[m] += x * x / nb;
+      }
+    }
+    out->n_batches += (uint64_t)ns;
+    err_bits |= e;
+  }
+  out->n_histories = n_hist;
+  out->error_bits = (uint32_t)err_bits;
+  if (err_bits) return fail(CC_ERR_DEVICE, "device error bits 0x%x", err_bits);
+  return CC_OK;
+}
+
 int cc_run(void* ctx, const cc_source* src, uint64_t n_hist, uint64_t seed, uint64_t first, uint32_t batch,
            uint32_t flags, cc_run_out* out) {
   Ctx* c = (Ctx*)ctx;
@@ -2527,6 +2784,8 @@ int cc_run(void* ctx, const cc_source* src, uint64_t n_hist, uint64_t seed, uint
   Src s;
   int r = make_src_impl(c, src, &s);
   if (r) return r;
+  if (!region && !record && !imp && !bias && !(flags & CC_RUN_PER_BATCH) && n_hist > 0 && n_hist < (1ull << 32))
+    return run_stream(c, s, n_hist, seed, first, batch, flags, out);
   size_t pool = batch;
   if (imp) {
     const double f = std::max(1.0, (double)c->icfg.pool_factor);
@@ -3377,19 +3636,8 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
   const uint64_t total = start[n];
   HIP_TRY(hipMemcpy(c->b_hist_start, start.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice));
   const uint32_t pool = tune->pool;
-  int r = ensure_pool(c, pool);
+  int r = ensure_queue(c, pool);
   if (r) return r;
-  if (c->b_free_cap < pool) {
-    if (c->b_free) (void)hipFree(c->b_free);
-    c->b_free = nullptr;
-    c->b_free_cap = 0;
-    if ((r = dalloc(&c->b_free, pool))) return r;
-    c->b_free_cap = pool;
-  }
-  if (!c->b_free_cnt && ((r = dalloc(&c->b_free_cnt, 1)) || (r = dalloc(&c->b_cursor, 1)) ||
-                         (r = dalloc(&c->b_status, 2))))
-    return r;
-  if (!c->h_status) HIP_TRY(hipHostMalloc((void**)&c->h_status, 2 * sizeof(unsigned long long), hipHostMallocDefault));
   // fresh tallies, deposits, counters, free slots and queue
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks((size_t)ncopy * 2 * c->b_nvox)), dim3(kBlock), 0, c->stream, c->b_tally,
                      (size_t)ncopy * 2 * c->b_nvox, 0.0);
@@ -3398,7 +3646,7 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, c->stats, c->edep, (int)C_ACT_A, 0u);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->err, (size_t)1, 0);
   hipLaunchKernelGGL(k_iota, dim3(grid_blocks(pool)), dim3(kBlock), 0, c->stream, c->b_free, pool);
-  hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, c->stream, c->b_free_cnt, pool, c->b_cursor);
+  hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, c->stream, c->b_free_cnt, pool, c->b_cursor, 0ull);
   HIP_TRY(hipGetLastError());
   Launch L = make_launch(c, 0, 0, 1, (flags & CC_RUN_FLUORESCENCE) ? 1 : 0);
   L.edep = c->b_edep;
@@ -3474,7 +3722,7 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
   if (rc) return rc;
   if (ncopy > 1) {
     hipLaunchKernelGGL(k_fold_copies, dim3(grid_blocks(2 * c->b_nvox)), dim3(kBlock), 0, c->stream, c->b_tally,
-                       2 * c->b_nvox, (int)ncopy);
+                       2 * c->b_nvox, (int)ncopy, 0);
     HIP_TRY(hipGetLastError());
   }
   unsigned long long st[S_NSTAT];

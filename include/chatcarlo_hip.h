@@ -110,7 +110,8 @@ enum {
   CC_RUN_RECORD = 32,           /* contribution records (analog runs only, needs CC_RUN_REGION) */
   CC_RUN_IMPORTANCE = 64,       /* weight windows from the learned importance map */
   CC_RUN_SOURCE_BIAS = 128,     /* source cells drawn by learned importance (cc_set_importance) */
-  CC_RUN_TRACE = 256            /* cc_transport: log every flight segment (cc_trace_alloc, cc_trace_read) */
+  CC_RUN_TRACE = 256,           /* cc_transport: log every flight segment (cc_trace_alloc, cc_trace_read) */
+  CC_RUN_PER_BATCH = 512        /* cc_run: one batch after the other instead of streaming them (same results) */
 };
 
 typedef struct {
