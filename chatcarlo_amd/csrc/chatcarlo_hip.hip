@@ -61,11 +61,22 @@ enum { C_ACT_A = 0, C_ACT_B = 1, C_QP = 2, C_QC = 3, C_QR = 4, C_IMP = 5, C_POOL
 enum { S_ABS = 0, S_ESC = 1, S_SCAT = 2, S_FLUOR = 3, S_EVT = 4, S_FLIGHT = 5, S_VOX = 6, S_SPLIT = 7, S_ROUL = 8,
        S_NSTAT = 10 };
 
+// Particle pool: one 64 B record per photon (16 words, the record a cache line), so a
+// kernel that gathers a photon from a queue touches one line, not one line per field.
+// A Field is one word of the record across the pool: S.x[i] is word 0 of photon i.
+constexpr int kPhotonWords = 16;
+enum { W_X, W_Y, W_Z, W_DX, W_DY, W_DZ, W_E, W_TAU, W_WT, W_CTR, W_HID, W_TAG, W_NSC, W_MAT, W_FLAGS, W_SC };
+template <class T>
+struct Field {
+  T* p = nullptr;   // the word of photon 0
+  __host__ __device__ __forceinline__ T& operator[](size_t i) const { return p[i * kPhotonWords]; }
+};
 struct PState {
-  float *x, *y, *z, *dx, *dy, *dz, *e, *tau, *wt, *score;
-  uint32_t *ctr, *hid, *tag;   // Philox event counter, history offset, lane tag (split copies)
-  int *nsc, *mat, *flags;
-  int* sc;                     // scene of the photon (batched runs)
+  Field<float> x, y, z, dx, dy, dz, e, tau, wt;
+  Field<uint32_t> ctr, hid, tag;   // Philox event counter, history offset, lane tag (split copies)
+  Field<int> nsc, mat, flags;
+  Field<int> sc;                   // scene of the photon (batched runs) or batch slot (streamed runs)
+  float* score;                    // region score (its own array: region runs only)
 };
 
 struct Src {
@@ -1919,6 +1930,7 @@ struct Ctx {
   int64_t g_batches = 0, g_hist = 0;
   // particle pool
   size_t cap = 0;
+  uint32_t* photons = nullptr;   // [cap][kPhotonWords]
   PState S{};
   int *act0 = nullptr, *act1 = nullptr, *act2 = nullptr, *qp = nullptr, *qc = nullptr, *qr = nullptr;
   int* coll = nullptr;         // collision list (weight windows at collisions)
@@ -2002,12 +2014,11 @@ void set_default_frame(Ctx* c) {
 }
 
 void free_pool(Ctx* c) {
-  void* ps[] = {c->S.x, c->S.y, c->S.z, c->S.dx, c->S.dy, c->S.dz, c->S.e, c->S.tau, c->S.wt, c->S.score,
-                c->S.ctr, c->S.hid, c->S.tag, c->S.nsc, c->S.mat, c->S.flags, c->act0, c->act1, c->act2,
-                c->qp, c->qc, c->qr, c->hscore, c->coll, c->S.sc};
+  void* ps[] = {c->photons, c->S.score, c->act0, c->act1, c->act2, c->qp, c->qc, c->qr, c->hscore, c->coll};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   c->S = PState{};
+  c->photons = nullptr;
   c->act0 = c->act1 = c->act2 = c->qp = c->qc = c->qr = c->coll = nullptr;
   c->hscore = nullptr;
   c->cap = 0;
@@ -2017,16 +2028,20 @@ int ensure_pool(Ctx* c, size_t n) {
   if (n <= c->cap) return CC_OK;
   free_pool(c);
   int r;
-  if ((r = dalloc(&c->S.x, n)) || (r = dalloc(&c->S.y, n)) || (r = dalloc(&c->S.z, n)) ||
-      (r = dalloc(&c->S.dx, n)) || (r = dalloc(&c->S.dy, n)) || (r = dalloc(&c->S.dz, n)) ||
-      (r = dalloc(&c->S.e, n)) || (r = dalloc(&c->S.tau, n)) || (r = dalloc(&c->S.wt, n)) ||
-      (r = dalloc(&c->S.ctr, n)) || (r = dalloc(&c->S.nsc, n)) || (r = dalloc(&c->S.mat, n)) ||
-      (r = dalloc(&c->S.flags, n)) || (r = dalloc(&c->act0, n)) || (r = dalloc(&c->act1, n)) ||
+  if ((r = dalloc(&c->photons, n * kPhotonWords)) || (r = dalloc(&c->act0, n)) || (r = dalloc(&c->act1, n)) ||
       (r = dalloc(&c->qp, n)) || (r = dalloc(&c->qc, n)) || (r = dalloc(&c->qr, n)) ||
-      (r = dalloc(&c->S.score, n)) || (r = dalloc(&c->S.hid, n)) || (r = dalloc(&c->S.tag, n)) ||
-      (r = dalloc(&c->act2, n)) || (r = dalloc(&c->hscore, n)) || (r = dalloc(&c->coll, n)) ||
-      (r = dalloc(&c->S.sc, n)))
+      (r = dalloc(&c->S.score, n)) || (r = dalloc(&c->act2, n)) || (r = dalloc(&c->hscore, n)) ||
+      (r = dalloc(&c->coll, n)))
     return r;
+  {
+    uint32_t* b = c->photons;
+    c->S.x.p = (float*)(b + W_X); c->S.y.p = (float*)(b + W_Y); c->S.z.p = (float*)(b + W_Z);
+    c->S.dx.p = (float*)(b + W_DX); c->S.dy.p = (float*)(b + W_DY); c->S.dz.p = (float*)(b + W_DZ);
+    c->S.e.p = (float*)(b + W_E); c->S.tau.p = (float*)(b + W_TAU); c->S.wt.p = (float*)(b + W_WT);
+    c->S.ctr.p = b + W_CTR; c->S.hid.p = b + W_HID; c->S.tag.p = b + W_TAG;
+    c->S.nsc.p = (int*)(b + W_NSC); c->S.mat.p = (int*)(b + W_MAT); c->S.flags.p = (int*)(b + W_FLAGS);
+    c->S.sc.p = (int*)(b + W_SC);
+  }
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(n)), dim3(kBlock), 0, c->stream, c->hscore, n, 0.0);
   HIP_TRY(hipGetLastError());
   c->cap = n;
@@ -2899,12 +2914,19 @@ int cc_transport(void* ctx, int64_t n, double* pos, double* dir, double* energy,
   if (n == 0) return CC_OK;
   int r = ensure_pool(c, (size_t)n);
   if (r) return r;
-  std::vector<float> h((size_t)n);
-  float* dsts[7] = {c->S.x, c->S.y, c->S.z, c->S.dx, c->S.dy, c->S.dz, c->S.e};
-  for (int k = 0; k < 7; ++k) {
-    for (int64_t i = 0; i < n; ++i) h[i] = (float)(k < 3 ? pos[3 * i + k] : (k < 6 ? dir[3 * i + k - 3] : energy[i]));
-    HIP_TRY(hipMemcpy(dsts[k], h.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  std::vector<uint32_t> h((size_t)n * kPhotonWords, 0u);   // photon records: position, direction, energy
+  auto put = [&](int64_t i, int w, double v) {
+    const float f = (float)v;
+    std::memcpy(&h[(size_t)i * kPhotonWords + w], &f, sizeof(float));
+  };
+  for (int64_t i = 0; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      put(i, W_X + k, pos[3 * i + k]);
+      put(i, W_DX + k, dir[3 * i + k]);
+    }
+    put(i, W_E, energy[i]);
   }
+  HIP_TRY(hipMemcpy(c->photons, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->err, (size_t)1, 0);
   Timer tm{c, false};
   Launch L = make_launch(c, seed, first, grid_on ? 1 : 0, (flags & CC_RUN_FLUORESCENCE) ? 1 : 0);
@@ -2937,21 +2959,22 @@ int cc_transport(void* ctx, int64_t n, double* pos, double* dir, double* energy,
   int err_bits = 0;
   r = read_batch(c, (uint32_t)n, out, &err_bits);
   if (r) return r;
-  for (int k = 0; k < 7; ++k) {
-    HIP_TRY(hipMemcpy(h.data(), dsts[k], n * sizeof(float), hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < n; ++i) {
-      if (k < 3) pos[3 * i + k] = h[i];
-      else if (k < 6) dir[3 * i + k - 3] = h[i];
-      else energy[i] = h[i];
-    }
-  }
-  std::vector<int> iv((size_t)n), fl((size_t)n);
-  HIP_TRY(hipMemcpy(iv.data(), c->S.nsc, n * sizeof(int), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(fl.data(), c->S.flags, n * sizeof(int), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(h.data(), c->photons, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  auto getf = [&](int64_t i, int w) {
+    float f;
+    std::memcpy(&f, &h[(size_t)i * kPhotonWords + w], sizeof(float));
+    return (double)f;
+  };
   for (int64_t i = 0; i < n; ++i) {
-    if (n_scatter) n_scatter[i] = iv[i];
-    if (absorbed) absorbed[i] = (fl[i] & F_ABSORBED) ? 1 : 0;
-    if (escaped) escaped[i] = (fl[i] & F_ESCAPED) ? 1 : 0;
+    for (int k = 0; k < 3; ++k) {
+      pos[3 * i + k] = getf(i, W_X + k);
+      dir[3 * i + k] = getf(i, W_DX + k);
+    }
+    energy[i] = getf(i, W_E);
+    const int fl = (int)h[(size_t)i * kPhotonWords + W_FLAGS];
+    if (n_scatter) n_scatter[i] = (int)h[(size_t)i * kPhotonWords + W_NSC];
+    if (absorbed) absorbed[i] = (fl & F_ABSORBED) ? 1 : 0;
+    if (escaped) escaped[i] = (fl & F_ESCAPED) ? 1 : 0;
   }
   out->n_histories = (uint64_t)n;
   out->n_batches = 1;
