@@ -378,8 +378,12 @@ __device__ __forceinline__ float h10_coeff(const float* h10, int n, float E) {
   const float* hd = h10 + 2 * n;
   const float* hs = h10 + 3 * n;
   float ec = fminf(fmaxf(E, he[0]), he[n - 1]);
-  int j = 0;
-  while (j < n - 2 && ec >= he[j + 1]) ++j;
+  // the last j <= n - 2 with he[j] <= ec (binary search; the table is sorted)
+  int j = 0, hi = n - 2;
+  while (j < hi) {
+    const int mid = (j + hi + 1) >> 1;
+    if (ec >= he[mid]) j = mid; else hi = mid - 1;
+  }
   float f = logf(ec / he[j]) / hd[j];
   return hv[j] * expf(f * hs[j]);
 }

@@ -233,6 +233,14 @@ __device__ __forceinline__ void wave_count(bool pred, unsigned long long* counte
     atomicAdd(counter, (unsigned long long)__popcll(mask));
 }
 
+// Sum of small per-lane counts over the wave from bit-sliced ballots (one ballot and a
+// scalar popcount per bit of the largest count), added with one atomic.
+__device__ __forceinline__ void wave_sum_small(uint32_t v, unsigned long long* counter) {
+  unsigned long long s = 0;
+  for (int b = 0; __ballot(v != 0u); ++b, v >>= 1) s += (unsigned long long)__popcll(__ballot(v & 1u)) << b;
+  if (__lane_id() == 0 && s) atomicAdd(counter, s);
+}
+
 __device__ __forceinline__ void wave_sum_u64(unsigned long long v, unsigned long long* counter) {
   for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
   if (__lane_id() == 0 && v) atomicAdd(counter, v);
@@ -868,8 +876,8 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     }
     wave_count(valid && escaped, L.stats + S_ESC);
     wave_count(valid, L.stats + S_FLIGHT);
-    wave_sum_u64(events, L.stats + S_EVT);
-    wave_sum_u64(visits, L.stats + S_VOX);
+    wave_sum_small((uint32_t)events, L.stats + S_EVT);
+    wave_sum_small((uint32_t)visits, L.stats + S_VOX);
     int slot;
     slot = wave_append(type == 0, L.cnt + C_QP);
     if (slot >= 0) qp[slot] = i;
