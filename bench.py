@@ -100,7 +100,7 @@ def learned_fom(eng, target=0.02):
     t0 = time.perf_counter()
     n_rec, _ = li.generate_records(10_000_000, 32_000_000, keep=0.3, seed=61)
     t1 = time.perf_counter()
-    tr = li.train(0, steps=3000, batch=8192, lr=3e-3, lr_end=1e-4)
+    tr = li.train(0, steps=12000, batch=8192, lr=3e-3, lr_end=1e-4)
     t2 = time.perf_counter()
     li.configure()
     ana, lrn, chosen, z = li.compare(target_rel_err=target)
