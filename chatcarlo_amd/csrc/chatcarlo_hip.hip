@@ -60,6 +60,15 @@ enum { C_ACT_A = 0, C_ACT_B = 1, C_QP = 2, C_QC = 3, C_QR = 4, C_IMP = 5, C_POOL
 // device stats slots (u64)
 enum { S_ABS = 0, S_ESC = 1, S_SCAT = 2, S_FLUOR = 3, S_EVT = 4, S_FLIGHT = 5, S_VOX = 6, S_SPLIT = 7, S_ROUL = 8,
        S_NSTAT = 10 };
+// The event counters and the deposit sums are kept in kAccCopies copies: block b adds into
+// copy b mod kAccCopies, so the one-per-wave atomics of a launch spread over many L2 lines
+// instead of queueing on one line each (a launch has tens of thousands of waves and a
+// handful of counter and deposit addresses).  k_fold_acc folds the copies into copy 0,
+// and clears the others, before the host reads them.
+constexpr int kAccCopies = 32;
+constexpr int kStatStride = 16;     // u64 words per counter copy: one 128-byte line
+constexpr size_t kDepMain = 48;     // doubles per copy of a run's deposits (CC_MAX_MAT, 2 region sums, padding)
+static_assert(S_NSTAT <= kStatStride, "counter copy too small");
 
 // Particle pool: one 64 B record per photon (16 words, the record a cache line), so a
 // kernel that gathers a photon from a queue touches one line, not one line per field.
@@ -194,8 +203,9 @@ struct Launch {
   int grid_on, fluor;
   PState S;
   uint32_t* cnt;
-  unsigned long long* stats;
-  double* edep;
+  unsigned long long* stats;   // kAccCopies copies of kStatStride counters
+  double* edep;                // kAccCopies copies of dep_stride deposit sums
+  size_t dep_stride;
   double* hscore;          // per-history region score of the batch (indexed by history offset)
   int* err;
   uint64_t seed, hist0;
@@ -211,6 +221,14 @@ struct Launch {
 // Philox key of a photon's history: the run seed, or its scene's seed in a batched run.
 __device__ __forceinline__ uint64_t photon_seed(const Launch& L, const PState& S, int i) {
   return L.multi ? L.MS.seeds[S.sc[i]] : L.seed;
+}
+
+// The calling block's copies of the event counters and of the deposit sums.
+__device__ __forceinline__ unsigned long long* stat_copy(const Launch& L) {
+  return L.stats + (size_t)(blockIdx.x & (kAccCopies - 1)) * kStatStride;
+}
+__device__ __forceinline__ double* dep_copy(const Launch& L) {
+  return L.edep + (size_t)(blockIdx.x & (kAccCopies - 1)) * L.dep_stride;
 }
 
 // ---------------------------------------------------------------------------
@@ -580,8 +598,9 @@ __global__ void k_round_reset(uint32_t* cnt, int next_slot) {
 __global__ void k_batch_reset(uint32_t* cnt, unsigned long long* stats, double* edep, int cur_slot, uint32_t n) {
   const int t = threadIdx.x;
   if (t < C_NCNT) cnt[t] = 0;
-  if (t < S_NSTAT) stats[t] = 0ull;
-  if (t < CC_MAX_MAT + 2) edep[t] = 0.0;   // per-material deposits, then the region sums
+  for (int k = t; k < kAccCopies * kStatStride; k += blockDim.x) stats[k] = 0ull;
+  // every copy of the per-material deposits and the region sums
+  for (int k = t; k < kAccCopies * (int)kDepMain; k += blockDim.x) edep[k] = 0.0;
   __syncthreads();
   if (t == 0) {
     cnt[cur_slot] = n;
@@ -874,10 +893,10 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
       const int k = wave_append(valid && type < 0, L.MS.free_cnt);
       if (k >= 0) L.MS.free_stack[k] = i;
     }
-    wave_count(valid && escaped, L.stats + S_ESC);
-    wave_count(valid, L.stats + S_FLIGHT);
-    wave_sum_small((uint32_t)events, L.stats + S_EVT);
-    wave_sum_small((uint32_t)visits, L.stats + S_VOX);
+    wave_count(valid && escaped, stat_copy(L) + S_ESC);
+    wave_count(valid, stat_copy(L) + S_FLIGHT);
+    wave_sum_small((uint32_t)events, stat_copy(L) + S_EVT);
+    wave_sum_small((uint32_t)visits, stat_copy(L) + S_VOX);
     int slot;
     slot = wave_append(type == 0, L.cnt + C_QP);
     if (slot >= 0) qp[slot] = i;
@@ -947,10 +966,10 @@ __global__ __launch_bounds__(kBlock) void k_photo(Launch L, const int* q, const 
       }
       S.ctr[i] = rng.ctr;
     }
-    wave_add_by_key(valid, L.multi && valid ? S.sc[i] * CC_MAX_MAT + m : m, dep, L.edep);
-    wave_count(valid && !emit, L.stats + S_ABS);
-    wave_count(valid && emit, L.stats + S_FLUOR);
-    wave_count(valid, L.stats + S_SCAT);
+    wave_add_by_key(valid, L.multi && valid ? S.sc[i] * CC_MAX_MAT + m : m, dep, dep_copy(L));
+    wave_count(valid && !emit, stat_copy(L) + S_ABS);
+    wave_count(valid && emit, stat_copy(L) + S_FLUOR);
+    wave_count(valid, stat_copy(L) + S_SCAT);
     const int slot = wave_append(valid && emit, cnt_next);
     if (slot >= 0) act_next[slot] = i;
     if (L.multi) {   // absorbed photons free their slot
@@ -991,8 +1010,8 @@ __global__ __launch_bounds__(kBlock) void k_compton(Launch L, const int* q, cons
       S.nsc[i] += 1;
       S.ctr[i] = rng.ctr;
     }
-    wave_add_by_key(valid, L.multi && valid ? S.sc[i] * CC_MAX_MAT + m : m, dep, L.edep);
-    wave_count(valid, L.stats + S_SCAT);
+    wave_add_by_key(valid, L.multi && valid ? S.sc[i] * CC_MAX_MAT + m : m, dep, dep_copy(L));
+    wave_count(valid, stat_copy(L) + S_SCAT);
     const int slot = wave_append(valid, cnt_next);
     if (slot >= 0) act_next[slot] = i;
   }
@@ -1026,7 +1045,7 @@ __global__ __launch_bounds__(kBlock) void k_rayleigh(Launch L, const int* q, con
       S.nsc[i] += 1;
       S.ctr[i] = rng.ctr;
     }
-    wave_count(valid, L.stats + S_SCAT);
+    wave_count(valid, stat_copy(L) + S_SCAT);
     const int slot = wave_append(valid, cnt_next);
     if (slot >= 0) act_next[slot] = i;
   }
@@ -1080,7 +1099,7 @@ __device__ Window weight_window(const Launch& L, const Importance& I, bool valid
       else o.keep = false;
     }
   }
-  wave_count(valid && !o.keep, L.stats + S_ROUL);
+  wave_count(valid && !o.keep, stat_copy(L) + S_ROUL);
   uint32_t tot = 0;
   const uint32_t off = wave_excl_scan(o.ncopy, tot);
   uint32_t slot0 = 0;
@@ -1104,7 +1123,7 @@ __device__ Window weight_window(const Launch& L, const Importance& I, bool valid
       S.ctr[q] = rc.ctr;
     }
   }
-  wave_sum_u64((unsigned long long)o.ncopy, L.stats + S_SPLIT);
+  wave_sum_u64((unsigned long long)o.ncopy, stat_copy(L) + S_SPLIT);
   return o;
 }
 
@@ -1411,7 +1430,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_split(Launch L, Importance I
         if (type == 0 && !out_ok) S.flags[i] |= F_ABSORBED;
       }
     }
-    wave_add_by_key(valid, m, dep_acc, L.edep);
+    wave_add_by_key(valid, m, dep_acc, dep_copy(L));
     // move the staged photons to pool slots; append parents and new photons to the next list
     uint32_t npar = 0;
     const uint32_t poff = wave_excl_scan(parent_alive ? 1u : 0u, npar);
@@ -1437,11 +1456,11 @@ __global__ __launch_bounds__(kBlock) void k_scatter_split(Launch L, Importance I
       act_next[a0 + npar + k] = q;
     }
   }
-  wave_sum_u64(n_abs, L.stats + S_ABS);
-  wave_sum_u64(n_fluor, L.stats + S_FLUOR);
-  wave_sum_u64(n_scat, L.stats + S_SCAT);
-  wave_sum_u64(n_roul, L.stats + S_ROUL);
-  wave_sum_u64(n_split, L.stats + S_SPLIT);
+  wave_sum_u64(n_abs, stat_copy(L) + S_ABS);
+  wave_sum_u64(n_fluor, stat_copy(L) + S_FLUOR);
+  wave_sum_u64(n_scat, stat_copy(L) + S_SCAT);
+  wave_sum_u64(n_roul, stat_copy(L) + S_ROUL);
+  wave_sum_u64(n_split, stat_copy(L) + S_SPLIT);
 }
 
 __global__ __launch_bounds__(kBlock) void k_record(Launch L, const int* act, const uint32_t* cnt, uint32_t batch) {
@@ -1739,6 +1758,28 @@ __global__ void k_fold_copies(double* a, size_t n, int ncopy, int clear) {
       if (clear && x != 0.0) x = 0.0;
     }
     if (ncopy > 1) a[v] += s;
+  }
+}
+
+// Counter copies and the first n_dep deposit sums of every deposit copy folded into copy 0;
+// copies 1 .. kAccCopies - 1 are left at zero.
+__global__ void k_fold_acc(unsigned long long* stats, double* dep, size_t dep_stride, size_t n_dep) {
+  const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 < (size_t)kStatStride) {
+    unsigned long long s = 0;
+    for (int k = 1; k < kAccCopies; ++k) {
+      s += stats[(size_t)k * kStatStride + t0];
+      stats[(size_t)k * kStatStride + t0] = 0ull;
+    }
+    stats[t0] += s;
+  }
+  for (size_t v = t0; v < n_dep; v += (size_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int k = 1; k < kAccCopies; ++k) {
+      s += dep[(size_t)k * dep_stride + v];
+      dep[(size_t)k * dep_stride + v] = 0.0;
+    }
+    dep[v] += s;
   }
 }
 
@@ -2077,6 +2118,7 @@ Launch make_launch(Ctx* c, uint64_t seed, uint64_t hist0, int grid_on, int fluor
   L.cnt = c->cnt;
   L.stats = c->stats;
   L.edep = c->edep;
+  L.dep_stride = kDepMain;
   L.hscore = c->hscore;
   L.err = c->err;
   L.seed = seed;
@@ -2244,6 +2286,8 @@ int read_batch(Ctx* c, uint32_t nb, cc_run_out* out, int* err_bits) {
   unsigned long long st[S_NSTAT];
   double ed[CC_MAX_MAT + 2];
   int e = 0;
+  hipLaunchKernelGGL(k_fold_acc, dim3(1), dim3(kBlock), 0, c->stream, c->stats, c->edep, kDepMain,
+                     (size_t)CC_MAX_MAT + 2);
   HIP_TRY(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(ed, c->edep, sizeof(ed), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(&e, c->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -2381,7 +2425,8 @@ int cc_ctx_create(int32_t device, void** out) {
   c->device = device;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   int r;
-  if ((r = dalloc(&c->cnt, C_NCNT)) || (r = dalloc(&c->stats, S_NSTAT)) || (r = dalloc(&c->edep, CC_MAX_MAT + 2)) ||
+  if ((r = dalloc(&c->cnt, C_NCNT)) || (r = dalloc(&c->stats, (size_t)kAccCopies * kStatStride)) ||
+      (r = dalloc(&c->edep, kAccCopies * kDepMain)) ||
       (r = dalloc(&c->err, 1))) {
     delete c;
     return r;
@@ -2661,7 +2706,8 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
     }
     hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(need)), dim3(kBlock), 0, c->stream, c->st_tally, need, 0.0);
   }
-  if (!c->st_edep && ((r = dalloc(&c->st_edep, (size_t)kStreamSlots * CC_MAX_MAT)) ||
+  constexpr size_t kStreamDep = (size_t)kStreamSlots * CC_MAX_MAT;   // deposit sums per copy: [slot][material]
+  if (!c->st_edep && ((r = dalloc(&c->st_edep, kAccCopies * kStreamDep)) ||
                       (r = dalloc(&c->st_seeds, (size_t)kStreamSlots))))
     return r;
   {
@@ -2676,13 +2722,15 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
     const int ns = (int)std::min<uint64_t>((uint64_t)slots, nbt - g0);
     const uint64_t h_begin = g0 * batch, h_end = std::min<uint64_t>(n_hist, (g0 + ns) * (uint64_t)batch);
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, c->stats, c->edep, (int)C_ACT_A, 0u);
-    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(kBlock), 0, c->stream, c->st_edep, (size_t)ns * CC_MAX_MAT, 0.0);
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(kAccCopies * kStreamDep)), dim3(kBlock), 0, c->stream, c->st_edep,
+                       kAccCopies * kStreamDep, 0.0);
     hipLaunchKernelGGL(k_iota, dim3(grid_blocks(pool)), dim3(kBlock), 0, c->stream, c->b_free, pool);
     hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, c->stream, c->b_free_cnt, pool, c->b_cursor,
                        (unsigned long long)h_begin);
     Launch L = make_launch(c, seed, first, grid_on ? 1 : 0, (flags & CC_RUN_FLUORESCENCE) ? 1 : 0);
     L.multi = 1;   // deposits and RNG keys per batch slot, freed slots to the stack
     L.edep = c->st_edep;
+    L.dep_stride = kStreamDep;
     L.MS.seeds = c->st_seeds;
     L.MS.free_stack = c->b_free;
     L.MS.free_cnt = c->b_free_cnt;
@@ -2756,6 +2804,8 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
     unsigned long long st[S_NSTAT];
     std::vector<double> ed((size_t)ns * CC_MAX_MAT);
     int e = 0;
+    hipLaunchKernelGGL(k_fold_acc, dim3(grid_blocks(ed.size())), dim3(kBlock), 0, c->stream, c->stats, c->st_edep,
+                       kStreamDep, ed.size());
     HIP_TRY(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(ed.data(), c->st_edep, ed.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(&e, c->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -3580,7 +3630,7 @@ int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int
       (r = dalloc(&c->b_worlds, n_scenes)) || (r = dalloc(&c->b_grids, n_scenes)) ||
       (r = dalloc(&c->b_srcs, n_scenes)) || (r = dalloc(&c->b_seeds, n_scenes)) ||
       (r = dalloc(&c->b_hist_start, n_scenes + 1)) || (r = dalloc(&c->b_spec, nspec)) ||
-      (r = dalloc(&c->b_tally, 2 * nvox)) || (r = dalloc(&c->b_edep, (size_t)n_scenes * CC_MAX_MAT)))
+      (r = dalloc(&c->b_tally, 2 * nvox)) || (r = dalloc(&c->b_edep, kAccCopies * (size_t)n_scenes * CC_MAX_MAT)))
     return r;
   if (total_prims > 0)
     HIP_TRY(hipMemcpy(c->b_prims, prims, sizeof(Prim) * total_prims, hipMemcpyHostToDevice));
@@ -3672,8 +3722,8 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
   // fresh tallies, deposits, counters, free slots and queue
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks((size_t)ncopy * 2 * c->b_nvox)), dim3(kBlock), 0, c->stream, c->b_tally,
                      (size_t)ncopy * 2 * c->b_nvox, 0.0);
-  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks((size_t)n * CC_MAX_MAT)), dim3(kBlock), 0, c->stream, c->b_edep,
-                     (size_t)n * CC_MAX_MAT, 0.0);
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(kAccCopies * (size_t)n * CC_MAX_MAT)), dim3(kBlock), 0, c->stream,
+                     c->b_edep, kAccCopies * (size_t)n * CC_MAX_MAT, 0.0);
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, c->stats, c->edep, (int)C_ACT_A, 0u);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->err, (size_t)1, 0);
   hipLaunchKernelGGL(k_iota, dim3(grid_blocks(pool)), dim3(kBlock), 0, c->stream, c->b_free, pool);
@@ -3681,6 +3731,7 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
   HIP_TRY(hipGetLastError());
   Launch L = make_launch(c, 0, 0, 1, (flags & CC_RUN_FLUORESCENCE) ? 1 : 0);
   L.edep = c->b_edep;
+  L.dep_stride = (size_t)n * CC_MAX_MAT;
   L.multi = 1;
   L.MS.prims = c->b_prims; L.MS.prim_off = c->b_prim_off; L.MS.worlds = c->b_worlds; L.MS.grids = c->b_grids;
   L.MS.srcs = c->b_srcs; L.MS.seeds = c->b_seeds; L.MS.hist_start = c->b_hist_start; L.MS.n = n;
@@ -3756,9 +3807,13 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
                        2 * c->b_nvox, (int)ncopy, 0);
     HIP_TRY(hipGetLastError());
   }
+  hipLaunchKernelGGL(k_fold_acc, dim3(grid_blocks((size_t)n * CC_MAX_MAT)), dim3(kBlock), 0, c->stream, c->stats,
+                     c->b_edep, (size_t)n * CC_MAX_MAT, (size_t)n * CC_MAX_MAT);
+  HIP_TRY(hipGetLastError());
   unsigned long long st[S_NSTAT];
   int e = 0;
-  HIP_TRY(hipMemcpy(st, c->stats, sizeof(st), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemcpy(&e, c->err, sizeof(int), hipMemcpyDeviceToHost));
   out->n_histories = total;
   out->n_absorbed = st[S_ABS];
