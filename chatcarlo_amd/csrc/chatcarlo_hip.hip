@@ -245,6 +245,46 @@ __device__ __forceinline__ int wave_append(bool pred, uint32_t* counter) {
   return pred ? (int)(base + __popcll(mask & ((1ull << lane) - 1ull))) : -1;
 }
 
+// Queue slots for a whole block, for up to kMaxQueues queues at once.  bq_count puts each
+// wave's count for queue q in LDS; after a block barrier, thread q reserves the block's slots
+// of queue q with one atomic (bq_reserve; all queues in parallel); after a second barrier,
+// bq_slot gives each appending lane that base, plus the counts of the earlier waves, plus its
+// rank among its wave's appending lanes.  Per queue this is one atomic per block and one
+// round trip per wave, where per-wave appends take one round trip per queue, in sequence,
+// on counters that every wave of the launch hits.  Every thread of the block takes part.
+constexpr int kMaxQueues = 5;
+struct BlockQueues {
+  uint32_t cnt[kMaxQueues][kBlock / kWave];
+  uint32_t base[kMaxQueues][kBlock / kWave];
+};
+__device__ __forceinline__ unsigned long long bq_count(BlockQueues& bq, int q, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  if (__lane_id() == 0) bq.cnt[q][threadIdx.x / kWave] = (uint32_t)__popcll(m);
+  return m;
+}
+__device__ __forceinline__ void bq_reserve(BlockQueues& bq, int q, uint32_t* counter) {
+  const int nw = blockDim.x / kWave;
+  uint32_t tot = 0;
+  for (int w = 0; w < nw; ++w) tot += bq.cnt[q][w];
+  uint32_t b = tot ? atomicAdd(counter, tot) : 0u;
+  for (int w = 0; w < nw; ++w) {
+    bq.base[q][w] = b;
+    b += bq.cnt[q][w];
+  }
+}
+__device__ __forceinline__ int bq_slot(const BlockQueues& bq, int q, bool pred, unsigned long long m) {
+  const unsigned long long below = (1ull << __lane_id()) - 1ull;
+  return pred ? (int)(bq.base[q][threadIdx.x / kWave] + (uint32_t)__popcll(m & below)) : -1;
+}
+// One queue: the slot of each appending lane of the block, -1 elsewhere.
+__device__ __forceinline__ int block_append1(BlockQueues& bq, bool pred, uint32_t* counter) {
+  const unsigned long long m = bq_count(bq, 0, pred);
+  __syncthreads();
+  if (threadIdx.x == 0) bq_reserve(bq, 0, counter);
+  __syncthreads();
+  return bq_slot(bq, 0, pred, m);
+}
+
 __device__ __forceinline__ void wave_count(bool pred, unsigned long long* counter) {
   unsigned long long mask = __ballot(pred);
   if (mask != 0ull && __lane_id() == __ffsll((long long)mask) - 1)
@@ -691,6 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
   __shared__ SegStage seg_stage[kMulti ? 1 : kBlock / kWave];
   SegStage& sst = seg_stage[kMulti ? 0 : threadIdx.x / kWave];
   __shared__ TallyCache tc_lds;
+  __shared__ BlockQueues bq;
   TallyCache* tcache = &tc_lds;
   if (L.grid_on) cache_init(tcache);
   if (!kMulti) stage_prims(L.prims, sp_lds, L.W.n_prim);
@@ -889,23 +930,32 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
       if (take) put_record(L.Rec, 1, (uint32_t)k, p, d, E, S.score[i], i, L.rec_batch);
     }
     if (sreg > 0.f) atomic_add_f64(L.hscore + S.hid[i], (double)sreg);
-    if (kMulti || kStream) {   // a photon that does not interact (escaped, or stopped by an error) frees its slot
-      const int k = wave_append(valid && type < 0, L.MS.free_cnt);
-      if (k >= 0) L.MS.free_stack[k] = i;
-    }
     wave_count(valid && escaped, stat_copy(L) + S_ESC);
     wave_count(valid, stat_copy(L) + S_FLIGHT);
     wave_sum_small((uint32_t)events, stat_copy(L) + S_EVT);
     wave_sum_small((uint32_t)visits, stat_copy(L) + S_VOX);
-    int slot;
-    slot = wave_append(type == 0, L.cnt + C_QP);
-    if (slot >= 0) qp[slot] = i;
-    slot = wave_append(type == 1, L.cnt + C_QC);
-    if (slot >= 0) qc[slot] = i;
-    slot = wave_append(type == 2, L.cnt + C_QR);
-    if (slot >= 0) qr[slot] = i;
-    slot = wave_append(type == 3, L.cnt + C_COLL);
-    if (slot >= 0) coll[slot] = i;
+    {
+      // interaction queues; a photon that does not interact (escaped, or stopped by an
+      // error) frees its slot when slots are recycled
+      const bool frees = (kMulti || kStream) && valid && type < 0;
+      const unsigned long long m0 = bq_count(bq, 0, type == 0), m1 = bq_count(bq, 1, type == 1),
+                               m2 = bq_count(bq, 2, type == 2), m3 = bq_count(bq, 3, type == 3),
+                               m4 = bq_count(bq, 4, frees);
+      __syncthreads();
+      if (threadIdx.x < 5) {
+        const int q = threadIdx.x;
+        uint32_t* const c = q == 0 ? L.cnt + C_QP : q == 1 ? L.cnt + C_QC : q == 2 ? L.cnt + C_QR
+                          : q == 3 ? L.cnt + C_COLL : L.MS.free_cnt;
+        bq_reserve(bq, q, c);
+      }
+      __syncthreads();
+      int k;
+      if ((k = bq_slot(bq, 0, type == 0, m0)) >= 0) qp[k] = i;
+      if ((k = bq_slot(bq, 1, type == 1, m1)) >= 0) qc[k] = i;
+      if ((k = bq_slot(bq, 2, type == 2, m2)) >= 0) qr[k] = i;
+      if ((k = bq_slot(bq, 3, type == 3, m3)) >= 0) coll[k] = i;
+      if ((k = bq_slot(bq, 4, frees, m4)) >= 0) L.MS.free_stack[k] = i;
+    }
   }
   if (L.grid_on) {   // the block's cached tallies into its copy of the grid(s)
     __syncthreads();
@@ -928,6 +978,7 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_photo(Launch L, const int* q, const uint32_t* cnt_q, int* act_next,
                                                   uint32_t* cnt_next) {
+  __shared__ BlockQueues bq;
   const uint32_t n = *cnt_q;
   const Tables& T = L.T;
   PState S = L.S;
@@ -970,17 +1021,21 @@ __global__ __launch_bounds__(kBlock) void k_photo(Launch L, const int* q, const 
     wave_count(valid && !emit, stat_copy(L) + S_ABS);
     wave_count(valid && emit, stat_copy(L) + S_FLUOR);
     wave_count(valid, stat_copy(L) + S_SCAT);
-    const int slot = wave_append(valid && emit, cnt_next);
-    if (slot >= 0) act_next[slot] = i;
-    if (L.multi) {   // absorbed photons free their slot
-      const int k = wave_append(valid && !emit, L.MS.free_cnt);
-      if (k >= 0) L.MS.free_stack[k] = i;
-    }
+    // fluorescence photons fly on; absorbed photons free their slot when slots are recycled
+    const bool fly = valid && emit, frees = L.multi && valid && !emit;
+    const unsigned long long m0 = bq_count(bq, 0, fly), m1 = bq_count(bq, 1, frees);
+    __syncthreads();
+    if (threadIdx.x < 2) bq_reserve(bq, threadIdx.x, threadIdx.x == 0 ? cnt_next : L.MS.free_cnt);
+    __syncthreads();
+    int k;
+    if ((k = bq_slot(bq, 0, fly, m0)) >= 0) act_next[k] = i;
+    if ((k = bq_slot(bq, 1, frees, m1)) >= 0) L.MS.free_stack[k] = i;
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_compton(Launch L, const int* q, const uint32_t* cnt_q, int* act_next,
                                                     uint32_t* cnt_next) {
+  __shared__ BlockQueues bq;
   const uint32_t n = *cnt_q;
   const Tables& T = L.T;
   PState S = L.S;
@@ -1012,13 +1067,14 @@ __global__ __launch_bounds__(kBlock) void k_compton(Launch L, const int* q, cons
     }
     wave_add_by_key(valid, L.multi && valid ? S.sc[i] * CC_MAX_MAT + m : m, dep, dep_copy(L));
     wave_count(valid, stat_copy(L) + S_SCAT);
-    const int slot = wave_append(valid, cnt_next);
+    const int slot = block_append1(bq, valid, cnt_next);
     if (slot >= 0) act_next[slot] = i;
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_rayleigh(Launch L, const int* q, const uint32_t* cnt_q, int* act_next,
                                                      uint32_t* cnt_next) {
+  __shared__ BlockQueues bq;
   const uint32_t n = *cnt_q;
   const Tables& T = L.T;
   PState S = L.S;
@@ -1046,7 +1102,7 @@ __global__ __launch_bounds__(kBlock) void k_rayleigh(Launch L, const int* q, con
       S.ctr[i] = rng.ctr;
     }
     wave_count(valid, stat_copy(L) + S_SCAT);
-    const int slot = wave_append(valid, cnt_next);
+    const int slot = block_append1(bq, valid, cnt_next);
     if (slot >= 0) act_next[slot] = i;
   }
 }
