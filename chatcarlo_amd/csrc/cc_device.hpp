@@ -570,8 +570,12 @@ struct GridDev {
   size_t cstride; // elements between copies
 };
 
+// FP64 add to device memory.  Every caller passes global memory, and the pointer is cast
+// to say so: through a generic pointer the atomic is a flat one, which also counts
+// against the LDS wait counter, so the wave's next LDS access would wait for its L2 round trip.
+typedef __attribute__((address_space(1))) double global_f64;
 __device__ __forceinline__ void atomic_add_f64(double* p, double v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add((global_f64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Block-private cache of tally contributions in LDS: a direct-mapped table of

@@ -223,6 +223,31 @@ __device__ __forceinline__ uint64_t photon_seed(const Launch& L, const PState& S
   return L.multi ? L.MS.seeds[S.sc[i]] : L.seed;
 }
 
+// Photon i's 64-byte record as four 16-byte words (the pool is 256-byte aligned):
+// [0] x y z dx, [1] dy dz e tau, [2] wt ctr hid tag, [3] nsc mat flags sc.
+__device__ __forceinline__ const uint4* photon_record(const PState& S, int i) {
+  static_assert(W_X == 0 && W_DY == 4 && W_WT == 8 && W_NSC == 12 && W_SC == 15, "record layout");
+  return reinterpret_cast<const uint4*>(&S.x[(size_t)i]);
+}
+
+// A photon record read whole, in four 16-byte loads; fields by word index (W_*).
+struct PhotonWords {
+  uint4 q[4];
+  __device__ __forceinline__ uint32_t u(int w) const {
+    const uint4& v = q[w >> 2];
+    return (w & 3) == 0 ? v.x : (w & 3) == 1 ? v.y : (w & 3) == 2 ? v.z : v.w;
+  }
+  __device__ __forceinline__ float f(int w) const { return __uint_as_float(u(w)); }
+};
+__device__ __forceinline__ PhotonWords load_photon(const PState& S, int i) {
+  const uint4* r = photon_record(S, i);
+  return PhotonWords{{r[0], r[1], r[2], r[3]}};
+}
+// The Philox key and stream of a loaded photon (photon_seed for a record in registers).
+__device__ __forceinline__ PhiloxRng photon_rng(const Launch& L, const PhotonWords& P) {
+  return PhiloxRng(L.multi ? L.MS.seeds[P.u(W_SC)] : L.seed, L.hist0 + (uint64_t)P.u(W_HID), P.u(W_TAG), P.u(W_CTR));
+}
+
 // The calling block's copies of the event counters and of the deposit sums.
 __device__ __forceinline__ unsigned long long* stat_copy(const Launch& L) {
   return L.stats + (size_t)(blockIdx.x & (kAccCopies - 1)) * kStatStride;
@@ -759,21 +784,26 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
     GridDev G = L.G;
     if (valid) {
       i = act[j];
-      if (kStream) scene = S.sc[i];   // the batch slot of the photon's history
+      // the record in 16-byte loads: words 0-11 always, 12-15 (scatter count, slot) when used
+      const uint4* rec = photon_record(S, i);
+      const uint4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+      if (kStream || kMulti || kTrace) {
+        const uint4 r3 = rec[3];
+        if (kStream || kMulti) scene = (int)r3.w;   // W_SC: the batch slot (streamed run) or scene (batched scenes)
+        if (kTrace) flight = (int)r3.x;             // W_NSC
+      }
       if (kMulti) {
-        scene = S.sc[i];
         sp = L.MS.prims + L.MS.prim_off[scene];
         Wd = L.MS.worlds[scene];
         G = L.MS.grids[scene];
       }
-      p[0] = S.x[i]; p[1] = S.y[i]; p[2] = S.z[i];
-      d[0] = S.dx[i]; d[1] = S.dy[i]; d[2] = S.dz[i];
-      E = S.e[i];
-      tau = S.tau[i];
-      w = S.wt[i];
+      p[0] = __uint_as_float(r0.x); p[1] = __uint_as_float(r0.y); p[2] = __uint_as_float(r0.z);
+      d[0] = __uint_as_float(r0.w); d[1] = __uint_as_float(r1.x); d[2] = __uint_as_float(r1.y);
+      E = __uint_as_float(r1.z);
+      tau = __uint_as_float(r1.w);
+      w = __uint_as_float(r2.x);
       key = kMulti ? L.MS.seeds[scene] : L.seed;
-      hid = S.hid[i]; tag = S.tag[i]; ctr = S.ctr[i];
-      if (kTrace) flight = S.nsc[i];
+      ctr = r2.y; hid = r2.z; tag = r2.w;
       if (!energy_ok(E)) {
         raise_err(L.err, CC_DEV_ENERGY_RANGE);
         S.flags[i] |= F_DEAD;
@@ -985,15 +1015,17 @@ __global__ __launch_bounds__(kBlock) void k_photo(Launch L, const int* q, const 
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t j = base + threadIdx.x;
     const bool valid = j < n;
-    int i = 0, m = 0;
+    int i = 0, m = 0, sc = 0;
     bool emit = false;
     double dep = 0.0;
     if (valid) {
       i = q[j];
-      m = S.mat[i];
-      const float E = S.e[i];
-      const float w = S.wt[i];
-      PhiloxRng rng(photon_seed(L, S, i), L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
+      const PhotonWords P = load_photon(S, i);
+      m = (int)P.u(W_MAT);
+      sc = (int)P.u(W_SC);
+      const float E = P.f(W_E);
+      const float w = P.f(W_WT);
+      PhiloxRng rng = photon_rng(L, P);
       float eline = 0.f;
       if (L.fluor) {
         const EPos ep = energy_pos(T, E);
@@ -1001,7 +1033,7 @@ __global__ __launch_bounds__(kBlock) void k_photo(Launch L, const int* q, const 
         emit = fluor_sample(T, m, E, ep, rng, el, eline);
       }
       dep = (double)w * (double)(emit ? E - eline : E);
-      S.nsc[i] += 1;
+      S.nsc[i] = (int)P.u(W_NSC) + 1;
       if (emit && L.Tr.on) {
         const int k = L.Tr.last[i];
         if (k >= 0 && (uint32_t)k < L.Tr.cap) L.Tr.rec[(size_t)k * 12 + 11] = __int_as_float(CC_EV_FLUORESCENCE);
@@ -1013,11 +1045,11 @@ __global__ __launch_bounds__(kBlock) void k_photo(Launch L, const int* q, const 
         S.e[i] = eline;
         S.tau[i] = -logf(rng.next());
       } else {
-        S.flags[i] |= F_ABSORBED;
+        S.flags[i] = (int)P.u(W_FLAGS) | F_ABSORBED;
       }
       S.ctr[i] = rng.ctr;
     }
-    wave_add_by_key(valid, L.multi && valid ? S.sc[i] * CC_MAX_MAT + m : m, dep, dep_copy(L));
+    wave_add_by_key(valid, L.multi && valid ? sc * CC_MAX_MAT + m : m, dep, dep_copy(L));
     wave_count(valid && !emit, stat_copy(L) + S_ABS);
     wave_count(valid && emit, stat_copy(L) + S_FLUOR);
     wave_count(valid, stat_copy(L) + S_SCAT);
@@ -1042,14 +1074,16 @@ __global__ __launch_bounds__(kBlock) void k_compton(Launch L, const int* q, cons
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t j = base + threadIdx.x;
     const bool valid = j < n;
-    int i = 0, m = 0;
+    int i = 0, m = 0, sc = 0;
     double dep = 0.0;
     if (valid) {
       i = q[j];
-      m = S.mat[i];
-      const float E = S.e[i];
-      const float w = S.wt[i];
-      PhiloxRng rng(photon_seed(L, S, i), L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
+      const PhotonWords P = load_photon(S, i);
+      m = (int)P.u(W_MAT);
+      sc = (int)P.u(W_SC);
+      const float E = P.f(W_E);
+      const float w = P.f(W_WT);
+      PhiloxRng rng = photon_rng(L, P);
       const EPos ep = energy_pos(T, E);
       float cs, sn;
       int el, trials;
@@ -1057,15 +1091,15 @@ __global__ __launch_bounds__(kBlock) void k_compton(Launch L, const int* q, cons
       if (trials == 0) raise_err(L.err, CC_DEV_ITER_CAP);
       const float enew = E * eps;
       dep = (double)w * ((double)E - (double)enew);
-      float d[3] = {S.dx[i], S.dy[i], S.dz[i]};
+      float d[3] = {P.f(W_DX), P.f(W_DY), P.f(W_DZ)};
       rotate_dir(d, cs, sn, rng.next());
       S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
       S.e[i] = enew;
       S.tau[i] = -logf(rng.next());
-      S.nsc[i] += 1;
+      S.nsc[i] = (int)P.u(W_NSC) + 1;
       S.ctr[i] = rng.ctr;
     }
-    wave_add_by_key(valid, L.multi && valid ? S.sc[i] * CC_MAX_MAT + m : m, dep, dep_copy(L));
+    wave_add_by_key(valid, L.multi && valid ? sc * CC_MAX_MAT + m : m, dep, dep_copy(L));
     wave_count(valid, stat_copy(L) + S_SCAT);
     const int slot = block_append1(bq, valid, cnt_next);
     if (slot >= 0) act_next[slot] = i;
@@ -1084,9 +1118,10 @@ __global__ __launch_bounds__(kBlock) void k_rayleigh(Launch L, const int* q, con
     int i = 0;
     if (valid) {
       i = q[j];
-      const int m = S.mat[i];
-      const float E = S.e[i];
-      PhiloxRng rng(photon_seed(L, S, i), L.hist0 + (uint64_t)S.hid[i], S.tag[i], S.ctr[i]);
+      const PhotonWords P = load_photon(S, i);
+      const int m = (int)P.u(W_MAT);
+      const float E = P.f(W_E);
+      PhiloxRng rng = photon_rng(L, P);
       const EPos ep = energy_pos(T, E);
       int el, trials;
       bool qerr;
@@ -1094,11 +1129,11 @@ __global__ __launch_bounds__(kBlock) void k_rayleigh(Launch L, const int* q, con
       const float cs = rayleigh_sample(T, m, E, ep, rng, kTrialCap, sn, el, trials, qerr);
       if (trials == 0) raise_err(L.err, CC_DEV_ITER_CAP);
       if (qerr) raise_err(L.err, CC_DEV_Q_RANGE);
-      float d[3] = {S.dx[i], S.dy[i], S.dz[i]};
+      float d[3] = {P.f(W_DX), P.f(W_DY), P.f(W_DZ)};
       rotate_dir(d, cs, sn, rng.next());
       S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
       S.tau[i] = -logf(rng.next());
-      S.nsc[i] += 1;
+      S.nsc[i] = (int)P.u(W_NSC) + 1;
       S.ctr[i] = rng.ctr;
     }
     wave_count(valid, stat_copy(L) + S_SCAT);
