@@ -27,16 +27,33 @@ EVENT_B = 128              # SoA state read + write per transport event
 SOURCE_B = 64              # source write per history
 
 
-def measured_traffic(kernel="k_flight"):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_chest_room_hbm_traffic.md")
+def _profile_figure(name, key):
+    """A figure written as `key: value` in a committed rocprofv3 summary under profiles/."""
     try:
-        for line in open(path):
-            if line.startswith(f"{kernel}_traffic_bytes_per_launch:"):
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            if line.startswith(f"{key}:"):
                 return float(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def measured_traffic(kernel="k_flight"):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary."""
+    return _profile_figure(f"{kernel}_chest_room_hbm_traffic.md", f"{kernel}_traffic_bytes_per_launch")
+
+
+def measured_atomics(kernel="k_flight"):
+    """L2 atomic requests per launch of the dominant kernel (rocprofv3 TCC_ATOMIC) and per add of the
+    atomic-rate calibration, from the committed summary."""
+    name = f"{kernel}_chest_room_atomics.md"
+    return (_profile_figure(name, f"{kernel}_atomics_per_launch"),
+            _profile_figure(name, "calibration_requests_per_add"))
+
+
+# atomic-rate calibration: random FP64 adds over the footprint of 16 copies of the chest-room grid
+ATOMIC_CAL_SLOTS = 16 * 108 * 108 * 88
+ATOMIC_CAL_OPS = 64 << 21
 
 
 def _cpu_sample(args):
@@ -182,7 +199,15 @@ def main():
                     help="only the 64-scene batched sweep (prints its JSON object)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (PMC byte-counter calibration)")
+    ap.add_argument("--calibrate-atomics", action="store_true",
+                    help="only run the FP64 atomic-rate calibration (PMC atomic-counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_atomics:
+        from chatcarlo_amd.transport import default_engine
+        rate, total = default_engine().atomic_rate(ATOMIC_CAL_SLOTS, ATOMIC_CAL_OPS)
+        print(json.dumps({"adds_per_s": rate, "adds_per_launch": ATOMIC_CAL_OPS, "launches": 2,
+                          "array_total": total}))
+        return
     if args.calibrate_copy:
         import torch
         nbytes = args.calibrate_copy << 20
@@ -252,6 +277,11 @@ def main():
     n_vox = visits / hist
     b_hist = VOXEL_B * n_vox + EVENT_B * n_evt + SOURCE_B
     o = outs[-1]
+    cal_rate, cal_total = eng.atomic_rate(ATOMIC_CAL_SLOTS, ATOMIC_CAL_OPS)
+    atom_launch, req_per_add = measured_atomics("k_flight")
+    atom_achieved = None
+    if atom_launch and req_per_add:
+        atom_achieved = atom_launch / req_per_add / f_avg_s   # FP64 adds per second in k_flight
     kernel_share = {name: round(sum(x.kernel_ms[i] for x in outs) / (dt * 1000.0), 4)
                     for i, name in enumerate(["source", "flight", "photo", "compton", "rayleigh", "end_batch"])}
     line = {
@@ -266,6 +296,12 @@ def main():
                      "kernel": "k_flight",
                      "algorithmic_bytes_per_launch": f_bytes / max(f_launch, 1),
                      "avg_launch_ms": f_avg_s * 1000.0, "launches": int(f_launch)},
+        "atomic_roofline": {"bound": "fp64 atomic adds (performed beyond the L2s)", "kernel": "k_flight",
+                            "achieved": atom_achieved, "peak": cal_rate, "unit": "adds/s",
+                            "frac": atom_achieved / cal_rate if atom_achieved and cal_rate else None,
+                            "adds_per_launch": atom_launch / req_per_add if atom_launch and req_per_add else None,
+                            "peak_from": f"cc_atomic_rate: {ATOMIC_CAL_OPS} random adds over {ATOMIC_CAL_SLOTS} doubles",
+                            "calibration_total_ok": cal_total == 2.0 * ATOMIC_CAL_OPS},
         "path": {"voxel_scores_per_s": value * n_vox * 2, "n_evt_per_history": n_evt,
                  "n_vox_per_history": n_vox, "bytes_per_history": b_hist,
                  "path_gbs": value * b_hist / 1e9, "path_frac": value * b_hist / 1e9 / HBM_PEAK_GBS,

@@ -128,6 +128,7 @@ _SIGS = {
     "cc_bind_grid": (C.c_int, [C.c_void_p, dp, ip, C.c_double, C.c_int32]),
     "cc_unbind_grid": (C.c_int, [C.c_void_p]),
     "cc_set_grid_copies": (C.c_int, [C.c_void_p, C.c_int32]),
+    "cc_atomic_rate": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "cc_trace_alloc": (C.c_int, [C.c_void_p, C.c_uint32]),
     "cc_trace_read": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, u64p]),
     "cc_reset_grid": (C.c_int, [C.c_void_p]),

@@ -1874,6 +1874,19 @@ __global__ void k_fold_acc(unsigned long long* stats, double* dep, size_t dep_st
   }
 }
 
+// Calibration of the tally path's bound: every thread adds 1.0 at `per_thread` addresses
+// drawn uniformly from [0, n_slots) by a linear congruential sequence seeded by its index,
+// with the no-return FP64 atomic the tallies use.
+__global__ __launch_bounds__(kBlock) void k_atomic_rate(double* a, uint64_t n_slots, uint32_t per_thread,
+                                                        uint32_t seed) {
+  uint32_t h = (uint32_t)(blockIdx.x * blockDim.x + threadIdx.x) * 2654435761u ^ seed;
+  for (uint32_t k = 0; k < per_thread; ++k) {
+    h = h * 1664525u + 1013904223u;
+    const uint64_t v = ((uint64_t)h * n_slots) >> 32;
+    atomic_add_f64(a + v, 1.0);
+  }
+}
+
 __global__ void k_add_f64(double* dst, const double* src, size_t n) {
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) dst[k] += src[k];
 }
@@ -2678,6 +2691,44 @@ int cc_trace_read(void* ctx, uint64_t max_records, cc_trace_record* out, uint64_
   *n_total = cnt;
   const uint64_t k = std::min<uint64_t>(std::min<uint64_t>(cnt, c->tr_cap), max_records);
   if (k && out) HIP_TRY(hipMemcpy(out, c->tr_rec, k * sizeof(cc_trace_record), hipMemcpyDeviceToHost));
+  return CC_OK;
+}
+
+int cc_atomic_rate(void* ctx, uint64_t n_slots, uint64_t n_ops, double* ops_per_s, double* sum) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c || !ops_per_s) return fail(CC_ERR_ARG, "null argument");
+  if (n_slots < 1 || n_slots > (1ull << 32)) return fail(CC_ERR_ARG, "n_slots %llu out of [1, 2^32]",
+                                                        (unsigned long long)n_slots);
+  const uint64_t threads = (uint64_t)kBlock * 8192;
+  const uint64_t per = n_ops / threads;
+  if (per < 1 || per > 4096) return fail(CC_ERR_ARG, "n_ops must be 1..4096 times %llu", (unsigned long long)threads);
+  HIP_TRY(hipSetDevice(c->device));
+  double* a = nullptr;
+  int r = dalloc(&a, (size_t)n_slots);
+  if (r) return r;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks((size_t)n_slots)), dim3(kBlock), 0, c->stream, a, (size_t)n_slots, 0.0);
+  hipLaunchKernelGGL(k_atomic_rate, dim3(8192), dim3(kBlock), 0, c->stream, a, n_slots, (uint32_t)per, 1u);   // warm-up
+  HIP_TRY(hipEventRecord(e0, c->stream));
+  hipLaunchKernelGGL(k_atomic_rate, dim3(8192), dim3(kBlock), 0, c->stream, a, n_slots, (uint32_t)per, 2u);
+  HIP_TRY(hipEventRecord(e1, c->stream));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  *ops_per_s = ms > 0.f ? (double)(per * threads) / (ms * 1e-3) : 0.0;
+  if (sum) {   // the adds of both launches, summed on the host: 2 per * threads when none was lost
+    std::vector<double> h((size_t)n_slots);
+    HIP_TRY(hipMemcpy(h.data(), a, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double t = 0.0;
+    for (double x : h) t += x;
+    *sum = t;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
   return CC_OK;
 }
 

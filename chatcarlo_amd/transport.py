@@ -160,6 +160,13 @@ class Engine:
         N.check(self.lib.cc_set_grid_copies(self.ctx, int(copies)))
         self._grid_key = None
 
+    def atomic_rate(self, n_slots: int, n_ops: int):
+        """FP64 atomic adds per second at random addresses of an n_slots-double array (the bound
+        of the voxel tallies) and the array total after the timed and warm-up launches."""
+        rate, total = C.c_double(), C.c_double()
+        N.check(self.lib.cc_atomic_rate(self.ctx, int(n_slots), int(n_ops), C.byref(rate), C.byref(total)))
+        return rate.value, total.value
+
     def bind_grid(self, origin, shape, voxel_cm, track_uncertainty):
         key = (tuple(float(x) for x in origin), tuple(int(x) for x in shape), float(voxel_cm),
                bool(track_uncertainty))

@@ -190,6 +190,13 @@ int cc_bind_grid(void* ctx, const double* origin, const int32_t* shape, double v
  * the hot voxels of a beam are not contended across XCDs. No counterpart in the reference (its NumPy
  * tally has no concurrent writers); results do not depend on it beyond FP64 summation order. */
 int cc_set_grid_copies(void* ctx, int32_t copies);
+/* Calibration of the tally path's bound, with no counterpart in the reference: the rate of
+ * no-return FP64 atomic adds (the tally adds) at uniformly random addresses of an n_slots-double
+ * array, n_ops adds in one launch (n_ops a multiple 1..4096 of 2^21), as adds per second. On the
+ * MI355X these atomics are performed beyond the L2 caches, and this rate bounds the voxel tallies.
+ * sum (may be NULL) returns the array's total after the timed launch and a warm-up launch of the
+ * same size: 2 n_ops rounded down to whole per-thread counts when every add landed. */
+int cc_atomic_rate(void* ctx, uint64_t n_slots, uint64_t n_ops, double* ops_per_s, double* sum);
 int cc_unbind_grid(void* ctx);
 int cc_reset_grid(void* ctx);
 /* Copy the totals and moments back (any pointer may be NULL). */
