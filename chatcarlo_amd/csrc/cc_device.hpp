@@ -559,8 +559,7 @@ __device__ __forceinline__ void isotropic_dir(float d[3], float u1, float u2) {
 // 8 copies each XCD adds into its own copy and the hot voxels of a beam are not
 // contended across XCDs; the batch end sums the copies.
 struct GridDev {
-  double* dk;     // batch kerma [keV], copy 0
-  double* dh;     // batch H*(10) track integral [pSv cm^3], copy 0
+  double* t;      // batch tallies, copy 0: kerma [keV] of voxel v at t[2v], H*(10) track integral [pSv cm^3] at t[2v + 1]
   int nx, ny, nz;
   float org[3];
   float h;
@@ -618,6 +617,30 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
   return v;
 }
 
+// The value of the neighbouring lane (lane ^ 1): a DPP quad permutation (1, 0, 3, 2).
+__device__ __forceinline__ uint32_t xor1_u32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ uint64_t xor1_u64(uint64_t x) {
+  return (uint64_t)xor1_u32((uint32_t)x) | ((uint64_t)xor1_u32((uint32_t)(x >> 32)) << 32);
+}
+__device__ __forceinline__ double xor1_f64(double x) { return __longlong_as_double((long long)xor1_u64((uint64_t)__double_as_longlong(x))); }
+
+// Adds a at p[0] and b at p[1] (one voxel's kerma and H*(10)) for every lane with pred, two
+// lanes per voxel: in the first instruction each even lane adds its own a and its odd
+// neighbour that voxel's b, in the second the odd lanes' voxels likewise.  Both adds of a
+// voxel then fall in one 64-byte segment of one instruction, and the memory system merges
+// such adds into a single atomic request; beyond the L2, where these atomics are
+// performed, the request rate is the bound.  Every lane of the wave must call it.
+__device__ __forceinline__ void pair_add(bool pred, double* p, double a, double b) {
+  const bool odd = (__lane_id() & 1) != 0;
+  const bool qp = xor1_u32(pred ? 1u : 0u) != 0u;
+  double* const q = (double*)xor1_u64((uint64_t)p);
+  const double qa = xor1_f64(a), qb = xor1_f64(b);
+  if (odd ? qp : pred) atomic_add_f64(odd ? q + 1 : p, odd ? qb : a);   // the even lane's voxel
+  if (odd ? pred : qp) atomic_add_f64(odd ? p + 1 : q, odd ? b : qa);   // the odd lane's voxel
+}
+
 // Adds (ak, ah) to voxel v for every lane with has = true.  Lanes that share a
 // voxel are combined first (up to kAggRounds groups, led by the lowest pending
 // lane), so a wave whose photons walk the same voxels -- the primary beam near
@@ -631,8 +654,7 @@ __device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t 
                                            TallyCache* tc = nullptr) {
   const int lane = __lane_id();
   const size_t copy = (size_t)(blockIdx.x & G.cmask) * G.cstride;
-  double* const dk = G.dk + copy;
-  double* const dh = G.dh + copy;
+  double* const t = G.t + copy;
   unsigned long long pend = __ballot(has);
   for (int r = 0; r < kAggRounds && pend; ++r) {
     const int leader = __ffsll((long long)pend) - 1;
@@ -647,17 +669,16 @@ __device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t 
     if (m == (1ull << leader)) break;          // no partner: the rest go direct
     const double sk = wave_sum_f64(mine ? ak : 0.0);
     const double sh = wave_sum_f64(mine ? ah : 0.0);
-    if (lane == leader && !(tc && cache_add(tc, G.base + vl, sk, sh))) {
-      atomic_add_f64(dk + vl, sk);
-      atomic_add_f64(dh + vl, sh);
-    }
+    bool cached = false;
+    if (lane == leader && tc) cached = cache_add(tc, G.base + vl, sk, sh);
+    // the leader adds the kerma, its neighbour lane the H*(10), in one instruction
+    const bool direct = ((~__ballot(cached)) >> leader) & 1ull;
+    pair_add(direct && lane == leader, t + 2 * (size_t)vl, sk, sh);
     has = has && !mine;
     pend &= ~m;
   }
-  if (has && !(tc && cache_add(tc, G.base + v, ak, ah))) {
-    atomic_add_f64(dk + v, ak);
-    atomic_add_f64(dh + v, ah);
-  }
+  const bool direct = has && !(tc && cache_add(tc, G.base + v, ak, ah));
+  pair_add(direct, t + 2 * (size_t)v, ak, ah);
 }
 
 // Clips the segment o + t d, t in [0, L], to the grid box: [t0, t1]; false if
@@ -760,7 +781,7 @@ __device__ __forceinline__ int dda_walk(const GridDev& G, const float o[3], cons
   return visits;
 }
 
-// Scores wk*len into dk and wh*len into dh for every voxel the segment
+// Scores wk*len into the kerma and wh*len into the H*(10) tally of every voxel the segment
 // o + t d, t in [0, L], overlaps, for lanes with seg = true; each lane walks
 // its own segment.  All lanes of the wave must call it.
 template <bool kScene = false>
@@ -903,8 +924,7 @@ __device__ __forceinline__ int dda_balanced(const GridDev& G0, const GridDev* gr
       if (slot_stride) {   // streamed batches: the owner's tally slot (key = slot)
         GridDev Gs = G0;
         const size_t off = (size_t)skey * slot_stride;
-        Gs.dk += off;
-        Gs.dh += off;
+        Gs.t += 2 * off;
         Gs.base = (uint32_t)off;
         visits += dda_walk<false>(Gs, so, sd, ta, tb, swk, swh, act, skey, sa, fidx, cache);
       } else {

@@ -178,8 +178,7 @@ struct MultiScene {
   uint32_t* free_cnt;
   unsigned long long* cursor; // next history of the index space
   unsigned long long total;
-  double* dk0;                // scene 0's kerma tally: the tally-cache keys index from here
-  double* dh0;                // scene 0's H*(10) tally
+  double* t0;                 // scene 0's tally pairs (GridDev::t): the tally-cache keys index from here
   uint32_t cmask;             // tally copies - 1
   size_t cstride;             // doubles between tally copies
 };
@@ -596,17 +595,17 @@ __global__ __launch_bounds__(kBlock) void k_refill_stream(Launch L, Src s, int* 
 // Folds the batch slots of a streamed group into the grid totals and batch moments
 // (as k_end_batch does per batch), slot by slot in batch order, and clears them.  Batch
 // slot s holds n_s = min(batch, n_hist - (g0 + s) batch) histories.
-__global__ __launch_bounds__(kBlock) void k_end_group(double* dk, double* dh, size_t slot_stride, int nslot,
+__global__ __launch_bounds__(kBlock) void k_end_group(double* t, size_t slot_stride, int nslot,
                                                       double* tk, double* th, double* qk, double* qh, int* hits,
                                                       size_t nvox, uint64_t batch, uint64_t g0, uint64_t n_hist,
                                                       int track) {
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (size_t)gridDim.x * blockDim.x) {
     for (int s = 0; s < nslot; ++s) {
-      const size_t u = (size_t)s * slot_stride + v;
-      const double a = dk[u], b = dh[u];
+      double* const p = t + 2 * ((size_t)s * slot_stride + v);
+      const double a = p[0], b = p[1];
       if (a == 0.0 && b == 0.0) continue;
-      dk[u] = 0.0;
-      dh[u] = 0.0;
+      p[0] = 0.0;
+      p[1] = 0.0;
       tk[v] += a;
       th[v] += b;
       if (track) {
@@ -989,16 +988,14 @@ __global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, con
   }
   if (L.grid_on) {   // the block's cached tallies into its copy of the grid(s)
     __syncthreads();
-    double* const dk = kMulti ? L.MS.dk0 : L.G.dk;
-    double* const dh = kMulti ? L.MS.dh0 : L.G.dh;
     const size_t copy = kMulti ? (size_t)(blockIdx.x & L.MS.cmask) * L.MS.cstride
                                : (size_t)(blockIdx.x & L.G.cmask) * L.G.cstride;
-    for (int k = threadIdx.x; k < kCacheSlots; k += blockDim.x) {
+    double* const t = (kMulti ? L.MS.t0 : L.G.t) + copy;
+    // lanes 2j and 2j + 1 add the kerma and the H*(10) of one slot: one merged request per voxel
+    for (int e = threadIdx.x; e < 2 * kCacheSlots; e += blockDim.x) {
+      const int k = e >> 1, c = e & 1;
       const uint32_t v = tcache->key[k];
-      if (v != kCacheEmpty) {
-        atomic_add_f64(dk + copy + v, tcache->val[k][0]);
-        atomic_add_f64(dh + copy + v, tcache->val[k][1]);
-      }
+      if (v != kCacheEmpty) atomic_add_f64(t + 2 * (size_t)v + c, tcache->val[k][c]);
     }
   }
 }
@@ -1807,19 +1804,19 @@ __global__ void k_mlp_refresh(const float* w, uint16_t* wb) {
 // batch end: fold the batch buffers into totals and moments (chatcarlo/tally.py:59-94)
 // ---------------------------------------------------------------------------
 // Sums the batch copies of every voxel into the totals (and batch moments), clearing them.
-__global__ __launch_bounds__(kBlock) void k_end_batch(double* dk, double* dh, double* tk, double* th, double* qk,
+__global__ __launch_bounds__(kBlock) void k_end_batch(double* t, double* tk, double* th, double* qk,
                                                       double* qh, int* hits, size_t nvox, double inv_nb, int track,
                                                       int ncopy, size_t cstride) {
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (size_t)gridDim.x * blockDim.x) {
-    double a = dk[v], b = dh[v];
-    if (a != 0.0 || b != 0.0) dk[v] = dh[v] = 0.0;
+    double a = t[2 * v], b = t[2 * v + 1];
+    if (a != 0.0 || b != 0.0) t[2 * v] = t[2 * v + 1] = 0.0;
     for (int k = 1; k < ncopy; ++k) {
-      const size_t u = (size_t)k * cstride + v;
-      const double ak = dk[u], bk = dh[u];
+      double* const p = t + (size_t)k * cstride + 2 * v;
+      const double ak = p[0], bk = p[1];
       if (ak != 0.0 || bk != 0.0) {
         a += ak;
         b += bk;
-        dk[u] = dh[u] = 0.0;
+        p[0] = p[1] = 0.0;
       }
     }
     if (a == 0.0 && b == 0.0) continue;
@@ -2126,7 +2123,7 @@ struct Ctx {
   uint64_t* b_seeds = nullptr;
   uint64_t* b_hist_start = nullptr;
   float* b_spec = nullptr;
-  double* b_tally = nullptr;           // copies of (all scenes' kerma, then all scenes' H*(10)) totals
+  double* b_tally = nullptr;           // copies of all scenes' (kerma, H*(10)) pairs, voxel by voxel
   size_t b_tally_cap = 0;              // doubles allocated in b_tally
   size_t b_nvox = 0;
   std::vector<size_t> b_vox_off;       // first voxel of each scene
@@ -2548,7 +2545,7 @@ int cc_ctx_destroy(void* ctx) {
   (void)hipStreamSynchronize(c->stream);
   free_pool(c);
   for (void* p : c->tbufs) (void)hipFree(p);
-  void* ps[] = {c->st_tally, c->st_edep, c->st_seeds, c->tr_rec, c->tr_cnt, c->tr_last, c->prims, c->tk, c->th, c->qk, c->qh, c->hits, c->G.dk, c->G.dh, c->cnt, c->stats, c->edep, c->err,
+  void* ps[] = {c->st_tally, c->st_edep, c->st_seeds, c->tr_rec, c->tr_cnt, c->tr_last, c->prims, c->tk, c->th, c->qk, c->qh, c->hits, c->G.t, c->cnt, c->stats, c->edep, c->err,
                 c->spec, c->rec, c->rcnt, c->mw[0], c->mm[0], c->mv[0], c->mgrad[0], c->mwb[0],
                 c->mw[1], c->mm[1], c->mv[1], c->mgrad[1], c->mwb[1], c->bcdf, c->bwt, c->stage};
   for (void* p : ps)
@@ -2646,10 +2643,10 @@ int cc_bind_grid(void* ctx, const double* origin, const int32_t* shape, double v
   cc_unbind_grid(ctx);
   const size_t nv = (size_t)shape[0] * shape[1] * shape[2];
   if (nv >= 0xffffffffull) return fail(CC_ERR_ARG, "grid of %zu voxels: at most 2^32 - 2", nv);
-  const size_t cs = (nv + 31) & ~(size_t)31;   // copies start on 256 B boundaries
+  const size_t cs = 2 * ((nv + 31) & ~(size_t)31);   // (kerma, H*(10)) pairs; copies start on 512 B boundaries
   const size_t nc = (size_t)c->grid_copies;
   int r;
-  if ((r = dalloc(&c->G.dk, nc * cs)) || (r = dalloc(&c->G.dh, nc * cs)) || (r = dalloc(&c->tk, nv)) ||
+  if ((r = dalloc(&c->G.t, nc * cs)) || (r = dalloc(&c->tk, nv)) ||
       (r = dalloc(&c->th, nv)) || (r = dalloc(&c->qk, nv)) || (r = dalloc(&c->qh, nv)) || (r = dalloc(&c->hits, nv)))
     return r;
   c->nvox = nv;
@@ -2744,7 +2741,7 @@ int cc_set_grid_copies(void* ctx, int32_t copies) {
 int cc_unbind_grid(void* ctx) {
   Ctx* c = (Ctx*)ctx;
   if (!c) return fail(CC_ERR_ARG, "null ctx");
-  void* ps[] = {c->G.dk, c->G.dh, c->tk, c->th, c->qk, c->qh, c->hits};
+  void* ps[] = {c->G.t, c->tk, c->th, c->qk, c->qh, c->hits};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   c->G = GridDev{};
@@ -2760,8 +2757,7 @@ int cc_reset_grid(void* ctx) {
   if (!c || !c->have_grid) return fail(CC_ERR_STATE, "no grid bound");
   const int nb = grid_blocks(c->nvox);
   const size_t nbatch = (size_t)(c->G.cmask + 1) * c->G.cstride;
-  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(nbatch)), dim3(kBlock), 0, c->stream, c->G.dk, nbatch, 0.0);
-  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(nbatch)), dim3(kBlock), 0, c->stream, c->G.dh, nbatch, 0.0);
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(nbatch)), dim3(kBlock), 0, c->stream, c->G.t, nbatch, 0.0);
   double* ds[] = {c->tk, c->th, c->qk, c->qh};
   for (double* d : ds) hipLaunchKernelGGL(k_fill_f64, dim3(nb), dim3(kBlock), 0, c->stream, d, c->nvox, 0.0);
   hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(kBlock), 0, c->stream, c->hits, c->nvox, 0);
@@ -2882,8 +2878,7 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
     L.St.g0 = g0;
     L.St.slot_stride = nvp;
     if (grid_on) {
-      L.G.dk = c->st_tally;
-      L.G.dh = c->st_tally + (size_t)slots * nvp;
+      L.G.t = c->st_tally;   // copy: [slot][voxel][kerma, H*(10)]
       L.G.cstride = 2 * (size_t)slots * nvp;
       L.G.base = 0;
     }
@@ -2935,7 +2930,7 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
       if (ncopy > 1)
         hipLaunchKernelGGL(k_fold_copies, dim3(grid_blocks(2 * (size_t)slots * nvp)), dim3(kBlock), 0, c->stream,
                            c->st_tally, 2 * (size_t)slots * nvp, ncopy, 1);
-      hipLaunchKernelGGL(k_end_group, dim3(grid_blocks(c->nvox)), dim3(kBlock), 0, c->stream, L.G.dk, L.G.dh, nvp, ns,
+      hipLaunchKernelGGL(k_end_group, dim3(grid_blocks(c->nvox)), dim3(kBlock), 0, c->stream, L.G.t, nvp, ns,
                          c->tk, c->th, c->qk, c->qh, c->hits, c->nvox, (uint64_t)batch, g0, n_hist, track ? 1 : 0);
       tm.end();
       if (track) c->g_batches += ns;
@@ -3072,7 +3067,7 @@ int cc_run(void* ctx, const cc_source* src, uint64_t n_hist, uint64_t seed, uint
                          c->edep + CC_MAX_MAT);
     if (grid_on) {
       tm.begin(K_END);
-      hipLaunchKernelGGL(k_end_batch, dim3(grid_blocks(c->nvox)), dim3(kBlock), 0, c->stream, c->G.dk, c->G.dh, c->tk,
+      hipLaunchKernelGGL(k_end_batch, dim3(grid_blocks(c->nvox)), dim3(kBlock), 0, c->stream, c->G.t, c->tk,
                          c->th, c->qk, c->qh, c->hits, c->nvox, 1.0 / (double)nb, track ? 1 : 0,
                          (int)c->G.cmask + 1, c->G.cstride);
       tm.end();
@@ -3151,7 +3146,7 @@ int cc_transport(void* ctx, int64_t n, double* pos, double* dir, double* energy,
   if (r) return r;
   out->rounds_max = rounds;
   if (grid_on) {
-    hipLaunchKernelGGL(k_end_batch, dim3(grid_blocks(c->nvox)), dim3(kBlock), 0, c->stream, c->G.dk, c->G.dh, c->tk,
+    hipLaunchKernelGGL(k_end_batch, dim3(grid_blocks(c->nvox)), dim3(kBlock), 0, c->stream, c->G.t, c->tk,
                        c->th, c->qk, c->qh, c->hits, c->nvox, 1.0 / (double)n, 0,
                        (int)c->G.cmask + 1, c->G.cstride);
     HIP_TRY(hipGetLastError());
@@ -3359,7 +3354,7 @@ int cc_test_dda(void* ctx, int64_t n, const float* o, const float* d, const floa
     return r;
   hipLaunchKernelGGL(k_test_dda, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->G, (int)n, dO, dD, dL, dk,
                      dh);
-  hipLaunchKernelGGL(k_end_batch, dim3(grid_blocks(c->nvox)), dim3(kBlock), 0, c->stream, c->G.dk, c->G.dh, c->tk, c->th,
+  hipLaunchKernelGGL(k_end_batch, dim3(grid_blocks(c->nvox)), dim3(kBlock), 0, c->stream, c->G.t, c->tk, c->th,
                      c->qk, c->qh, c->hits, c->nvox, 1.0, 0, (int)c->G.cmask + 1, c->G.cstride);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -3798,8 +3793,7 @@ int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int
   c->b_grid_host.resize(n_scenes);
   for (int s = 0; s < n_scenes; ++s) {
     GridDev& G = c->b_grid_host[s];
-    G.dk = c->b_tally + voff[s];
-    G.dh = c->b_tally + nvox + voff[s];
+    G.t = c->b_tally + 2 * voff[s];
     G.nx = grid_shape[3 * s]; G.ny = grid_shape[3 * s + 1]; G.nz = grid_shape[3 * s + 2];
     for (int k = 0; k < 3; ++k) G.org[k] = (float)grid_origin[3 * s + k];
     G.h = (float)voxel_cm[s];
@@ -3843,8 +3837,7 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
     }
     for (int s = 0; s < n; ++s) {
       GridDev& G = c->b_grid_host[s];
-      G.dk = c->b_tally + c->b_vox_off[s];
-      G.dh = c->b_tally + c->b_nvox + c->b_vox_off[s];
+      G.t = c->b_tally + 2 * c->b_vox_off[s];
       G.cmask = ncopy - 1;
       G.cstride = 2 * c->b_nvox;
       G.base = (uint32_t)c->b_vox_off[s];
@@ -3879,8 +3872,7 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
   L.MS.srcs = c->b_srcs; L.MS.seeds = c->b_seeds; L.MS.hist_start = c->b_hist_start; L.MS.n = n;
   L.MS.free_stack = c->b_free; L.MS.free_cnt = c->b_free_cnt; L.MS.cursor = c->b_cursor;
   L.MS.total = (unsigned long long)total;
-  L.MS.dk0 = c->b_tally;
-  L.MS.dh0 = c->b_tally + c->b_nvox;
+  L.MS.t0 = c->b_tally;
   L.MS.cmask = ncopy - 1;
   L.MS.cstride = 2 * c->b_nvox;
   const int bs = tune->block;
@@ -3980,8 +3972,14 @@ int cc_batch_read(void* ctx, int32_t scene, double* kerma_keV, double* h10_pSv_c
   HIP_TRY(hipSetDevice(c->device));
   const GridDev& G = c->b_grid_host[scene];
   const size_t nv = (size_t)G.nx * G.ny * G.nz;
-  if (kerma_keV) HIP_TRY(hipMemcpy(kerma_keV, G.dk, nv * sizeof(double), hipMemcpyDeviceToHost));
-  if (h10_pSv_cm3) HIP_TRY(hipMemcpy(h10_pSv_cm3, G.dh, nv * sizeof(double), hipMemcpyDeviceToHost));
+  if (kerma_keV || h10_pSv_cm3) {   // (kerma, H*(10)) pairs
+    std::vector<double> pr(2 * nv);
+    HIP_TRY(hipMemcpy(pr.data(), G.t, pr.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t v = 0; v < nv; ++v) {
+      if (kerma_keV) kerma_keV[v] = pr[2 * v];
+      if (h10_pSv_cm3) h10_pSv_cm3[v] = pr[2 * v + 1];
+    }
+  }
   if (edep_keV)
     HIP_TRY(hipMemcpy(edep_keV, c->b_edep + (size_t)scene * CC_MAX_MAT, CC_MAX_MAT * sizeof(double),
                       hipMemcpyDeviceToHost));
