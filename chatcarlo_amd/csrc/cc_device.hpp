@@ -602,7 +602,10 @@ __device__ __forceinline__ void cache_init(TallyCache* c) {
 }
 
 __device__ __forceinline__ bool cache_add(TallyCache* c, uint32_t v, double a, double b) {
-  const uint32_t s = (v * 2654435761u) >> (32 - kCacheLog2);
+  // The four voxels whose tally pairs share a 64-byte segment (keys 4k .. 4k + 3) take four
+  // consecutive slots, so the block-end flush adds them from neighbouring lanes of one
+  // instruction, which the memory system merges into one request; groups are hashed.
+  const uint32_t s = ((((v >> 2) * 2654435761u) >> (32 - (kCacheLog2 - 2))) << 2) | (v & 3u);
   uint32_t cur = c->key[s];
   if (cur == kCacheEmpty) cur = atomicCAS(&c->key[s], kCacheEmpty, v);
   if (cur != kCacheEmpty && cur != v) return false;
