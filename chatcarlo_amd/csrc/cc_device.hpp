@@ -584,8 +584,7 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v) {
 // voxel owns goes straight to memory; the table is added into the grid once, when the
 // block ends.  (Probing further slots was measured slower: when the table misses, the
 // voxels are not shared anyway.)
-constexpr int kCacheSlots = 2048;
-constexpr int kCacheLog2 = 11;
+constexpr int kCacheSlots = 1536;   // a multiple of 4; 30 KB, so that three blocks fit a CU's LDS
 constexpr uint32_t kCacheEmpty = 0xffffffffu;
 struct TallyCache {
   uint32_t key[kCacheSlots];
@@ -605,7 +604,8 @@ __device__ __forceinline__ bool cache_add(TallyCache* c, uint32_t v, double a, d
   // The four voxels whose tally pairs share a 64-byte segment (keys 4k .. 4k + 3) take four
   // consecutive slots, so the block-end flush adds them from neighbouring lanes of one
   // instruction, which the memory system merges into one request; groups are hashed.
-  const uint32_t s = ((((v >> 2) * 2654435761u) >> (32 - (kCacheLog2 - 2))) << 2) | (v & 3u);
+  const uint32_t g = (uint32_t)(((uint64_t)((v >> 2) * 2654435761u) * (uint32_t)(kCacheSlots / 4)) >> 32);
+  const uint32_t s = (g << 2) | (v & 3u);
   uint32_t cur = c->key[s];
   if (cur == kCacheEmpty) cur = atomicCAS(&c->key[s], kCacheEmpty, v);
   if (cur != kCacheEmpty && cur != v) return false;

@@ -748,8 +748,11 @@ __device__ float region_expect(const Prim* sp, const World& W, const Tables& T, 
 // separate instantiation so the plain transport keeps its register budget.
 // kMulti: batched run; every photon reads its scene's primitives (from global
 // memory, cached), world box, grid and seed, and a photon that leaves frees its slot.
+// Three waves per SIMD (at most 168 VGPRs, a few spilled; 51 KB of LDS, three blocks per
+// CU) run faster than two with every value in registers: the kernel waits on memory
+// and LDS far more than it issues.
 template <bool kRec, bool kMulti, bool kTrace = false, bool kStream = false>
-__global__ __launch_bounds__(kBlock) void k_flight(Launch L, const int* act, const uint32_t* cnt_in, int* qp,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_flight(Launch L, const int* act, const uint32_t* cnt_in, int* qp,
                                                    int* qc, int* qr, int* coll) {
   __shared__ Prim sp_lds[kMulti ? 1 : CC_MAX_PRIMS];
   __shared__ SegStage seg_stage[kMulti ? 1 : kBlock / kWave];
@@ -1328,7 +1331,7 @@ __device__ __forceinline__ void dir_at(const float d[3], const float e1[3], cons
 }
 
 constexpr int kStageFields = 14;   // x y z dx dy dz e wt tau ctr tag hid nsc mat
-constexpr int kSplitBlocks = 1024;  // grid of the collision kernel (its staging is per wave)
+constexpr int kSplitBlocks = 4096;  // grid of the collision kernel (its staging is per wave: 470 MB at split_max 4)
 
 // Collisions of a learned-importance run, fused: split, sample, stratify, window.
 //
@@ -1360,7 +1363,9 @@ constexpr int kSplitBlocks = 1024;  // grid of the collision kernel (its staging
 // tile's allocation, so the pool cannot overflow); otherwise the tile does not
 // split (n = 1) and copy 0 is not stratified.  Counters and deposits are summed
 // in registers.
-__global__ __launch_bounds__(kBlock) void k_scatter_split(Launch L, Importance I, const int* in,
+// Two waves per SIMD (a few spilled registers) hide more of its latency than one wave with
+// every value in registers.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_scatter_split(Launch L, Importance I, const int* in,
                                                           const uint32_t* cnt_in, int* act_next, uint32_t* cnt_next) {
   __shared__ float sbw[2 * kMlpH];
   MlpWave M;

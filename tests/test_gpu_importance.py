@@ -309,7 +309,7 @@ def test_learned_importance_is_unbiased_and_beats_analog(engine):
     reaches the same relative error with a higher figure of merit 1/(R^2 t)."""
     li = _angio(engine)
     n_rec, _ = li.generate_records(10_000_000, 32_000_000, keep=0.3, seed=61)
-    t0 = li.train(0, steps=6000, batch=8192, lr=3e-3, lr_end=1e-4)
+    t0 = li.train(0, steps=12000, batch=8192, lr=3e-3, lr_end=1e-4)
     assert t0.loss_last < t0.loss_first
     li.configure()
     ana, lrn, chosen, z = li.compare(target_rel_err=0.02, seed_analog=71, seed_learned=72)
