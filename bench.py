@@ -44,8 +44,9 @@ def measured_traffic(kernel="k_flight"):
 
 
 def measured_atomics(kernel="k_flight"):
-    """L2 atomic requests per launch of the dominant kernel (rocprofv3 TCC_ATOMIC) and per add of the
-    atomic-rate calibration, from the committed summary."""
+    """Atomic requests per launch of the dominant kernel (rocprofv3 TCC_ATOMIC; adds of one instruction
+    to one 64-byte segment travel as one request) and requests per add of the atomic-rate calibration
+    (random addresses), from the committed summary."""
     name = f"{kernel}_chest_room_atomics.md"
     return (_profile_figure(name, f"{kernel}_atomics_per_launch"),
             _profile_figure(name, "calibration_requests_per_add"))
@@ -279,9 +280,8 @@ def main():
     o = outs[-1]
     cal_rate, cal_total = eng.atomic_rate(ATOMIC_CAL_SLOTS, ATOMIC_CAL_OPS)
     atom_launch, req_per_add = measured_atomics("k_flight")
-    atom_achieved = None
-    if atom_launch and req_per_add:
-        atom_achieved = atom_launch / req_per_add / f_avg_s   # FP64 adds per second in k_flight
+    atom_achieved = atom_launch / f_avg_s if atom_launch else None   # atomic requests per second in k_flight
+    atom_peak = cal_rate * req_per_add if req_per_add else None
     kernel_share = {name: round(sum(x.kernel_ms[i] for x in outs) / (dt * 1000.0), 4)
                     for i, name in enumerate(["source", "flight", "photo", "compton", "rayleigh", "end_batch"])}
     line = {
@@ -296,11 +296,12 @@ def main():
                      "kernel": "k_flight",
                      "algorithmic_bytes_per_launch": f_bytes / max(f_launch, 1),
                      "avg_launch_ms": f_avg_s * 1000.0, "launches": int(f_launch)},
-        "atomic_roofline": {"bound": "fp64 atomic adds (performed beyond the L2s)", "kernel": "k_flight",
-                            "achieved": atom_achieved, "peak": cal_rate, "unit": "adds/s",
-                            "frac": atom_achieved / cal_rate if atom_achieved and cal_rate else None,
-                            "adds_per_launch": atom_launch / req_per_add if atom_launch and req_per_add else None,
-                            "peak_from": f"cc_atomic_rate: {ATOMIC_CAL_OPS} random adds over {ATOMIC_CAL_SLOTS} doubles",
+        "atomic_roofline": {"bound": "fp64 atomic requests (performed beyond the L2s)", "kernel": "k_flight",
+                            "achieved": atom_achieved, "peak": atom_peak, "unit": "requests/s",
+                            "frac": atom_achieved / atom_peak if atom_achieved and atom_peak else None,
+                            "requests_per_launch": atom_launch,
+                            "peak_from": f"cc_atomic_rate: {ATOMIC_CAL_OPS} random adds over {ATOMIC_CAL_SLOTS} doubles, "
+                                         f"{cal_rate:.4g} adds/s x {req_per_add} requests per add",
                             "calibration_total_ok": cal_total == 2.0 * ATOMIC_CAL_OPS},
         "path": {"voxel_scores_per_s": value * n_vox * 2, "n_evt_per_history": n_evt,
                  "n_vox_per_history": n_vox, "bytes_per_history": b_hist,
