@@ -247,6 +247,10 @@ __device__ __forceinline__ PhiloxRng photon_rng(const Launch& L, const PhotonWor
   return PhiloxRng(L.multi ? L.MS.seeds[P.u(W_SC)] : L.seed, L.hist0 + (uint64_t)P.u(W_HID), P.u(W_TAG), P.u(W_CTR));
 }
 
+__device__ __forceinline__ uint4* photon_record_w(const PState& S, int i) {
+  return reinterpret_cast<uint4*>(&S.x[(size_t)i]);
+}
+
 // The calling block's copies of the event counters and of the deposit sums.
 __device__ __forceinline__ unsigned long long* stat_copy(const Launch& L) {
   return L.stats + (size_t)(blockIdx.x & (kAccCopies - 1)) * kStatStride;
@@ -570,24 +574,28 @@ __global__ __launch_bounds__(kBlock) void k_refill_stream(Launch L, Src s, int* 
   const uint32_t fc = *M.free_cnt;
   const uint32_t n_new = refill_count(M, limit);
   PState S = L.S;
+  __shared__ BlockQueues bq;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n_new; base += gridDim.x * blockDim.x) {
     const uint32_t t = base + threadIdx.x;
     const bool valid = t < n_new;
     int q = 0;
     if (valid) {
-      const unsigned long long h = cur + t;
+      const uint32_t h = (uint32_t)(cur + t);   // a streamed run has fewer than 2^32 histories
       q = M.free_stack[fc - 1 - t];
       PhiloxRng rng(L.seed, L.hist0 + (uint64_t)h, 0u, 0u);
       float p[3], d[3], E;
       sample_source(s, rng, p, d, E);
       const float tau = -logf(rng.next());
-      S.x[q] = p[0]; S.y[q] = p[1]; S.z[q] = p[2];
-      S.dx[q] = d[0]; S.dy[q] = d[1]; S.dz[q] = d[2];
-      S.e[q] = E; S.tau[q] = tau; S.wt[q] = 1.0f; S.score[q] = 0.f;
-      S.ctr[q] = rng.ctr; S.hid[q] = (uint32_t)h; S.tag[q] = 0u; S.nsc[q] = 0; S.mat[q] = -1; S.flags[q] = 0;
-      S.sc[q] = (int)(h / L.St.batch - L.St.g0);
+      // the whole record in four 16-byte stores: position, direction, energy, tau, weight 1,
+      // counter, history, tag 0, no scatters, material unknown, no flags, batch slot
+      uint4* const r = photon_record_w(S, q);
+      r[0] = make_uint4(__float_as_uint(p[0]), __float_as_uint(p[1]), __float_as_uint(p[2]), __float_as_uint(d[0]));
+      r[1] = make_uint4(__float_as_uint(d[1]), __float_as_uint(d[2]), __float_as_uint(E), __float_as_uint(tau));
+      r[2] = make_uint4(__float_as_uint(1.0f), rng.ctr, h, 0u);
+      r[3] = make_uint4(0u, (uint32_t)-1, 0u, h / (uint32_t)L.St.batch - (uint32_t)L.St.g0);
+      S.score[q] = 0.f;
     }
-    const int k = wave_append(valid, cnt_act);
+    const int k = block_append1(bq, valid, cnt_act);
     if (k >= 0) act[k] = q;
   }
 }
