@@ -644,42 +644,14 @@ __device__ __forceinline__ void pair_add(bool pred, double* p, double a, double 
   if (odd ? pred : qp) atomic_add_f64(odd ? p + 1 : q, odd ? b : qa);   // the odd lane's voxel
 }
 
-// Adds (ak, ah) to voxel v for every lane with has = true.  Lanes that share a
-// voxel are combined first (up to kAggRounds groups, led by the lowest pending
-// lane), so a wave whose photons walk the same voxels -- the primary beam near
-// the focal spot -- issues one pair of atomics per voxel instead of one per
-// lane.  Every lane of the wave must call this (converged control flow).
-constexpr int kAggRounds = 4;
-// kScene: lanes may score different grids (batched runs); only lanes of one
-// scene (key) and one voxel are combined.
-template <bool kScene>
-__device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t v, double ak, double ah, uint32_t key,
+// Adds (ak, ah) to voxel v for every lane with has = true: into the block's LDS tally cache
+// when the voxel owns its slot, else to memory, both adds of a voxel in one request
+// (pair_add).  Lanes that share a voxel meet in the cache; matching them across the wave
+// first (ballots and lane shuffles at every DDA step) was measured 14 % slower once the
+// cache was there.  Every lane of the wave must call this (converged control flow).
+__device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t v, double ak, double ah,
                                            TallyCache* tc = nullptr) {
-  const int lane = __lane_id();
-  const size_t copy = (size_t)(blockIdx.x & G.cmask) * G.cstride;
-  double* const t = G.t + copy;
-  unsigned long long pend = __ballot(has);
-  for (int r = 0; r < kAggRounds && pend; ++r) {
-    const int leader = __ffsll((long long)pend) - 1;
-    // lanes match on the voxel's key (G.base + v): equal keys are one voxel of one grid
-    // (one scene, or one batch slot of a streamed run), so the leader's grid is theirs
-    const uint32_t gv = G.base + v;
-    const uint32_t vl = __shfl(v, leader);
-    const uint32_t gl = __shfl(gv, leader);
-    const uint32_t kl = kScene ? (uint32_t)__shfl((int)key, leader) : 0u;
-    const bool mine = has && gv == gl && (!kScene || key == kl);
-    const unsigned long long m = __ballot(mine);
-    if (m == (1ull << leader)) break;          // no partner: the rest go direct
-    const double sk = wave_sum_f64(mine ? ak : 0.0);
-    const double sh = wave_sum_f64(mine ? ah : 0.0);
-    bool cached = false;
-    if (lane == leader && tc) cached = cache_add(tc, G.base + vl, sk, sh);
-    // the leader adds the kerma, its neighbour lane the H*(10), in one instruction
-    const bool direct = ((~__ballot(cached)) >> leader) & 1ull;
-    pair_add(direct && lane == leader, t + 2 * (size_t)vl, sk, sh);
-    has = has && !mine;
-    pend &= ~m;
-  }
+  double* const t = G.t + (size_t)(blockIdx.x & G.cmask) * G.cstride;
   const bool direct = has && !(tc && cache_add(tc, G.base + v, ak, ah));
   pair_add(direct, t + 2 * (size_t)v, ak, ah);
 }
@@ -721,9 +693,8 @@ __device__ __forceinline__ float plane_t(float org_a, float h, float o_a, float 
 // overlapped.  fa >= 0 fixes the start index on axis fa to fidx (a walk that
 // starts on a plane of that axis).  All lanes of the wave must call it.
 // Returns the number of voxel visits of the calling lane.
-template <bool kScene>
 __device__ __forceinline__ int dda_walk(const GridDev& G, const float o[3], const float d[3], float t0, float t1,
-                                        double wk, double wh, bool act, uint32_t key, int fa = -1, int fidx = 0,
+                                        double wk, double wh, bool act, int fa = -1, int fidx = 0,
                                         TallyCache* cache = nullptr) {
   const int n[3] = {G.nx, G.ny, G.nz};
   int idx[3] = {0, 0, 0}, step[3] = {0, 0, 0};
@@ -779,7 +750,7 @@ __device__ __forceinline__ int dda_walk(const GridDev& G, const float o[3], cons
         if (out) act = false;
       }
     }
-    wave_tally<kScene>(G, has, v, ak, ah, key, cache);
+    wave_tally(G, has, v, ak, ah, cache);
   }
   return visits;
 }
@@ -787,13 +758,11 @@ __device__ __forceinline__ int dda_walk(const GridDev& G, const float o[3], cons
 // Scores wk*len into the kerma and wh*len into the H*(10) tally of every voxel the segment
 // o + t d, t in [0, L], overlaps, for lanes with seg = true; each lane walks
 // its own segment.  All lanes of the wave must call it.
-template <bool kScene = false>
 __device__ __forceinline__ int dda_score(const GridDev& G, const float o[3], const float d[3], float L,
-                                         double wk, double wh, bool seg, uint32_t key = 0,
-                                         TallyCache* cache = nullptr) {
+                                         double wk, double wh, bool seg, TallyCache* cache = nullptr) {
   float t0 = 0.f, t1 = 0.f;
   if (seg) seg = dda_clip(G, o, d, L, t0, t1);
-  return dda_walk<kScene>(G, o, d, t0, t1, wk, wh, seg, key, -1, 0, cache);
+  return dda_walk(G, o, d, t0, t1, wk, wh, seg, -1, 0, cache);
 }
 
 // Per-wave staging of the segments of a balanced tally.
@@ -922,16 +891,16 @@ __device__ __forceinline__ int dda_balanced(const GridDev& G0, const GridDev* gr
     }
     if (kScene) {
       const GridDev Gs = grids[act ? skey : 0];
-      visits += dda_walk<true>(Gs, so, sd, ta, tb, swk, swh, act, skey, sa, fidx);
+      visits += dda_walk(Gs, so, sd, ta, tb, swk, swh, act, sa, fidx);
     } else {
       if (slot_stride) {   // streamed batches: the owner's tally slot (key = slot)
         GridDev Gs = G0;
         const size_t off = (size_t)skey * slot_stride;
         Gs.t += 2 * off;
         Gs.base = (uint32_t)off;
-        visits += dda_walk<false>(Gs, so, sd, ta, tb, swk, swh, act, skey, sa, fidx, cache);
+        visits += dda_walk(Gs, so, sd, ta, tb, swk, swh, act, sa, fidx, cache);
       } else {
-        visits += dda_walk<false>(G0, so, sd, ta, tb, swk, swh, act, skey, sa, fidx, cache);
+        visits += dda_walk(G0, so, sd, ta, tb, swk, swh, act, sa, fidx, cache);
       }
     }
   }

@@ -889,7 +889,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         // leave most lanes idle otherwise); several scenes: each lane walks its own segment, as
         // the balanced walk's per-chunk scene lookups cost more than balancing gains there
         if constexpr (kMulti)
-          visits += dda_score<true>(G, pend_p, d, pend_len, pend_wk, wh, pend, (uint32_t)scene, tcache);
+          visits += dda_score(G, pend_p, d, pend_len, pend_wk, wh, pend, tcache);
         else
           visits += dda_balanced<false>(G, L.MS.grids, sst, pend_p, d, pend_len, pend_wk, wh, pend,
                                         kStream ? (uint32_t)scene : 0u, tcache, kStream ? L.St.slot_stride : 0);
