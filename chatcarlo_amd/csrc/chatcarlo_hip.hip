@@ -789,7 +789,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     uint64_t key = 0;
     uint32_t hid = 0, tag = 0, ctr = 0;
     EPos ep{0, 0.f, 0.f};
-    const Prim* sp = sp_lds;
+    // batched scenes read their scene's primitives from global memory: starting from that array's
+    // base keeps every value of sp a global pointer, so its loads are global, not flat, loads
+    const Prim* sp = kMulti ? L.MS.prims : sp_lds;
     World Wd = L.W;
     GridDev G = L.G;
     if (valid) {
