@@ -406,14 +406,19 @@ __device__ __forceinline__ int select_elem(const Tables& T, int m, int kind, con
 }
 
 // np.interp(x, xp, fp) on a sorted table; caller gives a starting guess j0.
+// The loads of the common case (the guess brackets x) are issued together, then checked.
 __device__ __forceinline__ float interp_guess(const float* xp, const float* fp, int n, float x, int j) {
-  if (x <= xp[0]) return fp[0];
-  if (x >= xp[n - 1]) return fp[n - 1];
   j = min(max(j, 0), n - 2);
-  while (j > 0 && x < xp[j]) --j;
-  while (j < n - 2 && x >= xp[j + 1]) ++j;
-  float x0 = xp[j], x1 = xp[j + 1];
-  return fp[j] + (fp[j + 1] - fp[j]) / (x1 - x0) * (x - x0);
+  const float xa = xp[0], xb = xp[n - 1], fa = fp[0], fb = fp[n - 1];
+  float x0 = xp[j], x1 = xp[j + 1], f0 = fp[j], f1 = fp[j + 1];
+  if (x <= xa) return fa;
+  if (x >= xb) return fb;
+  if (!(x >= x0 && x < x1)) {
+    while (j > 0 && x < xp[j]) --j;
+    while (j < n - 2 && x >= xp[j + 1]) ++j;
+    x0 = xp[j]; x1 = xp[j + 1]; f0 = fp[j]; f1 = fp[j + 1];
+  }
+  return f0 + (f1 - f0) / (x1 - x0) * (x - x0);
 }
 
 template <class Rng>
