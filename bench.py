@@ -283,7 +283,8 @@ def main():
     atom_achieved = atom_launch / f_avg_s if atom_launch else None   # atomic requests per second in k_flight
     atom_peak = cal_rate * req_per_add if req_per_add else None
     kernel_share = {name: round(sum(x.kernel_ms[i] for x in outs) / (dt * 1000.0), 4)
-                    for i, name in enumerate(["source", "flight", "photo", "compton", "rayleigh", "end_batch"])}
+                    for i, name in enumerate(["source", "flight", "photo", "compton", "rayleigh", "end_batch", "other",
+                                              "interact"])}
     line = {
         "metric": METRIC, "value": value, "unit": "histories/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt * 1000.0 / args.steps, "higher_is_better": True,

@@ -564,6 +564,20 @@ __global__ void k_refill_commit(Launch L, uint32_t limit) {
   }
 }
 
+// k_refill_commit and the round's counter reset (k_round_reset) in one launch: streamed runs
+// reset the interaction queues and the next list after the refill, which uses neither.
+__global__ void k_stream_commit(Launch L, uint32_t limit, int next_slot) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const MultiScene& M = L.MS;
+    const uint32_t n_new = refill_count(M, limit);
+    *M.free_cnt -= n_new;
+    *M.cursor += n_new;
+    uint32_t* const cnt = L.cnt;
+    cnt[C_QP] = 0; cnt[C_QC] = 0; cnt[C_QR] = 0; cnt[C_IMP] = 0; cnt[C_COLL] = 0; cnt[C_RESV] = 0;
+    cnt[next_slot] = 0;
+  }
+}
+
 // Starts the next histories of a streamed run in free pool slots: history h (counted from
 // the run's first) draws from the stream keyed by (seed, first + h), exactly as in the
 // source kernel of a batch-by-batch run, and goes to the tally slot of its batch.
@@ -1016,12 +1030,114 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 // ---------------------------------------------------------------------------
 // interaction kernels
 // ---------------------------------------------------------------------------
+// One queued interaction each: the photon's record is read whole, the deposit (keV, weighted)
+// and its material and slot or scene returned, the record updated.
+// Photoelectric absorption, with K-shell fluorescence when enabled; emit = the photon flies on.
+__device__ __forceinline__ void photo_one(const Launch& L, const PState& S, int i, int& m, int& sc, double& dep,
+                                          bool& emit) {
+  const Tables& T = L.T;
+  const PhotonWords P = load_photon(S, i);
+  m = (int)P.u(W_MAT);
+  sc = (int)P.u(W_SC);
+  const float E = P.f(W_E);
+  const float w = P.f(W_WT);
+  PhiloxRng rng = photon_rng(L, P);
+  float eline = 0.f;
+  emit = false;
+  if (L.fluor) {
+    const EPos ep = energy_pos(T, E);
+    int el;
+    emit = fluor_sample(T, m, E, ep, rng, el, eline);
+  }
+  dep = (double)w * (double)(emit ? E - eline : E);
+  S.nsc[i] = (int)P.u(W_NSC) + 1;
+  if (emit && L.Tr.on) {
+    const int k = L.Tr.last[i];
+    if (k >= 0 && (uint32_t)k < L.Tr.cap) L.Tr.rec[(size_t)k * 12 + 11] = __int_as_float(CC_EV_FLUORESCENCE);
+  }
+  if (emit) {
+    float d[3];
+    isotropic_dir(d, rng.next(), rng.next());
+    S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
+    S.e[i] = eline;
+    S.tau[i] = -logf(rng.next());
+  } else {
+    S.flags[i] = (int)P.u(W_FLAGS) | F_ABSORBED;
+  }
+  S.ctr[i] = rng.ctr;
+}
+
+// Bound Compton scattering: new energy and direction, the electron's energy deposited.
+__device__ __forceinline__ void compton_one(const Launch& L, const PState& S, int i, int& m, int& sc, double& dep) {
+  const Tables& T = L.T;
+  const PhotonWords P = load_photon(S, i);
+  m = (int)P.u(W_MAT);
+  sc = (int)P.u(W_SC);
+  const float E = P.f(W_E);
+  const float w = P.f(W_WT);
+  PhiloxRng rng = photon_rng(L, P);
+  const EPos ep = energy_pos(T, E);
+  float cs, sn;
+  int el, trials;
+  const float eps = compton_sample(T, m, E, ep, rng, kTrialCap, cs, sn, el, trials);
+  if (trials == 0) raise_err(L.err, CC_DEV_ITER_CAP);
+  const float enew = E * eps;
+  dep = (double)w * ((double)E - (double)enew);
+  float d[3] = {P.f(W_DX), P.f(W_DY), P.f(W_DZ)};
+  rotate_dir(d, cs, sn, rng.next());
+  S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
+  S.e[i] = enew;
+  S.tau[i] = -logf(rng.next());
+  S.nsc[i] = (int)P.u(W_NSC) + 1;
+  S.ctr[i] = rng.ctr;
+}
+
+// Rayleigh scattering: new direction, no deposit.
+__device__ __forceinline__ void rayleigh_one(const Launch& L, const PState& S, int i) {
+  const Tables& T = L.T;
+  const PhotonWords P = load_photon(S, i);
+  const int m = (int)P.u(W_MAT);
+  const float E = P.f(W_E);
+  PhiloxRng rng = photon_rng(L, P);
+  const EPos ep = energy_pos(T, E);
+  int el, trials;
+  bool qerr;
+  float sn;
+  const float cs = rayleigh_sample(T, m, E, ep, rng, kTrialCap, sn, el, trials, qerr);
+  if (trials == 0) raise_err(L.err, CC_DEV_ITER_CAP);
+  if (qerr) raise_err(L.err, CC_DEV_Q_RANGE);
+  float d[3] = {P.f(W_DX), P.f(W_DY), P.f(W_DZ)};
+  rotate_dir(d, cs, sn, rng.next());
+  S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
+  S.tau[i] = -logf(rng.next());
+  S.nsc[i] = (int)P.u(W_NSC) + 1;
+  S.ctr[i] = rng.ctr;
+}
+
+// The wave and block bookkeeping after a pass of interactions: deposits by material (by
+// slot or scene and material in batched and streamed runs), event counts, survivors to the
+// next list and, when slots are recycled, absorbed photons' slots to the free stack.
+__device__ __forceinline__ void interact_tail(const Launch& L, BlockQueues& bq, bool valid, int type, int i, int m,
+                                              int sc, double dep, bool emit, int* act_next, uint32_t* cnt_next) {
+  const bool dep_on = valid && type <= 1;
+  wave_add_by_key(dep_on, L.multi && dep_on ? sc * CC_MAX_MAT + m : m, dep, dep_copy(L));
+  wave_count(valid && type == 0 && !emit, stat_copy(L) + S_ABS);
+  wave_count(valid && type == 0 && emit, stat_copy(L) + S_FLUOR);
+  wave_count(valid, stat_copy(L) + S_SCAT);
+  const bool fly = valid && (type != 0 || emit), frees = L.multi && valid && type == 0 && !emit;
+  const unsigned long long m0 = bq_count(bq, 0, fly), m1 = bq_count(bq, 1, frees);
+  __syncthreads();
+  if (threadIdx.x < 2) bq_reserve(bq, threadIdx.x, threadIdx.x == 0 ? cnt_next : L.MS.free_cnt);
+  __syncthreads();
+  int k;
+  if ((k = bq_slot(bq, 0, fly, m0)) >= 0) act_next[k] = i;
+  if ((k = bq_slot(bq, 1, frees, m1)) >= 0) L.MS.free_stack[k] = i;
+}
+
 __global__ __launch_bounds__(kBlock) void k_photo(Launch L, const int* q, const uint32_t* cnt_q, int* act_next,
                                                   uint32_t* cnt_next) {
   __shared__ BlockQueues bq;
   const uint32_t n = *cnt_q;
-  const Tables& T = L.T;
-  PState S = L.S;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t j = base + threadIdx.x;
     const bool valid = j < n;
@@ -1030,48 +1146,9 @@ __global__ __launch_bounds__(kBlock) void k_photo(Launch L, const int* q, const 
     double dep = 0.0;
     if (valid) {
       i = q[j];
-      const PhotonWords P = load_photon(S, i);
-      m = (int)P.u(W_MAT);
-      sc = (int)P.u(W_SC);
-      const float E = P.f(W_E);
-      const float w = P.f(W_WT);
-      PhiloxRng rng = photon_rng(L, P);
-      float eline = 0.f;
-      if (L.fluor) {
-        const EPos ep = energy_pos(T, E);
-        int el;
-        emit = fluor_sample(T, m, E, ep, rng, el, eline);
-      }
-      dep = (double)w * (double)(emit ? E - eline : E);
-      S.nsc[i] = (int)P.u(W_NSC) + 1;
-      if (emit && L.Tr.on) {
-        const int k = L.Tr.last[i];
-        if (k >= 0 && (uint32_t)k < L.Tr.cap) L.Tr.rec[(size_t)k * 12 + 11] = __int_as_float(CC_EV_FLUORESCENCE);
-      }
-      if (emit) {
-        float d[3];
-        isotropic_dir(d, rng.next(), rng.next());
-        S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
-        S.e[i] = eline;
-        S.tau[i] = -logf(rng.next());
-      } else {
-        S.flags[i] = (int)P.u(W_FLAGS) | F_ABSORBED;
-      }
-      S.ctr[i] = rng.ctr;
+      photo_one(L, L.S, i, m, sc, dep, emit);
     }
-    wave_add_by_key(valid, L.multi && valid ? sc * CC_MAX_MAT + m : m, dep, dep_copy(L));
-    wave_count(valid && !emit, stat_copy(L) + S_ABS);
-    wave_count(valid && emit, stat_copy(L) + S_FLUOR);
-    wave_count(valid, stat_copy(L) + S_SCAT);
-    // fluorescence photons fly on; absorbed photons free their slot when slots are recycled
-    const bool fly = valid && emit, frees = L.multi && valid && !emit;
-    const unsigned long long m0 = bq_count(bq, 0, fly), m1 = bq_count(bq, 1, frees);
-    __syncthreads();
-    if (threadIdx.x < 2) bq_reserve(bq, threadIdx.x, threadIdx.x == 0 ? cnt_next : L.MS.free_cnt);
-    __syncthreads();
-    int k;
-    if ((k = bq_slot(bq, 0, fly, m0)) >= 0) act_next[k] = i;
-    if ((k = bq_slot(bq, 1, frees, m1)) >= 0) L.MS.free_stack[k] = i;
+    interact_tail(L, bq, valid, 0, i, m, sc, dep, emit, act_next, cnt_next);
   }
 }
 
@@ -1079,8 +1156,6 @@ __global__ __launch_bounds__(kBlock) void k_compton(Launch L, const int* q, cons
                                                     uint32_t* cnt_next) {
   __shared__ BlockQueues bq;
   const uint32_t n = *cnt_q;
-  const Tables& T = L.T;
-  PState S = L.S;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t j = base + threadIdx.x;
     const bool valid = j < n;
@@ -1088,31 +1163,9 @@ __global__ __launch_bounds__(kBlock) void k_compton(Launch L, const int* q, cons
     double dep = 0.0;
     if (valid) {
       i = q[j];
-      const PhotonWords P = load_photon(S, i);
-      m = (int)P.u(W_MAT);
-      sc = (int)P.u(W_SC);
-      const float E = P.f(W_E);
-      const float w = P.f(W_WT);
-      PhiloxRng rng = photon_rng(L, P);
-      const EPos ep = energy_pos(T, E);
-      float cs, sn;
-      int el, trials;
-      const float eps = compton_sample(T, m, E, ep, rng, kTrialCap, cs, sn, el, trials);
-      if (trials == 0) raise_err(L.err, CC_DEV_ITER_CAP);
-      const float enew = E * eps;
-      dep = (double)w * ((double)E - (double)enew);
-      float d[3] = {P.f(W_DX), P.f(W_DY), P.f(W_DZ)};
-      rotate_dir(d, cs, sn, rng.next());
-      S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
-      S.e[i] = enew;
-      S.tau[i] = -logf(rng.next());
-      S.nsc[i] = (int)P.u(W_NSC) + 1;
-      S.ctr[i] = rng.ctr;
+      compton_one(L, L.S, i, m, sc, dep);
     }
-    wave_add_by_key(valid, L.multi && valid ? sc * CC_MAX_MAT + m : m, dep, dep_copy(L));
-    wave_count(valid, stat_copy(L) + S_SCAT);
-    const int slot = block_append1(bq, valid, cnt_next);
-    if (slot >= 0) act_next[slot] = i;
+    interact_tail(L, bq, valid, 1, i, m, sc, dep, false, act_next, cnt_next);
   }
 }
 
@@ -1120,35 +1173,44 @@ __global__ __launch_bounds__(kBlock) void k_rayleigh(Launch L, const int* q, con
                                                      uint32_t* cnt_next) {
   __shared__ BlockQueues bq;
   const uint32_t n = *cnt_q;
-  const Tables& T = L.T;
-  PState S = L.S;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t j = base + threadIdx.x;
     const bool valid = j < n;
     int i = 0;
     if (valid) {
       i = q[j];
-      const PhotonWords P = load_photon(S, i);
-      const int m = (int)P.u(W_MAT);
-      const float E = P.f(W_E);
-      PhiloxRng rng = photon_rng(L, P);
-      const EPos ep = energy_pos(T, E);
-      int el, trials;
-      bool qerr;
-      float sn;
-      const float cs = rayleigh_sample(T, m, E, ep, rng, kTrialCap, sn, el, trials, qerr);
-      if (trials == 0) raise_err(L.err, CC_DEV_ITER_CAP);
-      if (qerr) raise_err(L.err, CC_DEV_Q_RANGE);
-      float d[3] = {P.f(W_DX), P.f(W_DY), P.f(W_DZ)};
-      rotate_dir(d, cs, sn, rng.next());
-      S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
-      S.tau[i] = -logf(rng.next());
-      S.nsc[i] = (int)P.u(W_NSC) + 1;
-      S.ctr[i] = rng.ctr;
+      rayleigh_one(L, L.S, i);
     }
-    wave_count(valid, stat_copy(L) + S_SCAT);
-    const int slot = block_append1(bq, valid, cnt_next);
-    if (slot >= 0) act_next[slot] = i;
+    interact_tail(L, bq, valid, 2, i, 0, 0, 0.0, false, act_next, cnt_next);
+  }
+}
+
+// The three interaction queues of a round in one launch (streamed runs): entry j of the
+// concatenated queues [photoelectric | Compton | Rayleigh] is thread j's; the work per photon
+// is that of k_photo, k_compton and k_rayleigh, without two of their three launches.
+__global__ __launch_bounds__(kBlock) void k_interact(Launch L, const int* qp, const int* qc, const int* qr,
+                                                     int* act_next, uint32_t* cnt_next) {
+  __shared__ BlockQueues bq;
+  const uint32_t np = L.cnt[C_QP], nc = L.cnt[C_QC], nr = L.cnt[C_QR];
+  const uint32_t n = np + nc + nr;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t j = base + threadIdx.x;
+    const bool valid = j < n;
+    const int type = !valid ? -1 : (j < np ? 0 : (j < np + nc ? 1 : 2));
+    int i = 0, m = 0, sc = 0;
+    bool emit = false;
+    double dep = 0.0;
+    if (type == 0) {
+      i = qp[j];
+      photo_one(L, L.S, i, m, sc, dep, emit);
+    } else if (type == 1) {
+      i = qc[j - np];
+      compton_one(L, L.S, i, m, sc, dep);
+    } else if (type == 2) {
+      i = qr[j - np - nc];
+      rayleigh_one(L, L.S, i);
+    }
+    interact_tail(L, bq, valid, type, i, m, sc, dep, emit, act_next, cnt_next);
   }
 }
 
@@ -2284,7 +2346,8 @@ struct Timer {
   }
 };
 
-enum { K_SOURCE = 0, K_FLIGHT = 1, K_PHOTO = 2, K_COMPTON = 3, K_RAYLEIGH = 4, K_END = 5, K_OTHER = 6 };
+enum { K_SOURCE = 0, K_FLIGHT = 1, K_PHOTO = 2, K_COMPTON = 3, K_RAYLEIGH = 4, K_END = 5, K_OTHER = 6,
+       K_INTERACT = 7 };
 
 // Per-round extras of the learned-importance runs.
 struct RoundOpts {
@@ -2904,26 +2967,17 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
     uint32_t rounds = 0;
     for (;;) {
       for (int k = 0; k < kRoundChunk; ++k) {
-        hipLaunchKernelGGL(k_round_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, nxt);
         tm.begin(K_SOURCE);
         hipLaunchKernelGGL(k_refill_stream, dim3(nbk), dim3(kBlock), 0, c->stream, L, s, act_cur, c->cnt + cur, pool);
-        hipLaunchKernelGGL(k_refill_commit, dim3(1), dim3(64), 0, c->stream, L, pool);
+        hipLaunchKernelGGL(k_stream_commit, dim3(1), dim3(64), 0, c->stream, L, pool, nxt);
         tm.end();
         tm.begin(K_FLIGHT);
         hipLaunchKernelGGL((k_flight<false, false, false, true>), dim3(nbk), dim3(kBlock), 0, c->stream, L,
                            (const int*)act_cur, (const uint32_t*)(c->cnt + cur), c->qp, c->qc, c->qr, (int*)nullptr);
         tm.end();
-        tm.begin(K_PHOTO);
-        hipLaunchKernelGGL(k_photo, dim3(nbk), dim3(kBlock), 0, c->stream, L, c->qp, c->cnt + C_QP, act_nxt,
-                           c->cnt + nxt);
-        tm.end();
-        tm.begin(K_COMPTON);
-        hipLaunchKernelGGL(k_compton, dim3(nbk), dim3(kBlock), 0, c->stream, L, c->qc, c->cnt + C_QC, act_nxt,
-                           c->cnt + nxt);
-        tm.end();
-        tm.begin(K_RAYLEIGH);
-        hipLaunchKernelGGL(k_rayleigh, dim3(nbk), dim3(kBlock), 0, c->stream, L, c->qr, c->cnt + C_QR, act_nxt,
-                           c->cnt + nxt);
+        tm.begin(K_INTERACT);
+        hipLaunchKernelGGL(k_interact, dim3(nbk), dim3(kBlock), 0, c->stream, L, (const int*)c->qp,
+                           (const int*)c->qc, (const int*)c->qr, act_nxt, c->cnt + nxt);
         tm.end();
         std::swap(cur, nxt);
         std::swap(act_cur, act_nxt);

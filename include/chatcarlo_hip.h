@@ -122,7 +122,8 @@ typedef struct {
   uint32_t error_bits, rounds_max;
   double edep_keV[CC_MAX_MAT];  /* collision estimator per material */
   double edep_sum2[CC_MAX_MAT]; /* sum over batches of S_b^2 / n_b */
-  double kernel_ms[8];          /* source, flight, photo, compton, rayleigh, end_batch, other, spare */
+  double kernel_ms[8];          /* source, flight, photo, compton, rayleigh, end_batch, other, interact
+                                   (photo + Compton + Rayleigh in one launch: streamed runs) */
   uint64_t kernel_launches[8];  /* launches timed per kernel kind */
   /* region tally (CC_RUN_REGION), pSv per history summed: per-history sum and sum of squares,
    * and the batch form sum over batches of S_b^2 / n_b */
