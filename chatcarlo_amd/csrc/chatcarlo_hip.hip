@@ -555,18 +555,10 @@ __global__ __launch_bounds__(kBlock) void k_refill(Launch L, int* act, uint32_t*
 
 // Advances the queue past the histories k_refill started (same count: nothing
 // else touches the counters between the two launches).
-__global__ void k_refill_commit(Launch L, uint32_t limit) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const MultiScene& M = L.MS;
-    const uint32_t n_new = refill_count(M, limit);
-    *M.free_cnt -= n_new;
-    *M.cursor += n_new;
-  }
-}
-
-// k_refill_commit and the round's counter reset (k_round_reset) in one launch: streamed runs
-// reset the interaction queues and the next list after the refill, which uses neither.
-__global__ void k_stream_commit(Launch L, uint32_t limit, int next_slot) {
+// After a refill (k_refill, k_refill_stream): the free-slot stack is popped and the history
+// cursor advanced by the photons started; and the round's counters are reset (the interaction
+// queues and the next list), which the refill does not use.
+__global__ void k_commit_and_reset(Launch L, uint32_t limit, int next_slot) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     const MultiScene& M = L.MS;
     const uint32_t n_new = refill_count(M, limit);
@@ -1185,7 +1177,7 @@ __global__ __launch_bounds__(kBlock) void k_rayleigh(Launch L, const int* q, con
   }
 }
 
-// The three interaction queues of a round in one launch (streamed runs): entry j of the
+// The three interaction queues of a round in one launch (streamed and batched runs): entry j of the
 // concatenated queues [photoelectric | Compton | Rayleigh] is thread j's; the work per photon
 // is that of k_photo, k_compton and k_rayleigh, without two of their three launches.
 __global__ __launch_bounds__(kBlock) void k_interact(Launch L, const int* qp, const int* qc, const int* qr,
@@ -2969,7 +2961,7 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
       for (int k = 0; k < kRoundChunk; ++k) {
         tm.begin(K_SOURCE);
         hipLaunchKernelGGL(k_refill_stream, dim3(nbk), dim3(kBlock), 0, c->stream, L, s, act_cur, c->cnt + cur, pool);
-        hipLaunchKernelGGL(k_stream_commit, dim3(1), dim3(64), 0, c->stream, L, pool, nxt);
+        hipLaunchKernelGGL(k_commit_and_reset, dim3(1), dim3(64), 0, c->stream, L, pool, nxt);
         tm.end();
         tm.begin(K_FLIGHT);
         hipLaunchKernelGGL((k_flight<false, false, false, true>), dim3(nbk), dim3(kBlock), 0, c->stream, L,
@@ -3954,14 +3946,12 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
     const int cur = half == 0 ? C_ACT_A : C_ACT_B, nxt = half == 0 ? C_ACT_B : C_ACT_A;
     int* act_cur = half == 0 ? c->act0 : c->act1;
     int* act_nxt = half == 0 ? c->act1 : c->act0;
-    hipLaunchKernelGGL(k_round_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, nxt);
     hipLaunchKernelGGL(k_refill, dim3(nb), dim3(bs), 0, c->stream, L, act_cur, c->cnt + cur, pool);
-    hipLaunchKernelGGL(k_refill_commit, dim3(1), dim3(64), 0, c->stream, L, pool);
+    hipLaunchKernelGGL(k_commit_and_reset, dim3(1), dim3(64), 0, c->stream, L, pool, nxt);
     hipLaunchKernelGGL((k_flight<false, true>), dim3(nb), dim3(bs), 0, c->stream, L, (const int*)act_cur,
                        (const uint32_t*)(c->cnt + cur), c->qp, c->qc, c->qr, (int*)nullptr);
-    hipLaunchKernelGGL(k_photo, dim3(nb), dim3(bs), 0, c->stream, L, c->qp, c->cnt + C_QP, act_nxt, c->cnt + nxt);
-    hipLaunchKernelGGL(k_compton, dim3(nb), dim3(bs), 0, c->stream, L, c->qc, c->cnt + C_QC, act_nxt, c->cnt + nxt);
-    hipLaunchKernelGGL(k_rayleigh, dim3(nb), dim3(bs), 0, c->stream, L, c->qr, c->cnt + C_QR, act_nxt, c->cnt + nxt);
+    hipLaunchKernelGGL(k_interact, dim3(nb), dim3(bs), 0, c->stream, L, (const int*)c->qp, (const int*)c->qc,
+                       (const int*)c->qr, act_nxt, c->cnt + nxt);
   }
   hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(64), 0, c->stream, L, (const uint32_t*)(c->cnt + C_ACT_A),
                      c->b_status);
