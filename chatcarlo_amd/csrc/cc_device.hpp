@@ -159,7 +159,7 @@ struct BufRng {
 // geometry (chatcarlo/geometry.py:20-231)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void slab(const float* lo, const float* hi, const float o[3],
-                                     const float d[3], float& ten, float& tex, bool& hit) {
+                                     const float d[3], const float inv_d[3], float& ten, float& tex, bool& hit) {
   ten = -INFINITY;
   tex = INFINITY;
   bool miss = false;
@@ -168,8 +168,8 @@ __device__ __forceinline__ void slab(const float* lo, const float* hi, const flo
     if (fabsf(d[k]) < 1e-12f) {
       miss |= (o[k] < lo[k]) || (o[k] > hi[k]);
     } else {
-      float ta = (lo[k] - o[k]) / d[k];
-      float tb = (hi[k] - o[k]) / d[k];
+      float ta = (lo[k] - o[k]) * inv_d[k];
+      float tb = (hi[k] - o[k]) * inv_d[k];
       ten = fmaxf(ten, fminf(ta, tb));
       tex = fminf(tex, fmaxf(ta, tb));
     }
@@ -261,11 +261,11 @@ __device__ __forceinline__ bool isect_cylinder(const Prim& P, const float o[3], 
   return !(t1 > t2);
 }
 
-__device__ __forceinline__ bool isect_prim(const Prim& P, const float o[3], const float d[3],
+__device__ __forceinline__ bool isect_prim(const Prim& P, const float o[3], const float d[3], const float inv_d[3],
                                            float& t1, float& t2) {
   if (P.kind == CC_PRIM_BOX) {
     bool hit;
-    slab(&P.p[0], &P.p[3], o, d, t1, t2, hit);
+    slab(&P.p[0], &P.p[3], o, d, inv_d, t1, t2, hit);
     return hit;
   }
   if (P.kind == CC_PRIM_SPHERE) return isect_sphere(P, o, d, t1, t2);
@@ -310,17 +310,20 @@ __device__ __forceinline__ int locate(const Prim* prims, int n_prim, int backgro
 __device__ __forceinline__ float next_boundary(const Prim* prims, int n_prim, const World& W,
                                                const float o[3], const float d[3], float tmin,
                                                bool& escape) {
+  // one reciprocal per axis and call, not two divisions per axis and box; every box root is
+  // formed the same way on every call, so a crossed surface keeps the root it had
+  const float inv_d[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
   float t_obj = INFINITY;
   for (int i = 0; i < n_prim; ++i) {
     float t1, t2;
-    if (isect_prim(prims[i], o, d, t1, t2)) {
+    if (isect_prim(prims[i], o, d, inv_d, t1, t2)) {
       if (t1 > tmin && t1 < t_obj) t_obj = t1;
       if (t2 > tmin && t2 < t_obj) t_obj = t2;
     }
   }
   float ten, tex;
   bool hit;
-  slab(W.lo, W.hi, o, d, ten, tex, hit);
+  slab(W.lo, W.hi, o, d, inv_d, ten, tex, hit);
   if (!hit) tex = INFINITY;
   escape = tex <= t_obj;
   return fminf(t_obj, tex);
