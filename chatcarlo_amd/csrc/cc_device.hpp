@@ -587,8 +587,8 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v) {
 
 // Block-private cache of tally contributions in LDS: a direct-mapped table of
 // (voxel, kerma, H*(10)) sums.  The photons of one block share the hot voxels of a beam
-// across waves and steps, which the wave-level aggregation cannot see; adds to a voxel
-// that owns its slot cost LDS atomics instead of L2 atomics.  A voxel whose slot another
+// across lanes, waves and steps; adds to a voxel that owns its slot cost LDS atomics
+// instead of atomics beyond the L2.  A voxel whose slot another
 // voxel owns goes straight to memory; the table is added into the grid once, when the
 // block ends.  (Probing further slots was measured slower: when the table misses, the
 // voxels are not shared anyway.)

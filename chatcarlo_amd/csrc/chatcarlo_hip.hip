@@ -6,9 +6,10 @@
 // until it interacts or escapes (scoring the track-length tallies with an
 // exact DDA voxel walk and FP64 atomics) and appends it to one of three
 // interaction queues; photoelectric(+K fluorescence), bound-Compton and
-// Rayleigh kernels then drain their own queue and append survivors to the
-// next round's active list.  Queue appends and per-material energy deposits
-// are wavefront-aggregated (one atomic per wave and key).
+// Rayleigh work then drains the queues (one kernel per queue, or one for all
+// three) and appends survivors to the next round's active list.  Queue appends
+// take one atomic per block and queue; per-material energy deposits one per
+// wave and key, into per-block copies of the sums.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -2063,7 +2064,7 @@ __global__ void k_test_iso(int n, const float* u1, const float* u2, float* out) 
 
 __global__ void k_test_dda(GridDev G, int n, const float* o, const float* d, const float* len, const double* wk,
                            const double* wh) {
-  // every lane takes part in the wave-aggregated tally; lanes past n score nothing
+  // every lane takes part in the tally's lane-pair exchange; lanes past n score nothing
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n;
   float oo[3] = {0.f, 0.f, 0.f}, dd[3] = {0.f, 0.f, 1.f};
