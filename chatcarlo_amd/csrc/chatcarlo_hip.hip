@@ -218,10 +218,6 @@ struct Launch {
   Stream St;
 };
 
-// Philox key of a photon's history: the run seed, or its scene's seed in a batched run.
-__device__ __forceinline__ uint64_t photon_seed(const Launch& L, const PState& S, int i) {
-  return L.multi ? L.MS.seeds[S.sc[i]] : L.seed;
-}
 
 // Photon i's 64-byte record as four 16-byte words (the pool is 256-byte aligned):
 // [0] x y z dx, [1] dy dz e tau, [2] wt ctr hid tag, [3] nsc mat flags sc.
@@ -243,7 +239,8 @@ __device__ __forceinline__ PhotonWords load_photon(const PState& S, int i) {
   const uint4* r = photon_record(S, i);
   return PhotonWords{{r[0], r[1], r[2], r[3]}};
 }
-// The Philox key and stream of a loaded photon (photon_seed for a record in registers).
+// The Philox stream of a loaded photon: keyed by the run's seed, or in a batched run by its
+// scene's; counter (history, tag, event counter).
 __device__ __forceinline__ PhiloxRng photon_rng(const Launch& L, const PhotonWords& P) {
   return PhiloxRng(L.multi ? L.MS.seeds[P.u(W_SC)] : L.seed, L.hist0 + (uint64_t)P.u(W_HID), P.u(W_TAG), P.u(W_CTR));
 }
