@@ -218,7 +218,6 @@ struct Launch {
   Stream St;
 };
 
-
 // Photon i's 64-byte record as four 16-byte words (the pool is 256-byte aligned):
 // [0] x y z dx, [1] dy dz e tau, [2] wt ctr hid tag, [3] nsc mat flags sc.
 __device__ __forceinline__ const uint4* photon_record(const PState& S, int i) {
