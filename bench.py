@@ -107,9 +107,12 @@ def cpu_baseline(raw, all_cores):
     return out
 
 
-def learned_fom(eng, target=0.02):
+def learned_fom(eng, target=0.02, pairs=5):
     """Angio room, operator sphere: contribution records from analog GPU runs, the MLP trained on
-    the GPU, then analog and learned runs to the same relative error of the operator H*(10)."""
+    the GPU, then analog and learned runs to the same relative error of the operator H*(10), for
+    `pairs` seed pairs.  The learned run's weights are heavy-tailed, so the relative error of one
+    run, and the FOM ratio with it, varies by a factor of a few between seeds; the ratio reported
+    is the median over the pairs (each pair listed), the run details those of the median pair."""
     from chatcarlo_amd.importance import LearnedImportance
     from chatcarlo_amd.workloads import angio_room, angio_region
     raw, _ = angio_room()
@@ -121,14 +124,20 @@ def learned_fom(eng, target=0.02):
     tr = li.train(0, steps=12000, batch=8192, lr=3e-3, lr_end=1e-4)
     t2 = time.perf_counter()
     li.configure()
-    ana, lrn, chosen, z = li.compare(target_rel_err=target)
+    runs = []
+    for k in range(pairs):
+        ana, lrn, chosen, z = li.compare(target_rel_err=target, seed_analog=101 + k, seed_learned=202 + k)
+        runs.append((lrn.fom / ana.fom, k, ana, lrn, chosen, z))
+    ratio, k, ana, lrn, chosen, z = sorted(runs, key=lambda x: x[0])[pairs // 2]
     return {"workload": "angio_room_90kVp_operator_sphere_r10cm", "target_rel_err": target,
+            "seed_pairs": [[101 + j, 202 + j] for j in range(pairs)], "median_pair": k,
+            "fom_ratio_per_pair": [x[0] for x in runs], "z_per_pair": [x[5] for x in runs],
             "analog": {"histories": ana.n_histories, "rel_err": ana.rel_err, "seconds": ana.seconds,
                        "fom": ana.fom, "h10_pSv_per_history": ana.mean},
             "learned": {"histories": lrn.n_histories, "rel_err": lrn.rel_err, "seconds": lrn.seconds,
                         "fom": lrn.fom, "h10_pSv_per_history": lrn.mean, "splits": lrn.n_split,
                         "roulette_kills": lrn.n_roulette},
-            "fom_ratio": lrn.fom / ana.fom, "z_learned_vs_analog": z, "learned_chosen": chosen is lrn,
+            "fom_ratio": ratio, "z_learned_vs_analog": z, "learned_chosen": chosen is lrn,
             "fom_ratio_with_training": (1.0 / (lrn.rel_err ** 2 * (lrn.seconds + (t1 - t0) + (t2 - t1)))) / ana.fom,
             "records": n_rec, "record_seconds": t1 - t0, "train_seconds": t2 - t1,
             "train_device_ms": tr.ms, "train_loss_first_last": [tr.loss_first, tr.loss_last]}

@@ -2767,10 +2767,19 @@ int cc_atomic_rate(void* ctx, uint64_t n_slots, uint64_t n_ops, double* ops_per_
   const uint64_t per = n_ops / threads;
   if (per < 1 || per > 4096) return fail(CC_ERR_ARG, "n_ops must be 1..4096 times %llu", (unsigned long long)threads);
   HIP_TRY(hipSetDevice(c->device));
+  DevBufs B;   // the array and the two events are released on every return path
+  struct Events {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Events() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } ev;
   double* a = nullptr;
-  int r = dalloc(&a, (size_t)n_slots);
+  int r = B.out((size_t)n_slots, &a);
   if (r) return r;
-  hipEvent_t e0, e1;
+  hipEvent_t& e0 = ev.e0;
+  hipEvent_t& e1 = ev.e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks((size_t)n_slots)), dim3(kBlock), 0, c->stream, a, (size_t)n_slots, 0.0);
@@ -2790,9 +2799,6 @@ int cc_atomic_rate(void* ctx, uint64_t n_slots, uint64_t n_ops, double* ops_per_
     for (double x : h) t += x;
     *sum = t;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(a);
   return CC_OK;
 }
 
