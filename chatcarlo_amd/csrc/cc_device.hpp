@@ -146,6 +146,32 @@ struct PhiloxRng {
   }
 };
 
+// The same stream as PhiloxRng, with the block's four uniforms in four registers picked by
+// selects.  In the interaction kernels an indexed buffer lands in scratch memory, every draw
+// then waiting on a scratch load; with the selects k_interact needs no scratch at five waves
+// per SIMD.  (Where the buffer is promoted to LDS instead, as in the flight and splitting
+// kernels, PhiloxRng is as fast or faster.)
+struct PhiloxRegRng {
+  uint2 key;
+  uint32_t h_lo, h_hi, tag, ctr;
+  float b0, b1, b2, b3;
+  int pos;
+  __device__ PhiloxRegRng(uint64_t seed, uint64_t hist, uint32_t tag_, uint32_t ctr_)
+      : key(make_uint2((uint32_t)seed, (uint32_t)(seed >> 32))), h_lo((uint32_t)hist),
+        h_hi((uint32_t)(hist >> 32)), tag(tag_), ctr(ctr_), b0(0.f), b1(0.f), b2(0.f), b3(0.f), pos(4) {}
+  __device__ __forceinline__ float next() {
+    if (pos == 4) {
+      uint4 r = philox4x32_10(make_uint4(h_lo, h_hi, ctr, tag), key);
+      ++ctr;
+      b0 = u01(r.x); b1 = u01(r.y); b2 = u01(r.z); b3 = u01(r.w);
+      pos = 0;
+    }
+    const float v = pos == 0 ? b0 : (pos == 1 ? b1 : (pos == 2 ? b2 : b3));
+    ++pos;
+    return v;
+  }
+};
+
 // Caller-supplied uniforms (sub-kernel parity tests).
 struct BufRng {
   const float* u;

@@ -240,8 +240,8 @@ __device__ __forceinline__ PhotonWords load_photon(const PState& S, int i) {
 }
 // The Philox stream of a loaded photon: keyed by the run's seed, or in a batched run by its
 // scene's; counter (history, tag, event counter).
-__device__ __forceinline__ PhiloxRng photon_rng(const Launch& L, const PhotonWords& P) {
-  return PhiloxRng(L.multi ? L.MS.seeds[P.u(W_SC)] : L.seed, L.hist0 + (uint64_t)P.u(W_HID), P.u(W_TAG), P.u(W_CTR));
+__device__ __forceinline__ PhiloxRegRng photon_rng(const Launch& L, const PhotonWords& P) {
+  return PhiloxRegRng(L.multi ? L.MS.seeds[P.u(W_SC)] : L.seed, L.hist0 + (uint64_t)P.u(W_HID), P.u(W_TAG), P.u(W_CTR));
 }
 
 __device__ __forceinline__ uint4* photon_record_w(const PState& S, int i) {
@@ -1030,7 +1030,7 @@ __device__ __forceinline__ void photo_one(const Launch& L, const PState& S, int 
   sc = (int)P.u(W_SC);
   const float E = P.f(W_E);
   const float w = P.f(W_WT);
-  PhiloxRng rng = photon_rng(L, P);
+  PhiloxRegRng rng = photon_rng(L, P);
   float eline = 0.f;
   emit = false;
   if (L.fluor) {
@@ -1039,21 +1039,20 @@ __device__ __forceinline__ void photo_one(const Launch& L, const PState& S, int 
     emit = fluor_sample(T, m, E, ep, rng, el, eline);
   }
   dep = (double)w * (double)(emit ? E - eline : E);
-  S.nsc[i] = (int)P.u(W_NSC) + 1;
   if (emit && L.Tr.on) {
     const int k = L.Tr.last[i];
     if (k >= 0 && (uint32_t)k < L.Tr.cap) L.Tr.rec[(size_t)k * 12 + 11] = __int_as_float(CC_EV_FLUORESCENCE);
   }
+  uint4* const rw = photon_record_w(S, i);
   if (emit) {
     float d[3];
     isotropic_dir(d, rng.next(), rng.next());
-    S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
-    S.e[i] = eline;
-    S.tau[i] = -logf(rng.next());
-  } else {
-    S.flags[i] = (int)P.u(W_FLAGS) | F_ABSORBED;
+    const float tau_new = -logf(rng.next());
+    rw[0] = make_uint4(P.q[0].x, P.q[0].y, P.q[0].z, __float_as_uint(d[0]));
+    rw[1] = make_uint4(__float_as_uint(d[1]), __float_as_uint(d[2]), __float_as_uint(eline), __float_as_uint(tau_new));
   }
-  S.ctr[i] = rng.ctr;
+  rw[2] = make_uint4(P.q[2].x, rng.ctr, P.q[2].z, P.q[2].w);
+  rw[3] = make_uint4(P.q[3].x + 1u, P.q[3].y, emit ? P.q[3].z : (P.q[3].z | (uint32_t)F_ABSORBED), P.q[3].w);
 }
 
 // Bound Compton scattering: new energy and direction, the electron's energy deposited.
@@ -1064,7 +1063,7 @@ __device__ __forceinline__ void compton_one(const Launch& L, const PState& S, in
   sc = (int)P.u(W_SC);
   const float E = P.f(W_E);
   const float w = P.f(W_WT);
-  PhiloxRng rng = photon_rng(L, P);
+  PhiloxRegRng rng = photon_rng(L, P);
   const EPos ep = energy_pos(T, E);
   float cs, sn;
   int el, trials;
@@ -1074,11 +1073,12 @@ __device__ __forceinline__ void compton_one(const Launch& L, const PState& S, in
   dep = (double)w * ((double)E - (double)enew);
   float d[3] = {P.f(W_DX), P.f(W_DY), P.f(W_DZ)};
   rotate_dir(d, cs, sn, rng.next());
-  S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
-  S.e[i] = enew;
-  S.tau[i] = -logf(rng.next());
-  S.nsc[i] = (int)P.u(W_NSC) + 1;
-  S.ctr[i] = rng.ctr;
+  const float tau_new = -logf(rng.next());
+  uint4* const rw = photon_record_w(S, i);
+  rw[0] = make_uint4(P.q[0].x, P.q[0].y, P.q[0].z, __float_as_uint(d[0]));
+  rw[1] = make_uint4(__float_as_uint(d[1]), __float_as_uint(d[2]), __float_as_uint(enew), __float_as_uint(tau_new));
+  rw[2] = make_uint4(P.q[2].x, rng.ctr, P.q[2].z, P.q[2].w);
+  rw[3] = make_uint4(P.q[3].x + 1u, P.q[3].y, P.q[3].z, P.q[3].w);   // n_scatter + 1
 }
 
 // Rayleigh scattering: new direction, no deposit.
@@ -1087,7 +1087,7 @@ __device__ __forceinline__ void rayleigh_one(const Launch& L, const PState& S, i
   const PhotonWords P = load_photon(S, i);
   const int m = (int)P.u(W_MAT);
   const float E = P.f(W_E);
-  PhiloxRng rng = photon_rng(L, P);
+  PhiloxRegRng rng = photon_rng(L, P);
   const EPos ep = energy_pos(T, E);
   int el, trials;
   bool qerr;
@@ -1097,10 +1097,12 @@ __device__ __forceinline__ void rayleigh_one(const Launch& L, const PState& S, i
   if (qerr) raise_err(L.err, CC_DEV_Q_RANGE);
   float d[3] = {P.f(W_DX), P.f(W_DY), P.f(W_DZ)};
   rotate_dir(d, cs, sn, rng.next());
-  S.dx[i] = d[0]; S.dy[i] = d[1]; S.dz[i] = d[2];
-  S.tau[i] = -logf(rng.next());
-  S.nsc[i] = (int)P.u(W_NSC) + 1;
-  S.ctr[i] = rng.ctr;
+  const float tau_new = -logf(rng.next());
+  uint4* const rw = photon_record_w(S, i);
+  rw[0] = make_uint4(P.q[0].x, P.q[0].y, P.q[0].z, __float_as_uint(d[0]));
+  rw[1] = make_uint4(__float_as_uint(d[1]), __float_as_uint(d[2]), P.q[1].z, __float_as_uint(tau_new));
+  rw[2] = make_uint4(P.q[2].x, rng.ctr, P.q[2].z, P.q[2].w);
+  rw[3] = make_uint4(P.q[3].x + 1u, P.q[3].y, P.q[3].z, P.q[3].w);   // n_scatter + 1
 }
 
 // The wave and block bookkeeping after a pass of interactions: deposits by material (by
@@ -1176,8 +1178,9 @@ __global__ __launch_bounds__(kBlock) void k_rayleigh(Launch L, const int* q, con
 
 // The three interaction queues of a round in one launch (streamed and batched runs): entry j of the
 // concatenated queues [photoelectric | Compton | Rayleigh] is thread j's; the work per photon
-// is that of k_photo, k_compton and k_rayleigh, without two of their three launches.
-__global__ __launch_bounds__(kBlock) void k_interact(Launch L, const int* qp, const int* qc, const int* qr,
+// is that of k_photo, k_compton and k_rayleigh, without two of their three launches.  Its waves
+// mostly wait on table loads, so it runs five waves per SIMD (96 VGPRs).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_interact(Launch L, const int* qp, const int* qc, const int* qr,
                                                      int* act_next, uint32_t* cnt_next) {
   __shared__ BlockQueues bq;
   const uint32_t np = L.cnt[C_QP], nc = L.cnt[C_QC], nr = L.cnt[C_QR];
