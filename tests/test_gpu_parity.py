@@ -191,6 +191,44 @@ def test_rayleigh_parity(engine, dpack):
     assert np.mean(err <= 1e-6) > 0.99
 
 
+def _rayleigh_gpu(engine, dpack, name, e_kev, n, seed):
+    """The device Rayleigh sampler on n photons of one material and energy (16 trials each)."""
+    N, lib = _lib()
+    k = 16
+    rng = np.random.default_rng(seed)
+    mat = np.full(n, dpack.mat_id(name), np.int32)
+    ef = np.full(n, e_kev, np.float32)
+    u = f32(1.0 - rng.random((n, 1 + 2 * k)))
+    cs = np.zeros(n, np.float32); el = np.zeros(n, np.int32); tr = np.zeros(n, np.int32)
+    N.check(lib.cc_test_rayleigh(engine.ctx, n, N.ptr(mat, C.c_int32), N.ptr(ef, C.c_float), N.ptr(u, C.c_float),
+                                 k, N.ptr(cs, C.c_float), N.ptr(el, C.c_int32), N.ptr(tr, C.c_int32)))
+    assert np.all(tr > 0)
+    return cs.astype(np.float64), el
+
+
+def test_rayleigh_keeps_the_reference_invariants(engine, dpack):
+    """The device sampler keeps the reference's Rayleigh invariants (tests/test_rayleigh.py:31-116):
+    forward bias (lead at 60 keV: mean cos > 0.3, Thomson's is 0), lighter elements more forward
+    (aluminium against lead), higher energies more forward (calcium, 150 against 20 keV), cos in
+    [-1, 1] at 150 keV, and bone's picks among bone's elements only, calcium above its mass fraction."""
+    n = 20000
+    pb, _ = _rayleigh_gpu(engine, dpack, "lead", 60.0, n, 1)
+    al, _ = _rayleigh_gpu(engine, dpack, "aluminum", 60.0, n, 2)
+    assert pb.mean() > 0.3
+    assert al.mean() > pb.mean()
+    lo, _ = _rayleigh_gpu(engine, dpack, "calcium", 20.0, n, 3)
+    hi, _ = _rayleigh_gpu(engine, dpack, "calcium", 150.0, n, 4)
+    assert hi.mean() > lo.mean()
+    for k, name in enumerate(("water", "calcium", "lead")):
+        c, _ = _rayleigh_gpu(engine, dpack, name, 150.0, 200_000, 5 + k)
+        assert c.min() >= -1.0 and c.max() <= 1.0
+    _, el = _rayleigh_gpu(engine, dpack, "bone", 50.0, n, 8)
+    m = dpack.mat_id("bone")
+    assert set(np.unique(el).tolist()) <= set(dpack.mat_elem[m])
+    ca = dpack.elements.index(20)
+    assert np.mean(el == ca) > dpack.mat_frac[m][list(dpack.mat_elem[m]).index(ca)]
+
+
 def test_fluorescence_parity(engine, dpack):
     N, lib = _lib()
     n = 50000

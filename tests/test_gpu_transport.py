@@ -173,6 +173,29 @@ def test_run_transport_api(engine):
     assert r3.fraction_absorbed == r1.fraction_absorbed
 
 
+def test_results_do_not_depend_on_n_workers(engine):
+    """The reference's worker tests (tests/test_parallel_transport.py:19-36) on its chest room:
+    the same (seed, n_workers) reproduces, and n_workers = 1 equals the default call.  Here one
+    GPU replaces the worker pool and the stream is keyed by (seed, history), so every worker
+    count gives the same counters, and deposits equal up to FP64 atomic ordering."""
+    from chatcarlo_amd.scene import validate_scene
+    from chatcarlo_amd.transport import run_transport
+    from chatcarlo_amd.workloads import chest_room
+    raw, _ = chest_room()
+    sc = validate_scene(raw)
+    assert sc.ok
+    runs = [run_transport(sc, n_histories=8000, seed=123, n_workers=w) for w in (2, 2, 1, 4)]
+    runs.append(run_transport(sc, n_histories=8000, seed=123))
+    r0 = runs[0]
+    assert r0.fraction_absorbed > 0 and r0.energy_deposited_MeV
+    for r in runs[1:]:
+        assert (r.fraction_absorbed, r.fraction_escaped, r.mean_scatter_events, r.n_fluorescence) == \
+               (r0.fraction_absorbed, r0.fraction_escaped, r0.mean_scatter_events, r0.n_fluorescence)
+        assert r.energy_deposited_MeV.keys() == r0.energy_deposited_MeV.keys()
+        for m, v in r0.energy_deposited_MeV.items():
+            assert abs(r.energy_deposited_MeV[m] - v) <= 1e-12 * abs(v), m
+
+
 def test_grazing_flights_do_not_creep(engine):
     """Photons on a cylinder's side moving almost along its axis, and photons leaving a sphere:
     with boundary roots measured from the flight origin a crossed surface is never found again,

@@ -217,3 +217,37 @@ def test_energy_range_error(pack):
         core.xs_lookup(pack, np.array([0]), np.array([0.5]))
     with pytest.raises(ValueError):
         core.xs_lookup(pack, np.array([0]), np.array([151.0]))
+
+
+# ---- the reference's Rayleigh invariants (tests/test_rayleigh.py), on the pack and the oracle sampler
+
+def test_form_factor_at_zero_momentum_is_z():
+    """F(Z, 0) = Z: every electron scatters in phase (reference tests/test_rayleigh.py:20-23)."""
+    from chatcarlo_amd import atomic as A
+    for z in (1, 8, 20, 82):
+        assert abs(float(A.form_factor(z, np.array([0.0]))[0]) - z) < 1e-6
+
+
+def test_form_factor_decreases_with_momentum(pack):
+    """F(82, q) never rises over the pack's q grid (reference tests/test_rayleigh.py:26-28)."""
+    from chatcarlo_amd import atomic as A
+    q = np.sqrt(pack.ray_x)
+    assert np.all(np.diff(A.form_factor(82, q)) <= 1e-9)
+
+
+def test_rayleigh_element_selection_in_bone(pack):
+    """Bone's Rayleigh element-selection CDF picks only bone's elements, ends at 1 at 30, 60 and
+    100 keV, and picks calcium more often than its mass fraction, the coherent cross section
+    rising steeply with Z (reference tests/test_rayleigh.py:65-78,106-116)."""
+    m = pack.mat_id("bone")
+    ca = list(pack.mat_elem[m]).index(pack.elements.index(20))
+    for e in (30.0, 50.0, 60.0, 100.0):
+        i, f = core.energy_interval(pack, np.array([e]))
+        cdf = core.sel_cdf_at(pack, m, 2, i[0], f[0])
+        assert abs(cdf[-1] - 1.0) < 1e-12 and np.all(np.diff(cdf) >= 0)
+        p_ca = cdf[ca] - (cdf[ca - 1] if ca > 0 else 0.0)
+        assert p_ca > pack.mat_frac[m][ca], (e, p_ca, pack.mat_frac[m][ca])
+    n = 5000
+    i, f = core.energy_interval(pack, np.full(n, 50.0))
+    picked = core.select_elements(pack, np.full(n, m), 2, i, f, RNG.random(n))
+    assert set(np.unique(picked).tolist()) <= set(pack.mat_elem[m])
