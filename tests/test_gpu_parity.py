@@ -229,6 +229,38 @@ def test_rayleigh_keeps_the_reference_invariants(engine, dpack):
     assert np.mean(el == ca) > dpack.mat_frac[m][list(dpack.mat_elem[m]).index(ca)]
 
 
+@pytest.mark.parametrize("name,z,e_kev,n", [("water", 1, 150.0, 400_000), ("water", 8, 80.0, 100_000),
+                                            ("calcium", 20, 60.0, 100_000), ("lead", 82, 20.0, 100_000)])
+def test_rayleigh_fits_the_form_factor_distribution(engine, dpack, name, z, e_kev, n):
+    """The reference's four representative (Z, E) of tests/test_rayleigh_distribution.py:27-53: the
+    device's cos theta of element Z fits p(c) ~ (1 + c^2) F(Z, q)^2 by a one-sample KS test at the
+    reference's Bonferroni level 0.05/4.  Hydrogen and oxygen come from the same water photons,
+    each routed to its own distribution (tests/test_rayleigh_distribution.py:68-87); hydrogen at
+    150 keV, the old sampler's worst case, is accepted within 10 trials (lines 90-106)."""
+    from scipy import stats
+    from chatcarlo_amd import atomic as A
+    N, lib = _lib()
+    k = 16
+    rng = np.random.default_rng(1000 + z)
+    mat = np.full(n, dpack.mat_id(name), np.int32)
+    ef = np.full(n, e_kev, np.float32)
+    u = f32(1.0 - rng.random((n, 1 + 2 * k)))
+    cs = np.zeros(n, np.float32); el = np.zeros(n, np.int32); tr = np.zeros(n, np.int32)
+    N.check(lib.cc_test_rayleigh(engine.ctx, n, N.ptr(mat, C.c_int32), N.ptr(ef, C.c_float), N.ptr(u, C.c_float),
+                                 k, N.ptr(cs, C.c_float), N.ptr(el, C.c_int32), N.ptr(tr, C.c_int32)))
+    sel = el == dpack.elements.index(z)
+    assert sel.sum() > 500
+    assert np.all(tr[sel] > 0) and (z != 1 or tr[sel].max() <= 10)
+    c = np.linspace(-1.0, 1.0, 50001)
+    qg = np.linspace(0.0, 20.0, 4001)
+    f = np.interp(e_kev / A.HC_KEV_ANGSTROM * np.sqrt((1.0 - c) / 2.0), qg, A.form_factor(z, qg))
+    pdf = (1.0 + c ** 2) * f ** 2
+    cdf = np.concatenate([[0.0], np.cumsum(0.5 * (pdf[1:] + pdf[:-1]) * np.diff(c))])
+    cdf /= cdf[-1]
+    ks = stats.kstest(cs[sel].astype(np.float64), lambda x: np.interp(x, c, cdf))
+    assert ks.pvalue >= 0.05 / 4, ks.pvalue
+
+
 def test_fluorescence_parity(engine, dpack):
     N, lib = _lib()
     n = 50000

@@ -212,6 +212,14 @@ def test_edge_jumps_are_kept(pack):
     assert above / below > 3.5
 
 
+def test_lead_k_edge_jump_is_sharp(pack):
+    """Photoelectric jump across lead's K edge, 87.9 against 88.1 keV, above 4 as the reference's
+    tests/test_xs_tables.py:99-112 requires of its tables (the pack gives 4.7)."""
+    m = pack.mat_id("lead")
+    below, above = core.xs_part(pack, np.array([m, m]), np.array([87.9, 88.1]), core.PHOTO)
+    assert above / below > 4.0
+
+
 def test_energy_range_error(pack):
     with pytest.raises(ValueError):
         core.xs_lookup(pack, np.array([0]), np.array([0.5]))
