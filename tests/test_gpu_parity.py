@@ -191,6 +191,34 @@ def test_rayleigh_parity(engine, dpack):
     assert np.mean(err <= 1e-6) > 0.99
 
 
+def test_bound_compton_against_free_electrons(engine, dpack):
+    """Water at 60 keV (reference tests/test_bound_compton.py:45-96): S(q) suppresses small-angle,
+    small-transfer scattering, so the device's bound sampler transfers more energy on average than
+    free-electron Klein-Nishina and scatters further back.  The Klein-Nishina means are integrals
+    of its cross section by quadrature."""
+    N, lib = _lib()
+    n, k = 200_000, 16
+    rng = np.random.default_rng(0)
+    mat = np.full(n, dpack.mat_id("water"), np.int32)
+    ef = np.full(n, 60.0, np.float32)
+    u = f32(1.0 - rng.random((n, 1 + 3 * k)))
+    eps = np.zeros(n, np.float32); cs = np.zeros(n, np.float32)
+    el = np.zeros(n, np.int32); tr = np.zeros(n, np.int32)
+    N.check(lib.cc_test_compton(engine.ctx, n, N.ptr(mat, C.c_int32), N.ptr(ef, C.c_float), N.ptr(u, C.c_float), k,
+                                N.ptr(eps, C.c_float), N.ptr(cs, C.c_float), N.ptr(el, C.c_int32),
+                                N.ptr(tr, C.c_int32)))
+    assert np.all(tr > 0)
+    a = 60.0 / 510.99895
+    c = np.linspace(-1.0, 1.0, 200_001)
+    ek = 1.0 / (1.0 + a * (1.0 - c))
+    w = ek ** 2 * (ek + 1.0 / ek - (1.0 - c ** 2))   # dsigma/dOmega of Klein-Nishina, up to a constant
+    t_free = np.trapezoid((1.0 - ek) * w, c) / np.trapezoid(w, c)
+    cos_free = np.trapezoid(c * w, c) / np.trapezoid(w, c)
+    t_bound = 1.0 - eps.astype(np.float64).mean()
+    assert t_bound > t_free, (t_bound, t_free)
+    assert cs.astype(np.float64).mean() < cos_free
+
+
 def _rayleigh_gpu(engine, dpack, name, e_kev, n, seed):
     """The device Rayleigh sampler on n photons of one material and energy (16 trials each)."""
     N, lib = _lib()

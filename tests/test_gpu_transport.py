@@ -72,6 +72,33 @@ def test_no_fluorescence_below_k_edge(engine):
     assert r.n_fluorescence == 0
 
 
+def test_fluorescence_above_k_edge_and_switched_off(engine):
+    """A 100 keV pencil on 0.5 cm of lead (reference tests/test_fluorescence.py:100-118): K
+    fluorescence photons appear with fluorescence on and none with it off."""
+    from chatcarlo_amd.transport import transport_photons
+    g = _slab("lead", 0.5, 50.0)
+    pos, dirv, e = _pencil(20_000, 0.5, 100.0)
+    assert transport_photons(pos, dirv, e, g, np.random.default_rng(7)).n_fluorescence > 0
+    pos, dirv, e = _pencil(20_000, 0.5, 100.0)
+    r = transport_photons(pos, dirv, e, g, np.random.default_rng(9), fluorescence_enabled=False)
+    assert r.n_fluorescence == 0
+
+
+def test_escaped_spectrum_shows_the_lead_k_peak(engine):
+    """Photons leaving 0.05 cm of lead under a 100 keV pencil: the Pb K alpha / K beta band
+    (72-85 keV) holds over twice the share with fluorescence on as with it off (reference
+    tests/test_fluorescence.py:121-138)."""
+    from chatcarlo_amd.transport import transport_photons
+    g = _slab("lead", 0.05, 50.0)
+    frac = []
+    for on in (True, False):
+        pos, dirv, e = _pencil(200_000, 0.05, 100.0)
+        r = transport_photons(pos, dirv, e, g, np.random.default_rng(10), fluorescence_enabled=on)
+        fe = r.final_energy[r.escaped]
+        frac.append(np.mean((fe >= 72.0) & (fe <= 85.0)))
+    assert frac[0] > 2.0 * frac[1], frac
+
+
 def test_collision_vs_tracklength(engine):
     from chatcarlo_amd.tally import VoxelGrid
     from chatcarlo_amd.transport import transport_photons

@@ -259,3 +259,48 @@ def test_rayleigh_element_selection_in_bone(pack):
     i, f = core.energy_interval(pack, np.full(n, 50.0))
     picked = core.select_elements(pack, np.full(n, m), 2, i, f, RNG.random(n))
     assert set(np.unique(picked).tolist()) <= set(pack.mat_elem[m])
+
+
+# ---- the reference's bound-Compton and fluorescence invariants (tests/test_bound_compton.py,
+#      tests/test_fluorescence.py) on the pack
+
+def test_incoherent_function_runs_from_zero_to_z(pack):
+    """S(Z, 0) = 0, S(Z, q) rises monotonically, and S/Z is within 1 % of 1 at the table's end
+    (reference tests/test_bound_compton.py:17-26)."""
+    for z in (1, 8, 20, 82):
+        s = pack.sq_over_z[pack.elements.index(z)]
+        assert s[0] == 0.0
+        assert abs(s[-1] - 1.0) < 0.01
+        assert np.all(np.diff(s) >= -1e-9)
+
+
+def _share(pack, name, kind, z, e):
+    m = pack.mat_id(name)
+    i, f = core.energy_interval(pack, np.array([e]))
+    cdf = core.sel_cdf_at(pack, m, kind, i[0], f[0])
+    k = list(pack.mat_elem[m]).index(pack.elements.index(z))
+    assert abs(cdf[-1] - 1.0) < 1e-12 and np.all(np.diff(cdf) >= 0)
+    return cdf[k] - (cdf[k - 1] if k > 0 else 0.0), pack.mat_frac[m][k]
+
+
+def test_compton_picks_hydrogen_above_its_mass_fraction_in_bone(pack):
+    """Incoherent scattering per gram follows Z/A, largest for hydrogen, so bone's Compton element
+    selection picks H more often than its mass fraction (reference tests/test_bound_compton.py:99-112)."""
+    p, frac = _share(pack, "bone", 1, 1, 50.0)
+    assert p > frac
+
+
+def test_photo_selection_in_water_is_oxygen(pack):
+    """Water's photoelectric element selection is > 90 % oxygen at 50 keV (reference
+    tests/test_fluorescence.py:38-42)."""
+    p, _ = _share(pack, "water", 0, 8, 50.0)
+    assert p > 0.9
+
+
+def test_k_line_probabilities_sum_to_one(pack):
+    """Line probabilities of every fluorescing element sum to 1 (reference tests/test_fluorescence.py:31-35)."""
+    for z in (20, 26, 29, 74, 82):
+        i = pack.elements.index(z)
+        if pack.fl_edge[i] > 0:
+            probs = np.diff(np.concatenate([[0.0], pack.fl_cdf[i]]))
+            assert abs(probs.sum() - 1.0) < 1e-12 and np.all(probs >= -1e-15)
