@@ -2,10 +2,12 @@
 
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 
-Pinning of each function (tests/test_oracle_*.py):
-* intersect_* / material_at / next_boundary: closed-form roots checked against
-  brute-force ray marching on tiny scenes and against the reference's
-  known-answer distances (tests/test_geometry.py of the reference style).
+Pinning of each function (tests/test_oracle_golden.py, tests/test_oracle_geometry.py,
+tests/test_cpu_host.py, tests/test_oracle_mlp.py):
+* intersect_* / material_at / next_boundary: hand-computed roots of single
+  primitives, and next_boundary against a brute-force march of random rays through a
+  nested scene (boundaries where the set of containing primitives changes), material_at
+  against last-wins containment from the defining inequalities (tests/test_oracle_geometry.py).
 * xs_lookup: exact at the table nodes (equals the pack node values), agrees with
   the pack builder's independent log-log interpolation between nodes, matches the
   NIST spot values of the reference's test_materials.py:13-39 and the reference's

@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 Pinned by the published known-answer vectors of the Random123 distribution
-(kat_vectors, philox4x32 with 10 rounds), checked in tests/test_oracle_philox.py.
+(kat_vectors, philox4x32 with 10 rounds), checked in
+tests/test_oracle_golden.py::test_philox_known_answers.
 """
 from __future__ import annotations
 

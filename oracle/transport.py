@@ -12,7 +12,8 @@ estimator per material, track-length kerma and H*(10) per segment, batch
 moments per batch.  Pinned by Beer-Lambert (reference tests/test_transport.py:30-51),
 energy conservation (tests/test_fluorescence.py:80-97) and collision vs
 track-length agreement (tests/test_tally.py:54-75), restated in
-tests/test_oracle_transport.py.
+tests/test_cpu_host.py (test_oracle_beer_lambert, test_oracle_energy_conservation,
+test_oracle_collision_vs_tracklength).
 """
 from __future__ import annotations
 
