@@ -361,6 +361,30 @@ def test_dda_matches_exact_overlap(engine):
     assert np.max(np.abs(g["kerma"] - ref_k)) <= 1e-5 * scale
 
 
+def test_dda_segments_starting_on_a_voxel_face(engine):
+    """Segments that all start on the grid's x = 0 face, as a parallel field entering a phantom
+    does (reference tests/test_tally.py:115-137, where a mid-point sampler under-scored that layer
+    by 3 %): the surface voxel receives exactly sum min(L, 1 cm), every voxel of the row its exact
+    overlap, and the row the whole track length."""
+    N, lib = _lib()
+    origin, shape, vox = np.array([0.0, 0.0, 0.0]), (20, 10, 10), 1.0
+    engine.bind_grid(origin, shape, vox, False)
+    n = 200_000
+    rng = np.random.default_rng(11)
+    L = f32(rng.exponential(4.86, n).clip(max=19.9))
+    o = f32(np.tile([0.0, 5.5, 5.5], (n, 1)))
+    d = f32(np.tile([1.0, 0.0, 0.0], (n, 1)))
+    w = np.ones(n)
+    N.check(lib.cc_test_dda(engine.ctx, n, N.ptr(o, C.c_float), N.ptr(d, C.c_float), N.ptr(L, C.c_float),
+                            N.ptr(w, C.c_double), N.ptr(w, C.c_double)))
+    k = engine.read_grid(shape, moments=False)["kerma"]
+    Ld = L.astype(np.float64)
+    row = np.array([np.clip(Ld - j, 0.0, 1.0).sum() for j in range(shape[0])])
+    np.testing.assert_allclose(k[:, 5, 5], row, rtol=1e-6)
+    assert abs(k[0, 5, 5] - np.minimum(Ld, 1.0).sum()) <= 1e-6 * np.minimum(Ld, 1.0).sum()
+    assert np.isclose(k.sum(), Ld.sum(), rtol=1e-6) and np.isclose(k[:, 5, 5].sum(), k.sum(), rtol=1e-12)
+
+
 def test_source_parity(engine, dpack):
     from chatcarlo_amd.source import SourceRecord
     from oracle.source import source_from_uniforms
