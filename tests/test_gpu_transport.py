@@ -327,3 +327,27 @@ def test_segment_and_voxel_counts_match_oracle(engine, dpack, workload):
     for gpu, orc in ((out.n_events / n_gpu, seg), (out.n_voxel_visits / n_gpu, vox)):
         m, se = np.mean(orc), np.std(orc, ddof=1) / np.sqrt(len(orc))
         assert abs(gpu - m) < 5 * se + 0.005 * m, (workload, gpu, m, se)
+
+
+def test_h10_maximum_lands_in_air_near_the_focus_and_is_flagged(engine):
+    """The reference's real-scene diagnostic (tests/test_dose_diagnostics.py:78-97): a 120 kVp
+    field on a 30 cm water cube, H*(10) on a 2 cm grid; the hottest voxel is air close to the
+    focus (1/r^2 of a point source) and the near-source warning fires.  The source drops the
+    reference's mas key: mAs calibration needs SpekPy, absent here, and does not enter the
+    per-history map."""
+    from chatcarlo_amd.diagnostics import max_voxel_position_cm, near_source_air_warning
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.scene import validate_scene
+    from chatcarlo_amd.transport import run_transport
+    src = {"kvp": 120, "position": [0, -180, 140], "direction": [0, 1, 0],
+           "field": {"size_cm": [35, 43], "sid_cm": 180}, "filtration_mm_al": 2.5}
+    geo = [{"name": "slab", "shape": "box", "material": "water", "center": [0, 0, 140], "size_cm": [30, 30, 30]}]
+    sc = validate_scene({"source": src, "geometry": geo})
+    assert sc.ok
+    g = Geometry(sc.raw["geometry"])
+    r = run_transport(sc, n_histories=200_000, seed=3, dose_grid=True, grid_resolution_cm=2.0)
+    pos = max_voxel_position_cm(r.grid, r.grid.h10_map_pSv() / r.n_histories)
+    material = str(g.material_at(pos[None, :])[0])
+    dist = float(np.linalg.norm(pos - np.asarray(src["position"], dtype=float)))
+    w = near_source_air_warning(material, g.background, dist, g.nearest_object_distance_cm(src["position"]))
+    assert w is not None, (material, dist)
