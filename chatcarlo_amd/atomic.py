@@ -21,8 +21,8 @@ installable offline, so this module supplies a self-contained replacement:
   F = (1 + k^2/4)^-2.
 * K-shell fluorescence data (edge, omega_K, eight K lines and their relative
   radiative rates) for the elements whose K lines reach the 5 keV cut-off of
-  chatcarlo/physics.py:38; values are standard X-ray data-booklet energies and
-  Krause fluorescence yields.
+  chatcarlo/physics.py:38; line energies and rates are standard X-ray data-booklet values, the
+  fluorescence yields those of the Bambynek fit (k_fluorescence_yield).
 * Material compositions (mass fractions) and densities of the NIST compound
   data base entries the reference aliases (chatcarlo/materials.py:34-49).
 """
@@ -39,14 +39,14 @@ HC_KEV_ANGSTROM = 12.3984193   # chatcarlo/physics.py:35
 MEC2_KEV = 511.0               # chatcarlo/physics.py:34
 
 SYMBOL = {1: "H", 6: "C", 7: "N", 8: "O", 11: "Na", 12: "Mg", 13: "Al", 14: "Si",
-          15: "P", 16: "S", 17: "Cl", 18: "Ar", 19: "K", 20: "Ca", 26: "Fe",
-          29: "Cu", 30: "Zn", 74: "W", 82: "Pb"}
+          15: "P", 16: "S", 17: "Cl", 18: "Ar", 19: "K", 20: "Ca", 22: "Ti", 26: "Fe",
+          29: "Cu", 30: "Zn", 33: "As", 74: "W", 82: "Pb"}
 Z_OF = {v: k for k, v in SYMBOL.items()}
 
 ATOMIC_WEIGHT = {1: 1.008, 6: 12.011, 7: 14.007, 8: 15.999, 11: 22.98977, 12: 24.305,
                  13: 26.98154, 14: 28.0855, 15: 30.97376, 16: 32.06, 17: 35.453,
-                 18: 39.948, 19: 39.0983, 20: 40.078, 26: 55.845, 29: 63.546,
-                 30: 65.38, 74: 183.84, 82: 207.2}
+                 18: 39.948, 19: 39.0983, 20: 40.078, 22: 47.867, 26: 55.845, 29: 63.546,
+                 30: 65.38, 33: 74.9216, 74: 183.84, 82: 207.2}
 
 # Ground-state occupations grouped as Slater groups: (n, kind, electrons),
 # kind "sp" for (ns,np), "d", "f".
@@ -69,9 +69,11 @@ CONFIG = {
     18: _CORE_NE + [(3, "sp", 8)],
     19: _CORE_AR + [(4, "sp", 1)],
     20: _CORE_AR + [(4, "sp", 2)],
+    22: _CORE_AR + [(3, "d", 2), (4, "sp", 2)],
     26: _CORE_AR + [(3, "d", 6), (4, "sp", 2)],
     29: _CORE_AR + [(3, "d", 10), (4, "sp", 1)],
     30: _CORE_AR + [(3, "d", 10), (4, "sp", 2)],
+    33: _CORE_AR + [(3, "d", 10), (4, "sp", 5)],
     74: _CORE_XE + [(4, "f", 14), (5, "d", 4), (6, "sp", 2)],
     82: _CORE_XE + [(4, "f", 14), (5, "d", 10), (6, "sp", 4)],
 }
@@ -147,22 +149,34 @@ def incoherent_sf(z: int, q) -> np.ndarray:
 
 
 # ---- K-shell fluorescence (elements with K lines >= 5 keV) -----------------
+def k_fluorescence_yield(z: int) -> float:
+    """omega_K from the fit of W. Bambynek (1984) recommended by Hubbell et al., J. Phys. Chem.
+    Ref. Data 23, 339 (1994): s = sum_i C_i Z^i, omega_K = s^4 / (1 + s^4).  It reproduces the
+    yields the reference logs from xraylib (Cu 0.454, W 0.954, Pb 0.9634)."""
+    s = 3.70e-2 + 3.112e-2 * z + 5.44e-5 * z * z - 1.25e-6 * z ** 3
+    return s ** 4 / (1.0 + s ** 4)
+
+
+
 # line order: KL2, KL3, KM2, KM3, KN2, KN3, KO, KP (chatcarlo/materials.py:459-460)
 # energies keV; rates are relative radiative rates (normalised on use).
 K_SHELL = {
-    26: dict(edge=7.112, omega=0.340, jump=None,
+    26: dict(edge=7.112, omega=k_fluorescence_yield(26), jump=None,
              lines=[6.39084, 6.40384, 7.05798, 7.05798, 0, 0, 0, 0],
              rates=[0.2976, 0.5836, 0.0396, 0.0792, 0, 0, 0, 0]),
-    29: dict(edge=8.979, omega=0.440, jump=None,
+    29: dict(edge=8.979, omega=k_fluorescence_yield(29), jump=None,
              lines=[8.02783, 8.04778, 8.90529, 8.90529, 0, 0, 0, 0],
              rates=[0.2970, 0.5830, 0.0400, 0.0800, 0, 0, 0, 0]),
-    30: dict(edge=9.659, omega=0.474, jump=7.67,
+    30: dict(edge=9.659, omega=k_fluorescence_yield(30), jump=7.67,
              lines=[8.61578, 8.63886, 9.572, 9.572, 0, 0, 0, 0],
              rates=[0.2965, 0.5815, 0.0405, 0.0815, 0, 0, 0, 0]),
-    74: dict(edge=69.525, omega=0.950, jump=None,
+    33: dict(edge=11.867, omega=k_fluorescence_yield(33), jump=7.1,
+             lines=[10.50799, 10.54372, 11.7203, 11.7262, 11.8643, 11.8643, 0, 0],
+             rates=[0.2860, 0.5560, 0.0480, 0.0940, 0.0050, 0.0110, 0, 0]),
+    74: dict(edge=69.525, omega=k_fluorescence_yield(74), jump=None,
              lines=[57.9817, 59.3182, 66.9514, 67.2443, 69.0300, 69.1010, 69.4500, 69.5200],
              rates=[0.2880, 0.5000, 0.0550, 0.1070, 0.0140, 0.0270, 0.0080, 0.0010]),
-    82: dict(edge=88.004, omega=0.9634, jump=None,
+    82: dict(edge=88.004, omega=k_fluorescence_yield(82), jump=None,
              lines=[72.8042, 74.9694, 84.4500, 84.9360, 87.3000, 87.3670, 87.9100, 87.9900],
              rates=[0.2928, 0.4924, 0.0570, 0.1093, 0.0135, 0.0263, 0.0075, 0.0012]),
 }
@@ -171,7 +185,7 @@ K_SHELL = {
 # sections of elements without a bundled NIST element table.
 K_EDGE = {1: 0.0136, 6: 0.2838, 7: 0.4016, 8: 0.5320, 11: 1.0721, 12: 1.3050, 13: 1.5596,
           14: 1.8389, 15: 2.1455, 16: 2.4720, 17: 2.8224, 18: 3.2029, 19: 3.6074,
-          20: 4.0381, 26: 7.112, 29: 8.979, 30: 9.659, 74: 69.525, 82: 88.004}
+          20: 4.0381, 22: 4.9664, 26: 7.112, 29: 8.979, 30: 9.659, 33: 11.867, 74: 69.525, 82: 88.004}
 
 # ---- materials --------------------------------------------------------------
 # name -> (density g/cm^3, [(Z, mass fraction)], NIST XAAMDI table key)
@@ -199,10 +213,22 @@ MATERIALS = {
     "iron": (7.874, [(26, 1.0)], "z26_Fe"),
     "tungsten": (19.3, [(74, 1.0)], "z74_W"),
     "calcium": (1.55, [(20, 1.0)], "z20_Ca"),
+    # ICRU Report 44 compositions as tabulated by NIST XAAMDI (Hubbell & Seltzer, Table 2)
+    "muscle": (1.05, [(1, 0.102), (6, 0.143), (7, 0.034), (8, 0.710), (11, 0.001), (15, 0.002),
+                      (16, 0.003), (17, 0.001), (19, 0.004)], "muscle"),
+    "adipose": (0.95, [(1, 0.114), (6, 0.598), (7, 0.007), (8, 0.278), (11, 0.001), (16, 0.001),
+                       (17, 0.001)], "adipose"),
+    # NIST compound "Glass, Lead" (the reference's lead_glass alias, chatcarlo/materials.py:46)
+    "lead_glass": (6.22, [(8, 0.156453), (14, 0.080866), (22, 0.008092), (33, 0.002651),
+                          (82, 0.751938)], "lead_glass"),
 }
+# Element symbols resolve to the element materials (chatcarlo/materials.py:52-65 resolves any
+# symbol through xraylib; here the symbols of the bundled element tables).
+SYMBOL_MATERIAL = {"pb": "lead", "al": "aluminum", "cu": "copper", "fe": "iron", "w": "tungsten",
+                   "ca": "calcium"}
 # Element tables bundled from NIST XAAMDI (chatcarlo/data/nist_xaamdi/z*.csv).
 ELEMENT_TABLE = {1: "z01_H", 6: "z06_C", 7: "z07_N", 8: "z08_O", 13: "z13_Al",
                  20: "z20_Ca", 26: "z26_Fe", 29: "z29_Cu", 74: "z74_W", 82: "z82_Pb"}
 # Element without its own table -> tabulated element whose photo cross section
 # is scaled to it (edge-energy shifted, Z^4/A scaled).
-PHOTO_PROXY = {11: 13, 12: 13, 14: 13, 15: 13, 16: 20, 17: 20, 18: 20, 19: 20, 30: 29}
+PHOTO_PROXY = {11: 13, 12: 13, 14: 13, 15: 13, 16: 20, 17: 20, 18: 20, 19: 20, 22: 20, 30: 29, 33: 29}

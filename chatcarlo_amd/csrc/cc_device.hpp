@@ -29,7 +29,8 @@ constexpr float kMEC2 = 511.0f;         // chatcarlo/physics.py:34
 constexpr float kFluorCut = 5.0f;       // chatcarlo/physics.py:38
 constexpr float kTwoPi = 6.28318530717958647692f;
 constexpr int kMaxEdges = 16;
-constexpr int kFlStride = 20;
+constexpr int kFlStride = 32;   // edge, omega, 1/dlnE, ln edge, lines[8], cdf[8], K share[12]
+constexpr int kKFracNodes = 12;
 constexpr int kNParts = 5;
 enum { P_PHOTO = 0, P_COMPT = 1, P_RAYL = 2, P_MUEN = 3, P_TOTAL = 4 };
 
@@ -537,7 +538,12 @@ __device__ __forceinline__ bool fluor_sample(const Tables& T, int m, float E, co
   e_line = 0.0f;
   float edge = f[0];
   if (!(edge > 0.0f) || !(E > edge)) return false;
-  if (!(u1 < f[2])) return false;
+  // K-shell share, linear in ln E between nodes spaced 1/f[2] from ln(edge) = f[3]
+  float t = (logf(E) - f[3]) * f[2];
+  int j = min(max((int)t, 0), kKFracNodes - 2);
+  float k0 = f[20 + j], k1 = f[21 + j];
+  float kf = k0 + (t - (float)j) * (k1 - k0);
+  if (!(u1 < kf)) return false;
   if (!(u2 < f[1])) return false;
   int li = 0;
   for (int l = 0; l < 8; ++l) li += (u3 > f[12 + l]) ? 1 : 0;

@@ -129,9 +129,10 @@ class Engine:
         t.q_s = fa("qs", p.q_s)
         t.ray_x = fa("rx", p.ray_x)
         t.ray_a = fa("ra", p.ray_a)
-        fl = np.zeros((len(p.elements), 20))
-        fl[:, 0], fl[:, 1], fl[:, 2] = p.fl_edge, p.fl_omega, p.fl_kfrac
-        fl[:, 4:12], fl[:, 12:20] = p.fl_lines, p.fl_cdf
+        fl = np.zeros((len(p.elements), 32))
+        fl[:, 0], fl[:, 1], fl[:, 2] = p.fl_edge, p.fl_omega, p.fl_kinv
+        fl[:, 3] = np.log(np.where(p.fl_edge > 0, p.fl_edge, 1.0))
+        fl[:, 4:12], fl[:, 12:20], fl[:, 20:32] = p.fl_lines, p.fl_cdf, p.fl_kfrac
         t.fl = fa("fl", fl)
         nh = len(p.h10_e)
         h = np.zeros((4, nh))

@@ -61,7 +61,8 @@ typedef struct {
 /* Physics data pack (host pointers, float32 unless noted).  Layouts:
  * sig/slope [n_mat][5][n_e] parts photo, compton, rayleigh, mu_en, total;
  * edge_* [n_mat][max_edges](...); sel per material (3, n_e, n_elem) at sel_off[m];
- * fl [n_el][20] = edge, omega, k_frac, pad, lines[8], cdf[8];
+ * fl [n_el][32] = edge, omega, 1/dlnE, ln edge, lines[8], cdf[8], k_share[12] (K-shell share of
+ *   photoabsorption on nodes ln edge + j dlnE, linear in ln E between them);
  * h10 [4][n_h10]: energies, values, ln-ratios of energies, ln-ratios of values.
  * Replaces chatcarlo/materials.py:202-496 and dose_coefficients.py:26-42. */
 typedef struct {

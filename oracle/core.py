@@ -379,6 +379,26 @@ def rayleigh_u(pack, mat, e, u, k_trials):
     return rayleigh_sample(pack, mat, e, _matrix_draw(np.asarray(u)), k_trials)
 
 
+def k_share(pack, el, e):
+    """K-shell share of photoabsorption by element el at energy e above its K edge: the pack's
+    nodes, spaced 1/fl_kinv in ln E from ln(edge), linear in ln E between them (the pack's
+    restatement of CS_Photo_Partial(Z,K,E)/CS_Photo(Z,E), chatcarlo/physics.py:193-197).
+    Zero where the element has no fluorescence data."""
+    el = np.asarray(el)
+    e = np.asarray(e, dtype=float)
+    edge = pack.fl_edge[el]
+    ok = edge > 0
+    ln_edge = np.log(np.where(ok, edge, 1.0))
+    if pack.dtype == "f32":   # the device keeps ln(edge) as a float32 table value
+        ln_edge = ln_edge.astype(np.float32).astype(np.float64)
+    t = (np.log(e) - ln_edge) * pack.fl_kinv[el]
+    nk = pack.fl_kfrac.shape[1]
+    j = np.clip(np.floor(t).astype(int), 0, nk - 2)
+    k0 = pack.fl_kfrac[el, j]
+    k1 = pack.fl_kfrac[el, j + 1]
+    return np.where(ok, k0 + (t - j) * (k1 - k0), 0.0)
+
+
 def fluor_sample(pack, mat, e, draw):
     """K fluorescence after photoabsorption (chatcarlo/physics.py:138-212).
 
@@ -393,7 +413,7 @@ def fluor_sample(pack, mat, e, draw):
     el = select_elements(pack, mat, PHOTO, i, f, draw(allidx, 0))
     u1, u2, u3 = draw(allidx, 1), draw(allidx, 2), draw(allidx, 3)
     edge = pack.fl_edge[el]
-    cand = (edge > 0) & (e > edge) & (u1 < pack.fl_kfrac[el]) & (u2 < pack.fl_omega[el])
+    cand = (edge > 0) & (e > edge) & (u1 < k_share(pack, el, e)) & (u2 < pack.fl_omega[el])
     emit = np.zeros(n, dtype=bool)
     eline = np.zeros(n)
     for j in np.where(cand)[0]:
