@@ -17,6 +17,7 @@ CC_MAX_PRIMS, CC_MAX_MAT = 64, 32
 RUN_FLUORESCENCE, RUN_GRID, RUN_TRACK_UNCERTAINTY, RUN_TIMING = 1, 2, 4, 8
 RUN_REGION, RUN_RECORD, RUN_IMPORTANCE, RUN_SOURCE_BIAS = 16, 32, 64, 128
 RUN_TRACE, RUN_PER_BATCH = 256, 512
+NOALLOC_MAX_PHOTONS = 1 << 16   # photons per cc_transport_noalloc call
 EV_BOUNDARY, EV_ESCAPE, EV_PHOTOELECTRIC, EV_FLUORESCENCE, EV_COMPTON, EV_RAYLEIGH = range(6)
 TRACE_DTYPE = np.dtype([("photon", np.uint32), ("flight", np.int32), ("step", np.int32), ("material", np.int32),
                         ("start", np.float32, 3), ("end", np.float32, 3), ("energy_keV", np.float32),
@@ -138,6 +139,8 @@ _SIGS = {
                          C.c_uint32, C.c_uint32, C.POINTER(CcRunOut)]),
     "cc_transport": (C.c_int, [C.c_void_p, C.c_int64, dp, dp, dp, C.c_uint64, C.c_uint64, C.c_uint32,
                                ip, u8p, u8p, C.POINTER(CcRunOut)]),
+    "cc_transport_noalloc": (C.c_int, [C.c_void_p, C.c_int64, dp, dp, dp, C.c_uint64, C.c_uint64, C.c_uint32,
+                                       ip, u8p, u8p, C.POINTER(CcRunOut)]),
     "cc_locate": (C.c_int, [C.c_void_p, C.c_int64, fp, ip]),
     "cc_test_boundary": (C.c_int, [C.c_void_p, C.c_int64, fp, fp, C.c_float, fp, ip]),
     "cc_test_xs": (C.c_int, [C.c_void_p, C.c_int64, ip, fp, fp]),

@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2098,6 +2099,58 @@ __global__ void k_test_source(Src s, uint64_t seed, uint64_t hist0, int n, float
 }
 
 // ---------------------------------------------------------------------------
+// allocation-free transport of caller photons (cc_transport_noalloc)
+// ---------------------------------------------------------------------------
+// Every buffer of a cc_transport call beyond the context's set-up state lives in one file-scope
+// device array: photon records, active lists, queues, per-photon scores and the staging of the
+// caller's arrays.  Each GPU holds its own copy; host code reaches it only through the address
+// hipGetSymbolAddress gives for that device (scratch_base), never through the name itself.
+constexpr int64_t kNoAllocMax = CC_NOALLOC_MAX_PHOTONS;   // photons per allocation-free call
+constexpr size_t kNoAllocAlign = 256;
+constexpr size_t na_round(size_t b) { return (b + kNoAllocAlign - 1) / kNoAllocAlign * kNoAllocAlign; }
+constexpr size_t kNaRec = 0;                                                            // records
+constexpr size_t kNaLists = kNaRec + na_round((size_t)kNoAllocMax * kPhotonWords * 4);  // 7 int lists
+constexpr size_t kNaScore = kNaLists + 7 * na_round((size_t)kNoAllocMax * 4);          // float score
+constexpr size_t kNaHscore = kNaScore + na_round((size_t)kNoAllocMax * 4);             // double hscore
+constexpr size_t kNaIn = kNaHscore + na_round((size_t)kNoAllocMax * 8);                // 7 doubles / photon
+constexpr size_t kNaNsc = kNaIn + na_round((size_t)kNoAllocMax * 7 * 8);               // int n_scatter
+constexpr size_t kNaFlags = kNaNsc + na_round((size_t)kNoAllocMax * 4);                // 2 bytes / photon
+constexpr size_t kScratchBytes = kNaFlags + na_round((size_t)kNoAllocMax * 2);         // 10.9 MB
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// caller photons (pos, dir: n x 3 doubles; energy: n doubles, in one staging block) -> records
+__global__ void k_pack_given(int n, const double* in, uint32_t* rec) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float w[kPhotonWords];
+  for (int k = 0; k < kPhotonWords; ++k) w[k] = 0.0f;
+  for (int k = 0; k < 3; ++k) {
+    w[W_X + k] = (float)in[3 * (size_t)i + k];
+    w[W_DX + k] = (float)in[3 * (size_t)n + 3 * (size_t)i + k];
+  }
+  w[W_E] = (float)in[6 * (size_t)n + i];
+  uint4* r = (uint4*)(rec + (size_t)i * kPhotonWords);
+  const uint4* s = (const uint4*)w;
+  for (int k = 0; k < kPhotonWords / 4; ++k) r[k] = s[k];
+}
+
+// records -> the caller's layout: pos, dir, energy doubles; n_scatter; absorbed and escaped bytes
+__global__ void k_unpack_given(int n, const uint32_t* rec, double* out, int* nsc, uint8_t* fl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t* w = rec + (size_t)i * kPhotonWords;
+  for (int k = 0; k < 3; ++k) {
+    out[3 * (size_t)i + k] = (double)__uint_as_float(w[W_X + k]);
+    out[3 * (size_t)n + 3 * (size_t)i + k] = (double)__uint_as_float(w[W_DX + k]);
+  }
+  out[6 * (size_t)n + i] = (double)__uint_as_float(w[W_E]);
+  nsc[i] = (int)w[W_NSC];
+  const int f = (int)w[W_FLAGS];
+  fl[i] = (f & F_ABSORBED) ? 1 : 0;
+  fl[(size_t)n + i] = (f & F_ESCAPED) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 template <class T>
@@ -2119,6 +2172,7 @@ int grid_blocks(size_t n) {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  unsigned char* scratch = nullptr;   // this device's copy of `scratch` (cc_transport_noalloc)
   // tables
   bool have_tables = false;
   Tables T{};
@@ -2243,27 +2297,89 @@ void free_pool(Ctx* c) {
   c->cap = 0;
 }
 
+// Field views of a block of photon records
+PState records_state(uint32_t* b, float* score) {
+  PState S{};
+  S.x.p = (float*)(b + W_X); S.y.p = (float*)(b + W_Y); S.z.p = (float*)(b + W_Z);
+  S.dx.p = (float*)(b + W_DX); S.dy.p = (float*)(b + W_DY); S.dz.p = (float*)(b + W_DZ);
+  S.e.p = (float*)(b + W_E); S.tau.p = (float*)(b + W_TAU); S.wt.p = (float*)(b + W_WT);
+  S.ctr.p = b + W_CTR; S.hid.p = b + W_HID; S.tag.p = b + W_TAG;
+  S.nsc.p = (int*)(b + W_NSC); S.mat.p = (int*)(b + W_MAT); S.flags.p = (int*)(b + W_FLAGS);
+  S.sc.p = (int*)(b + W_SC);
+  S.score = score;
+  return S;
+}
+
 int ensure_pool(Ctx* c, size_t n) {
   if (n <= c->cap) return CC_OK;
   free_pool(c);
   int r;
+  float* score = nullptr;
   if ((r = dalloc(&c->photons, n * kPhotonWords)) || (r = dalloc(&c->act0, n)) || (r = dalloc(&c->act1, n)) ||
       (r = dalloc(&c->qp, n)) || (r = dalloc(&c->qc, n)) || (r = dalloc(&c->qr, n)) ||
-      (r = dalloc(&c->S.score, n)) || (r = dalloc(&c->act2, n)) || (r = dalloc(&c->hscore, n)) ||
-      (r = dalloc(&c->coll, n)))
+      (r = dalloc(&score, n)) || (r = dalloc(&c->act2, n)) || (r = dalloc(&c->hscore, n)) ||
+      (r = dalloc(&c->coll, n))) {
+    c->S.score = score;
     return r;
-  {
-    uint32_t* b = c->photons;
-    c->S.x.p = (float*)(b + W_X); c->S.y.p = (float*)(b + W_Y); c->S.z.p = (float*)(b + W_Z);
-    c->S.dx.p = (float*)(b + W_DX); c->S.dy.p = (float*)(b + W_DY); c->S.dz.p = (float*)(b + W_DZ);
-    c->S.e.p = (float*)(b + W_E); c->S.tau.p = (float*)(b + W_TAU); c->S.wt.p = (float*)(b + W_WT);
-    c->S.ctr.p = b + W_CTR; c->S.hid.p = b + W_HID; c->S.tag.p = b + W_TAG;
-    c->S.nsc.p = (int*)(b + W_NSC); c->S.mat.p = (int*)(b + W_MAT); c->S.flags.p = (int*)(b + W_FLAGS);
-    c->S.sc.p = (int*)(b + W_SC);
   }
+  c->S = records_state(c->photons, score);
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(n)), dim3(kBlock), 0, c->stream, c->hscore, n, 0.0);
   HIP_TRY(hipGetLastError());
   c->cap = n;
+  return CC_OK;
+}
+
+// The particle pool a call works on: the context's own, or (allocation-free calls) one laid out
+// in this device's `scratch`.  PoolSwap puts a pool in place for the duration of a call.
+struct PoolView {
+  uint32_t* photons;
+  PState S;
+  int *act0, *act1, *act2, *qp, *qc, *qr, *coll;
+  double* hscore;
+  size_t cap;
+};
+PoolView pool_of(const Ctx* c) {
+  return PoolView{c->photons, c->S, c->act0, c->act1, c->act2, c->qp, c->qc, c->qr, c->coll, c->hscore, c->cap};
+}
+void set_pool(Ctx* c, const PoolView& v) {
+  c->photons = v.photons; c->S = v.S;
+  c->act0 = v.act0; c->act1 = v.act1; c->act2 = v.act2; c->qp = v.qp; c->qc = v.qc; c->qr = v.qr;
+  c->coll = v.coll; c->hscore = v.hscore; c->cap = v.cap;
+}
+PoolView scratch_pool(unsigned char* base) {
+  PoolView v{};
+  v.photons = (uint32_t*)(base + kNaRec);
+  v.S = records_state(v.photons, (float*)(base + kNaScore));
+  int* lists[7];
+  for (int k = 0; k < 7; ++k) lists[k] = (int*)(base + kNaLists + (size_t)k * na_round((size_t)kNoAllocMax * 4));
+  v.act0 = lists[0]; v.act1 = lists[1]; v.act2 = lists[2]; v.qp = lists[3]; v.qc = lists[4]; v.qr = lists[5];
+  v.coll = lists[6];
+  v.hscore = (double*)(base + kNaHscore);
+  v.cap = (size_t)kNoAllocMax;
+  return v;
+}
+struct PoolSwap {
+  Ctx* c;
+  PoolView saved;
+  PoolSwap(Ctx* ctx, const PoolView& v) : c(ctx), saved(pool_of(ctx)) { set_pool(c, v); }
+  ~PoolSwap() { set_pool(c, saved); }
+};
+
+// Address of this device's `scratch`, resolved once per device with the device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_base(int device, unsigned char** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(CC_ERR_ARG, "device %d beyond %d", device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(CC_ERR_HIP, "hipGetSymbolAddress(scratch) gave a null address on device %d", device);
+    g_scratch_ptr[device] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[device];
   return CC_OK;
 }
 
@@ -2604,6 +2720,10 @@ int cc_ctx_create(int32_t device, void** out) {
   }
   HIP_TRY(hipHostMalloc((void**)&c->h_cnt, sizeof(uint32_t) * 16, hipHostMallocDefault));
   HIP_TRY(hipMemset(c->err, 0, sizeof(int)));
+  if ((r = scratch_base(device, &c->scratch))) {
+    (void)cc_ctx_destroy(c);
+    return r;
+  }
   *out = c;
   return CC_OK;
 }
@@ -3160,35 +3280,11 @@ int cc_run(void* ctx, const cc_source* src, uint64_t n_hist, uint64_t seed, uint
   return CC_OK;
 }
 
-int cc_transport(void* ctx, int64_t n, double* pos, double* dir, double* energy, uint64_t seed, uint64_t first,
-                 uint32_t flags, int32_t* n_scatter, uint8_t* absorbed, uint8_t* escaped, cc_run_out* out) {
-  Ctx* c = (Ctx*)ctx;
-  if (!c || !pos || !dir || !energy || !out) return fail(CC_ERR_ARG, "null argument");
-  if (n < 0 || n > (1ll << 30)) return fail(CC_ERR_ARG, "n out of range");
-  if (!c->have_tables || !c->have_scene) return fail(CC_ERR_STATE, "tables and scene must be uploaded");
+// Transport of the photons whose position, direction and energy are in records [0, n) of the
+// pool in place (chatcarlo/transport.py:98-233): tau drawn, rounds to the end, grid batch ended.
+static int transport_core(Ctx* c, int64_t n, uint64_t seed, uint64_t first, uint32_t flags, cc_run_out* out) {
   const bool grid_on = (flags & CC_RUN_GRID) != 0;
-  if (grid_on && !c->have_grid) return fail(CC_ERR_STATE, "CC_RUN_GRID without a bound grid");
-  for (int64_t i = 0; i < n; ++i)
-    if (!(energy[i] >= 1.0 && energy[i] <= 150.0))
-      return fail(CC_ERR_ARG, "energy %g keV outside the cross-section table range [1, 150] keV", energy[i]);
-  HIP_TRY(hipSetDevice(c->device));
-  std::memset(out, 0, sizeof(*out));
-  if (n == 0) return CC_OK;
-  int r = ensure_pool(c, (size_t)n);
-  if (r) return r;
-  std::vector<uint32_t> h((size_t)n * kPhotonWords, 0u);   // photon records: position, direction, energy
-  auto put = [&](int64_t i, int w, double v) {
-    const float f = (float)v;
-    std::memcpy(&h[(size_t)i * kPhotonWords + w], &f, sizeof(float));
-  };
-  for (int64_t i = 0; i < n; ++i) {
-    for (int k = 0; k < 3; ++k) {
-      put(i, W_X + k, pos[3 * i + k]);
-      put(i, W_DX + k, dir[3 * i + k]);
-    }
-    put(i, W_E, energy[i]);
-  }
-  HIP_TRY(hipMemcpy(c->photons, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  int r;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->err, (size_t)1, 0);
   Timer tm{c, false};
   Launch L = make_launch(c, seed, first, grid_on ? 1 : 0, (flags & CC_RUN_FLUORESCENCE) ? 1 : 0);
@@ -3221,6 +3317,48 @@ int cc_transport(void* ctx, int64_t n, double* pos, double* dir, double* energy,
   int err_bits = 0;
   r = read_batch(c, (uint32_t)n, out, &err_bits);
   if (r) return r;
+  out->n_histories = (uint64_t)n;
+  out->n_batches = 1;
+  out->error_bits = (uint32_t)err_bits;
+  return CC_OK;
+}
+
+// Argument checks shared by cc_transport and cc_transport_noalloc.
+static int check_transport(Ctx* c, int64_t n, const double* pos, const double* dir, const double* energy,
+                           uint32_t flags, const cc_run_out* out) {
+  if (!c || !pos || !dir || !energy || !out) return fail(CC_ERR_ARG, "null argument");
+  if (n < 0 || n > (1ll << 30)) return fail(CC_ERR_ARG, "n out of range");
+  if (!c->have_tables || !c->have_scene) return fail(CC_ERR_STATE, "tables and scene must be uploaded");
+  if ((flags & CC_RUN_GRID) && !c->have_grid) return fail(CC_ERR_STATE, "CC_RUN_GRID without a bound grid");
+  for (int64_t i = 0; i < n; ++i)
+    if (!(energy[i] >= 1.0 && energy[i] <= 150.0))
+      return fail(CC_ERR_ARG, "energy %g keV outside the cross-section table range [1, 150] keV", energy[i]);
+  return CC_OK;
+}
+
+int cc_transport(void* ctx, int64_t n, double* pos, double* dir, double* energy, uint64_t seed, uint64_t first,
+                 uint32_t flags, int32_t* n_scatter, uint8_t* absorbed, uint8_t* escaped, cc_run_out* out) {
+  Ctx* c = (Ctx*)ctx;
+  int r = check_transport(c, n, pos, dir, energy, flags, out);
+  if (r) return r;
+  HIP_TRY(hipSetDevice(c->device));
+  std::memset(out, 0, sizeof(*out));
+  if (n == 0) return CC_OK;
+  if ((r = ensure_pool(c, (size_t)n))) return r;
+  std::vector<uint32_t> h((size_t)n * kPhotonWords, 0u);   // photon records: position, direction, energy
+  auto put = [&](int64_t i, int w, double v) {
+    const float f = (float)v;
+    std::memcpy(&h[(size_t)i * kPhotonWords + w], &f, sizeof(float));
+  };
+  for (int64_t i = 0; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      put(i, W_X + k, pos[3 * i + k]);
+      put(i, W_DX + k, dir[3 * i + k]);
+    }
+    put(i, W_E, energy[i]);
+  }
+  HIP_TRY(hipMemcpy(c->photons, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if ((r = transport_core(c, n, seed, first, flags, out))) return r;
   HIP_TRY(hipMemcpy(h.data(), c->photons, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
   auto getf = [&](int64_t i, int w) {
     float f;
@@ -3238,10 +3376,47 @@ int cc_transport(void* ctx, int64_t n, double* pos, double* dir, double* energy,
     if (absorbed) absorbed[i] = (fl & F_ABSORBED) ? 1 : 0;
     if (escaped) escaped[i] = (fl & F_ESCAPED) ? 1 : 0;
   }
-  out->n_histories = (uint64_t)n;
-  out->n_batches = 1;
-  out->error_bits = (uint32_t)err_bits;
-  if (err_bits) return fail(CC_ERR_DEVICE, "device error bits 0x%x", err_bits);
+  if (out->error_bits) return fail(CC_ERR_DEVICE, "device error bits 0x%x", out->error_bits);
+  return CC_OK;
+}
+
+int cc_transport_noalloc(void* ctx, int64_t n, double* pos, double* dir, double* energy, uint64_t seed,
+                         uint64_t first, uint32_t flags, int32_t* n_scatter, uint8_t* absorbed, uint8_t* escaped,
+                         cc_run_out* out) {
+  Ctx* c = (Ctx*)ctx;
+  int r = check_transport(c, n, pos, dir, energy, flags, out);
+  if (r) return r;
+  if (n > kNoAllocMax)
+    return fail(CC_ERR_ARG, "cc_transport_noalloc takes at most %lld photons per call, got %lld",
+                (long long)kNoAllocMax, (long long)n);
+  if (flags & CC_RUN_TRACE) return fail(CC_ERR_ARG, "cc_transport_noalloc does not log segments (CC_RUN_TRACE)");
+  if (!c->scratch) return fail(CC_ERR_STATE, "no scratch address for device %d", c->device);
+  HIP_TRY(hipSetDevice(c->device));
+  std::memset(out, 0, sizeof(*out));
+  if (n == 0) return CC_OK;
+  PoolSwap swap(c, scratch_pool(c->scratch));
+  double* stage = (double*)(c->scratch + kNaIn);   // pos [3n], dir [3n], energy [n]
+  int* nsc = (int*)(c->scratch + kNaNsc);
+  uint8_t* fl = (uint8_t*)(c->scratch + kNaFlags);  // absorbed [n], escaped [n]
+  const size_t nn = (size_t)n;
+  HIP_TRY(hipMemcpyAsync(stage, pos, 3 * nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(stage + 3 * nn, dir, 3 * nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(stage + 6 * nn, energy, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const int nb = (int)((n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_pack_given, dim3(nb), dim3(kBlock), 0, c->stream, (int)n, (const double*)stage, c->photons);
+  HIP_TRY(hipGetLastError());
+  if ((r = transport_core(c, n, seed, first, flags, out))) return r;
+  hipLaunchKernelGGL(k_unpack_given, dim3(nb), dim3(kBlock), 0, c->stream, (int)n, (const uint32_t*)c->photons,
+                     stage, nsc, fl);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(pos, stage, 3 * nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(dir, stage + 3 * nn, 3 * nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(energy, stage + 6 * nn, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (n_scatter) HIP_TRY(hipMemcpyAsync(n_scatter, nsc, nn * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  if (absorbed) HIP_TRY(hipMemcpyAsync(absorbed, fl, nn, hipMemcpyDeviceToHost, c->stream));
+  if (escaped) HIP_TRY(hipMemcpyAsync(escaped, fl + nn, nn, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (out->error_bits) return fail(CC_ERR_DEVICE, "device error bits 0x%x", out->error_bits);
   return CC_OK;
 }
 

@@ -220,6 +220,19 @@ int cc_transport(void* ctx, int64_t n, double* pos, double* dir, double* energy,
                  uint64_t first_history, uint32_t flags, int32_t* n_scatter, uint8_t* absorbed,
                  uint8_t* escaped, cc_run_out* out);
 
+/* cc_transport without device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, results and errors as cc_transport, for at most
+ * CC_NOALLOC_MAX_PHOTONS photons per call (CC_ERR_ARG beyond) and without CC_RUN_TRACE.  The
+ * photon records, queues and staging live in one static device array of 10.9 MB (10.4 MiB) that each
+ * GPU holds until the process ends; the context's tables, scene and grid are used as bound.  All
+ * calls on one GPU share that array, so calls that overlap on one GPU (from several threads or
+ * contexts) race; calls in sequence are independent. */
+enum { CC_NOALLOC_MAX_PHOTONS = 1 << 16 };
+int cc_transport_noalloc(void* ctx, int64_t n, double* pos, double* dir, double* energy, uint64_t seed,
+                         uint64_t first_history, uint32_t flags, int32_t* n_scatter, uint8_t* absorbed,
+                         uint8_t* escaped, cc_run_out* out);
+
 /* Geometry queries on the GPU (chatcarlo/geometry.py:205-231). */
 int cc_locate(void* ctx, int64_t n, const float* pts, int32_t* material);
 int cc_test_boundary(void* ctx, int64_t n, const float* o, const float* d, float eps, float* t,
