@@ -104,3 +104,66 @@ def test_bad_tuning_is_rejected(engine):
             b.run(t)
     with pytest.raises(ValueError):
         b.run(Tuning(), fraction=0.0)
+
+
+def _config4_picks():
+    """Three scenes of the 64-scene sweep: the one with the most primitives, one with lead (the
+    most lead primitives among the rest) and the one with the highest kVp among the rest."""
+    from chatcarlo_amd.workloads import batched_scenes
+    allsc = batched_scenes(64)
+    nprim = [len(x[0]["geometry"]) - 1 for x in allsc]
+    kvp = [max(b["energy_keV"] for b in x[0]["source"]["spectrum"]) for x in allsc]
+    lead = [sum(g["material"] == "lead" for g in x[0]["geometry"]) for x in allsc]
+    a = int(np.argmax(nprim))
+    b = max((i for i in range(64) if i != a), key=lambda i: (lead[i], nprim[i]))
+    c = max((i for i in range(64) if i not in (a, b)), key=lambda i: kvp[i])
+    assert lead[b] > 0 and kvp[c] > 145.0
+    return [allsc[i] for i in (a, b, c)]
+
+
+def test_sweep_scenes_on_their_2cm_grids_chi2_against_oracle(engine, dpack):
+    """The 64-scene sweep's scenes at their real grids (2 cm voxels, 150^3 over the 300 cm room):
+    three of them run batched, and kerma and H*(10) are compared voxel by voxel with the FP64
+    oracle by chi^2.  Batched runs keep totals only, so the GPU's variance comes from eight
+    independent repetitions (other seeds); the oracle tallies the scene grid's voxels inside the
+    box that holds the GPU's well-converged voxels, over several processes."""
+    import sys
+    sys.path.insert(0, __import__("os").path.dirname(__file__))
+    from test_gpu_transport import _box_around, _chi2_voxels
+    from chatcarlo_amd.batch import BatchedRun, SceneSpec, Tuning
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.source import SourceRecord
+    from oracle import core
+    from oracle.parallel import run_parallel
+    picks = _config4_picks()
+    reps, n_gpu = 8, 1_000_000
+    tot = [[[], []] for _ in picks]
+    for r in range(reps):
+        specs = [SceneSpec(raw, grid, n_gpu, (seed + 1_000_003 * r) % (1 << 62)) for raw, grid, _, seed in picks]
+        b = BatchedRun(specs, engine)
+        st = b.run(Tuning(128, 4, 1 << 20))
+        assert st.n_histories == n_gpu * len(picks)
+        for k in range(len(picks)):
+            kk, hh, _ = b.read(k)
+            tot[k][0].append(kk / n_gpu)
+            tot[k][1].append(hh / n_gpu)
+    for k, (raw, (origin, shape, vox), _, seed) in enumerate(picks):
+        geo = Geometry(raw["geometry"])
+        rec = SourceRecord(raw["source"])
+        gk = np.array(tot[k][0])
+        m1k = gk.mean(axis=0)
+        v1k = gk.var(axis=0, ddof=1) / reps
+        good = (m1k > 0) & (np.sqrt(v1k) < 0.01 * m1k)
+        lo, hi = _box_around(good, (64, 64, 64))
+        sub = tuple(slice(int(a), int(c)) for a, c in zip(lo, hi))
+        prims, world = geo.packed(dpack.mat_id)
+        n_or = 400_000
+        print(f"scene {k}: {len(raw['geometry']) - 1} primitives, box {lo}..{hi}, oracle {n_or}", flush=True)
+        _, grid = run_parallel(raw["source"], prims, world, dpack, rec.spec_e, rec.spec_cdf, n_or, seed % (1 << 32),
+                               2_000, grid_spec=(np.asarray(origin) + lo * vox, tuple(int(x) for x in hi - lo), vox))
+        for key, q, i in (("kerma", "kerma_q", 0), ("h10", "h10_q", 1)):
+            g = np.array(tot[k][i])[(slice(None),) + sub]
+            m1, v1 = g.mean(axis=0), g.var(axis=0, ddof=1) / reps
+            m2 = getattr(grid, key) / grid.n_histories
+            v2 = core.batch_variance(getattr(grid, key), getattr(grid, q), grid.n_batches, grid.n_histories) / grid.n_histories
+            _chi2_voxels(m1, v1, m2, v2, 200, f"sweep scene {k} {key}")

@@ -27,6 +27,38 @@ def _pencil(n, thickness, energy):
     return pos, dirv, np.full(n, float(energy))
 
 
+def _mean_var(T, Q, M, Nh):
+    """Per-history mean and its variance from batch moments (chatcarlo/tally.py:135-175)."""
+    return T / Nh, core.batch_variance(T, Q, M, Nh) / Nh
+
+
+def _chi2_voxels(m1, v1, m2, v2, min_dof, label):
+    """Per-voxel chi^2 between two runs over the voxels with R < 0.1 in both: chi^2 / dof within
+    3 sigma of its expectation (plus 0.05 for the inflation ~(M-1)/(M-3) of batch-variance
+    estimates with M batches), no voxel beyond 5.5 sigma."""
+    r1 = np.sqrt(v1) / np.where(m1 > 0, m1, np.inf)
+    r2 = np.sqrt(v2) / np.where(m2 > 0, m2, np.inf)
+    sel = (m1 > 0) & (m2 > 0) & (r1 < 0.1) & (r2 < 0.1)
+    dof = int(sel.sum())
+    assert dof > min_dof, (label, dof)
+    z = (m1[sel] - m2[sel]) / np.sqrt(v1[sel] + v2[sel])
+    chi2 = float(np.sum(z ** 2))
+    print(f"{label}: chi2/dof {chi2 / dof:.4f} over {dof} voxels, max |z| {np.max(np.abs(z)):.2f}")
+    assert abs(chi2 / dof - 1.0) < 3.0 * np.sqrt(2.0 / dof) + 0.05, (label, chi2, dof)
+    assert np.max(np.abs(z)) < 5.5, label
+
+
+def _compare_grids(g, grid, min_dof, label, sub=None):
+    """chi^2 of kerma and H*(10) between a GPU grid read (g) and an oracle Grid; sub = slices of
+    the GPU grid that the oracle grid covers."""
+    sub = sub or (slice(None),) * 3
+    for key, q in (("kerma", "kerma_q"), ("h10", "h10_q")):
+        gs = "kerma_sum2" if key == "kerma" else "h10_sum2"
+        m1, v1 = _mean_var(g[key][sub], g[gs][sub], g["n_batches"], g["n_histories"])
+        m2, v2 = _mean_var(getattr(grid, key), getattr(grid, q), grid.n_batches, grid.n_histories)
+        _chi2_voxels(m1, v1, m2, v2, min_dof, f"{label} {key}")
+
+
 @pytest.mark.parametrize("material,thickness,energy,seed", [("water", 10.0, 60.0, 1), ("lead", 0.2, 80.0, 2)])
 def test_beer_lambert(engine, material, thickness, energy, seed):
     from chatcarlo_amd.materials import linear_mu
@@ -119,7 +151,8 @@ def _config1_scene():
 
 
 def test_chi2_against_oracle(engine, dpack):
-    """Per-voxel chi^2 between the GPU run and the FP64 oracle run (water slab, 60 keV)."""
+    """Per-voxel chi^2 of kerma and H*(10) between the GPU run and the FP64 oracle run (water
+    slab, 60 keV, 1 cm grid over the slab)."""
     from chatcarlo_amd import _native as N
     from chatcarlo_amd.geometry import Geometry
     from chatcarlo_amd.scene import validate_scene
@@ -143,30 +176,59 @@ def test_chi2_against_oracle(engine, dpack):
     grid = OT.Grid(origin, shape, vox)
     n_or = 100_000
     r = OT.run(sc.raw["source"], og, dpack, rec.spec_e, rec.spec_cdf, n_or, 7, batch_size=2_000, grid=grid)
-
-    def mean_var(T, Q, M, Nh):
-        var = core.batch_variance(T, Q, M, Nh)
-        return T / Nh, var / Nh
-
-    m1, v1 = mean_var(g["kerma"], g["kerma_sum2"], g["n_batches"], g["n_histories"])
-    m2, v2 = mean_var(grid.kerma, grid.kerma_q, grid.n_batches, grid.n_histories)
-    r1 = np.sqrt(v1) / np.where(m1 > 0, m1, np.inf)
-    r2 = np.sqrt(v2) / np.where(m2 > 0, m2, np.inf)
-    sel = (m1 > 0) & (m2 > 0) & (r1 < 0.1) & (r2 < 0.1)
-    assert sel.sum() > 500
-    z = (m1[sel] - m2[sel]) / np.sqrt(v1[sel] + v2[sel])
-    chi2 = float(np.sum(z ** 2))
-    dof = int(sel.sum())
-    # chi^2 / dof within 3 sigma of its expectation (the variance-estimate inflation of
-    # ~(M-1)/(M-3) for M = 50 batches is inside the 0.05 allowance), no voxel beyond 5.5 sigma
-    assert abs(chi2 / dof - 1.0) < 3.0 * np.sqrt(2.0 / dof) + 0.05, (chi2, dof)
-    assert np.max(np.abs(z)) < 5.5
+    _compare_grids(g, grid, 500, "water slab")
     # scalar physics summary within statistics
     for key, k_or in (("absorbed", "n_absorbed"), ("escaped", "n_escaped")):
         p1 = getattr(out, "n_" + key) / n_gpu
         p2 = r[k_or] / n_or
         se = np.sqrt(p1 * (1 - p1) / n_gpu + p2 * (1 - p2) / n_or)
         assert abs(p1 - p2) < 5 * se, (key, p1, p2)
+
+
+def _box_around(mask, limit):
+    """Index box (start, stop per axis) around the True voxels of mask, at most `limit` wide."""
+    idx = np.argwhere(mask)
+    lo, hi = idx.min(axis=0), idx.max(axis=0) + 1
+    for k in range(3):
+        if hi[k] - lo[k] > limit[k]:
+            c = int(np.median(idx[:, k]))
+            lo[k] = max(0, c - limit[k] // 2)
+            hi[k] = min(mask.shape[k], lo[k] + limit[k])
+    return lo, hi
+
+
+def test_chest_room_chi2_against_oracle(engine, dpack):
+    """The benchmark workload (chest room, 80 kVp, 5 cm grid over the whole world box): per-voxel
+    chi^2 of kerma and H*(10) between the GPU and the FP64 oracle over the voxels where both runs
+    reach R < 0.1.  The oracle tallies the world grid's voxels inside the box that holds the
+    GPU's well-converged voxels (the same voxel boundaries); it runs over several processes."""
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.source import SourceRecord
+    from chatcarlo_amd.tally import VoxelGrid
+    from chatcarlo_amd.workloads import chest_room
+    from oracle.parallel import run_parallel
+    raw, _ = chest_room()
+    geo = Geometry(raw["geometry"])
+    vg = VoxelGrid.from_bbox(geo.bbox_min, geo.bbox_max, 5.0)
+    engine.set_geometry(geo)
+    engine.bind_grid(vg.origin_cm, vg.shape, vg.voxel_size_cm, True)
+    rec = SourceRecord(raw["source"])
+    n_gpu = 4_000_000
+    out = engine.run(rec, n_gpu, 2024, 0, 40_000, N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY)
+    assert out.error_bits == 0
+    g = engine.read_grid(vg.shape)
+    m, v = _mean_var(g["kerma"], g["kerma_sum2"], g["n_batches"], g["n_histories"])
+    good = (m > 0) & (np.sqrt(v) < 0.03 * m)
+    lo, hi = _box_around(good, (48, 80, 40))
+    sub = tuple(slice(int(a), int(b)) for a, b in zip(lo, hi))
+    origin = np.asarray(vg.origin_cm) + lo * vg.voxel_size_cm
+    prims, world = geo.packed(dpack.mat_id)
+    n_or = 400_000
+    _, grid = run_parallel(raw["source"], prims, world, dpack, rec.spec_e, rec.spec_cdf, n_or, 77, 4_000,
+                           grid_spec=(origin, tuple(int(x) for x in hi - lo), vg.voxel_size_cm))
+    assert grid.n_histories == n_or
+    _compare_grids(g, grid, 1000, "chest room", sub)
 
 
 def test_run_transport_api(engine):
@@ -256,44 +318,33 @@ def test_grazing_flights_do_not_creep(engine):
 
 
 def test_ct_helical_chi2_against_oracle(engine, dpack):
-    """CTDI phantom under a helical 120 kVp scan (rotating, shifting source): per-voxel chi^2 of the
-    kerma grid (2 cm voxels over the phantom) between the GPU run and the FP64 oracle run."""
+    """CTDI phantom under a helical 120 kVp scan (rotating, shifting source), on the workload's
+    0.5 cm grid over the phantom (72 x 72 x 38): per-voxel chi^2 of kerma and H*(10) between the
+    GPU run and the FP64 oracle run (over several processes)."""
     from chatcarlo_amd import _native as N
     from chatcarlo_amd.geometry import Geometry
     from chatcarlo_amd.scene import validate_scene
     from chatcarlo_amd.source import SourceRecord
     from chatcarlo_amd.workloads import ct_helical
-    raw, _ = ct_helical()
+    from oracle.parallel import run_parallel
+    raw, gs = ct_helical()
     sc = validate_scene(raw)
     assert sc.ok, sc.errors
     geo = Geometry(sc.raw["geometry"])
-    origin, shape, vox = np.array([-18.0, -18.0, -10.0]), (18, 18, 10), 2.0
+    origin, shape, vox = gs
+    assert vox == 0.5
     engine.set_geometry(geo)
     engine.bind_grid(origin, shape, vox, True)
     rec = SourceRecord(sc.raw["source"])
-    n_gpu = 1_000_000
-    out = engine.run(rec, n_gpu, 4242, 0, 10_000, N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY)
+    n_gpu = 10_000_000
+    out = engine.run(rec, n_gpu, 4242, 0, 100_000, N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY)
+    assert out.error_bits == 0
     g = engine.read_grid(shape)
     prims, world = geo.packed(dpack.mat_id)
-    og = core.OGeom.from_packed(prims, world)
-    grid = OT.Grid(origin, shape, vox)
-    n_or = 60_000
-    OT.run(sc.raw["source"], og, dpack, rec.spec_e, rec.spec_cdf, n_or, 17, batch_size=1_200, grid=grid)
-
-    def mean_var(T, Q, M, Nh):
-        return T / Nh, core.batch_variance(T, Q, M, Nh) / Nh
-
-    m1, v1 = mean_var(g["kerma"], g["kerma_sum2"], g["n_batches"], g["n_histories"])
-    m2, v2 = mean_var(grid.kerma, grid.kerma_q, grid.n_batches, grid.n_histories)
-    r1 = np.sqrt(v1) / np.where(m1 > 0, m1, np.inf)
-    r2 = np.sqrt(v2) / np.where(m2 > 0, m2, np.inf)
-    sel = (m1 > 0) & (m2 > 0) & (r1 < 0.1) & (r2 < 0.1)
-    assert sel.sum() > 300
-    z = (m1[sel] - m2[sel]) / np.sqrt(v1[sel] + v2[sel])
-    chi2, dof = float(np.sum(z ** 2)), int(sel.sum())
-    assert abs(chi2 / dof - 1.0) < 3.0 * np.sqrt(2.0 / dof) + 0.05, (chi2, dof)
-    assert np.max(np.abs(z)) < 5.5
-    assert out.error_bits == 0
+    n_or = 800_000
+    _, grid = run_parallel(sc.raw["source"], prims, world, dpack, rec.spec_e, rec.spec_cdf, n_or, 17, 2_000,
+                           grid_spec=(origin, shape, vox))
+    _compare_grids(g, grid, 5000, "ct helical 0.5 cm")
 
 
 @pytest.mark.parametrize("workload", ["chest_room", "ct_helical"])
