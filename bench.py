@@ -57,6 +57,9 @@ ATOMIC_CAL_SLOTS = 16 * 108 * 108 * 88
 ATOMIC_CAL_OPS = 64 << 21
 
 
+REF_BATCH = 200_000   # run_transport's default batch_size (chatcarlo/transport.py:359)
+
+
 def _cpu_sample(args):
     """Time the oracle's reference-style NumPy transport on a bounded sample (one process)."""
     raw, n, seed, min_seconds = args
@@ -77,8 +80,9 @@ def _cpu_sample(args):
     t0 = time.perf_counter()
     k = 0
     # min_seconds > 0: stop at n histories or once min_seconds have passed; 0: run all n
+    # batches of the reference's default size (chatcarlo/transport.py:359), one OT.run call each
     while done < n and (min_seconds <= 0 or time.perf_counter() - t0 < min_seconds):
-        m = min(10_000, n - done)
+        m = min(REF_BATCH, n - done)
         OT.run(raw["source"], og, pack, e, cdf, m, seed + k, batch_size=m, grid=grid, exact_tally=False)
         done += m
         k += 1
@@ -87,14 +91,15 @@ def _cpu_sample(args):
 
 def cpu_baseline(raw, all_cores):
     import multiprocessing as mp
-    done, dt = _cpu_sample((raw, 400_000, 11, 12.0))
+    done, dt = _cpu_sample((raw, 2 * REF_BATCH, 11, 12.0))
     out = {"value": done / dt, "unit": "histories/s", "cores": 1, "kind": "port",
-           "sample": f"{done} histories of the same workload (oracle NumPy restatement of the "
-                     f"reference transport, stratified track-length tally), one process, {dt:.1f} s"}
+           "sample": f"{done} histories of the same workload in batches of {REF_BATCH} (oracle NumPy "
+                     f"restatement of the reference transport, stratified track-length tally, 5 cm grid), "
+                     f"one process, {dt:.1f} s"}
     if all_cores:
         nproc = max(1, min(os.cpu_count() or 1, 16))
         ctx = mp.get_context("spawn")
-        per = max(10_000, done // 2)
+        per = REF_BATCH
         with ctx.Pool(nproc) as pool:
             t0 = time.perf_counter()
             res = pool.map(_cpu_sample, [(raw, per, 100 + i, 0.0) for i in range(nproc)])
@@ -102,9 +107,34 @@ def cpu_baseline(raw, all_cores):
         tot = sum(r[0] for r in res)
         slowest = max(r[1] for r in res)
         out["all_cores"] = {"value": tot / slowest, "cores": nproc,
-                            "sample": f"{tot} histories over {nproc} processes ({per} each), slowest process "
-                                      f"{slowest:.1f} s of transport, {dt2:.1f} s wall incl. start-up"}
+                            "sample": f"{tot} histories over {nproc} processes (one batch of {per} each), "
+                                      f"slowest process {slowest:.1f} s of transport, {dt2:.1f} s wall incl. start-up"}
     return out
+
+
+def ct_helical_run(eng, n_histories=100_000_000):
+    """The CT helical workload at its own size: 1e8 histories of the 120 kVp helical scan on the
+    CTDI phantom, kerma and H*(10) on the 0.5 cm grid over the phantom, batch moments, grid read
+    back (one warm-up run of 1e7 first)."""
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.source import SourceRecord
+    from chatcarlo_amd.workloads import ct_helical
+    raw, (origin, shape, vox) = ct_helical()
+    eng.set_geometry(Geometry(raw["geometry"]))
+    rec = SourceRecord(raw["source"])
+    flags = N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY
+    eng.bind_grid(origin, shape, vox, True)
+    eng.run(rec, 10_000_000, 5, 0, 1_000_000, flags)
+    eng.bind_grid(origin, shape, vox, True)
+    t0 = time.perf_counter()
+    o = eng.run(rec, n_histories, 6, 0, 1_000_000, flags)
+    eng.read_grid(shape, moments=True)
+    dt = time.perf_counter() - t0
+    return {"workload": "ct_helical_120kVp_6mmAl_pmma_ctdi_phantom_0.5cm_grid", "histories": n_histories,
+            "grid": list(shape), "voxel_cm": vox, "seconds": dt, "histories_per_s": n_histories / dt,
+            "voxel_visits_per_history": o.n_voxel_visits / n_histories,
+            "flights_per_history": o.n_flights / n_histories}
 
 
 def learned_fom(eng, target=0.02, pairs=5):
@@ -121,7 +151,7 @@ def learned_fom(eng, target=0.02, pairs=5):
     t0 = time.perf_counter()
     n_rec, _ = li.generate_records(10_000_000, 32_000_000, keep=0.3, seed=61)
     t1 = time.perf_counter()
-    tr = li.train(0, steps=12000, batch=8192, lr=3e-3, lr_end=1e-4)
+    tr = li.train(0, steps=1500, batch=65536, lr=8e-3, lr_end=1e-4)
     t2 = time.perf_counter()
     li.configure()
     runs = []
@@ -143,7 +173,7 @@ def learned_fom(eng, target=0.02, pairs=5):
             "train_device_ms": tr.ms, "train_loss_first_last": [tr.loss_first, tr.loss_last]}
 
 
-def batched_sweep(eng, autotune_fraction=0.02, sequential=True):
+def batched_sweep(eng, autotune_fraction=0.01, sequential=True):
     """64 random scenes (1-40 primitives each, 50-150 kVp, 1e5-1e7 histories, 2 cm grids) in one
     particle pool: launch shape autotuned on a fraction of the histories, then the whole sweep;
     beside it the same sweep scene by scene (one single-scene run per scene, as a caller of
@@ -159,7 +189,7 @@ def batched_sweep(eng, autotune_fraction=0.02, sequential=True):
     t_up = time.perf_counter() - t0
     b.run(fraction=0.005)   # warm-up (code objects, pool allocation)
     t0 = time.perf_counter()
-    tune, table = b.autotune(autotune_fraction)
+    tune, table = b.autotune(autotune_fraction, cached=False)
     t_tune = time.perf_counter() - t0
     t0 = time.perf_counter()
     st = b.run(tune)
@@ -173,6 +203,8 @@ def batched_sweep(eng, autotune_fraction=0.02, sequential=True):
            "voxel_visits_per_history": st.n_voxel_visits / st.n_histories,
            "flights_per_history": st.n_flights / st.n_histories,
            "upload_s": t_up, "autotune_s": t_tune, "autotune_fraction": autotune_fraction,
+           "autotune_share_of_sweep": t_tune / wall, "wall_s_with_autotune": wall + t_tune,
+           "histories_per_s_with_autotune": st.n_histories / (wall + t_tune),
            "autotune_table": [[int(r[0]), int(r[1]), int(r[2]), round(float(r[3]))] for r in table]}
     if sequential:
         kerma_b = [float(b.read(k)[0].sum()) for k in range(len(scenes))]
@@ -205,6 +237,7 @@ def main():
     ap.add_argument("--learned-only", action="store_true",
                     help="only the learned-importance comparison (prints its JSON object)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched sweep")
+    ap.add_argument("--no-ct", action="store_true", help="skip the 1e8-history CT helical run")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the 64-scene batched sweep (prints its JSON object)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -261,9 +294,15 @@ def main():
     n = int(args.histories)
     flags = N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY | N.RUN_TIMING
 
-    def step(k):
+    def step(k, readback=False):
+        # one step: reset the tallies and transport (inputs and tallies resident in HBM); with
+        # readback also the grid and its moments copied to the host, as run_transport returns them
+        # (chatcarlo/transport.py:441-474) -- the PCIe-inclusive rate, reported beside the value
         eng.bind_grid(vg.origin_cm, vg.shape, vg.voxel_size_cm, True)
-        return eng.run(rec, n, 1000 + k, 0, args.batch, flags)
+        o = eng.run(rec, n, 1000 + k, 0, args.batch, flags)
+        if readback:
+            eng.read_grid(vg.shape, moments=True)
+        return o
 
     for k in range(args.warmup):
         step(k)
@@ -274,6 +313,9 @@ def main():
     dt = time.perf_counter() - t0
     hist = n * args.steps
     value = hist / dt
+    t0 = time.perf_counter()
+    step(args.warmup + args.steps, readback=True)
+    dt_rb = time.perf_counter() - t0
 
     flights = sum(o.n_flights for o in outs)
     visits = sum(o.n_voxel_visits for o in outs)
@@ -302,6 +344,7 @@ def main():
                    "batch": args.batch, "grid": list(vg.shape), "voxel_cm": 5.0,
                    "tallies": "kerma + H*(10) track-length, batch moments", "fluorescence": True},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "kernel_bound": "fp64 atomic request rate beyond the L2s (atomic_roofline), not HBM bandwidth",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_flight"),
                      "kernel": "k_flight",
                      "algorithmic_bytes_per_launch": f_bytes / max(f_launch, 1),
@@ -317,9 +360,13 @@ def main():
                  "n_vox_per_history": n_vox, "bytes_per_history": b_hist,
                  "path_gbs": value * b_hist / 1e9, "path_frac": value * b_hist / 1e9 / HBM_PEAK_GBS,
                  "kernel_time_share": kernel_share, "rounds_max": int(o.rounds_max)},
+        "with_grid_readback": {"histories_per_s": n / dt_rb, "ms_per_step": dt_rb * 1000.0,
+                               "readback_bytes": int(np.prod(vg.shape)) * 36},
         "physics": {"fraction_absorbed": o.n_absorbed / n, "fraction_escaped": o.n_escaped / n,
                     "mean_interactions": o.scatter_sum / n, "n_fluorescence": int(o.n_fluorescence)},
     }
+    if not args.no_ct:
+        line["ct_helical"] = ct_helical_run(eng)
     if not args.no_batched:
         line["batched_sweep"] = batched_sweep(eng)
     if not args.no_learned:

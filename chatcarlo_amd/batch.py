@@ -130,16 +130,39 @@ class BatchedRun:
                                int(out.graph_launches), float(out.device_ms), t)
         return self.last
 
-    def autotune(self, fraction: float = 0.05):
-        """Times every candidate launch shape on a fraction of the histories; returns the
-        fastest and the table (block, rounds_per_sync, pool, histories/s) of all of them."""
+    # launch shapes found by autotune, per workload signature (scenes, histories, seeds, grids,
+    # fraction, flags): a sweep that comes back is not timed again
+    _tuned: dict = {}
+
+    def signature(self, fraction: float) -> str:
+        import hashlib
+        import json
+        h = hashlib.sha256()
+        for s in self.scenes:
+            h.update(json.dumps(s.raw, sort_keys=True, default=float).encode())
+            h.update(repr((int(s.n_histories), int(s.seed), [float(x) for x in np.ravel(s.grid[0])],
+                           [int(x) for x in s.grid[1]], float(s.grid[2]))).encode())
+        h.update(repr((float(fraction), int(self.flags))).encode())
+        return h.hexdigest()
+
+    def autotune(self, fraction: float = 0.02, cached: bool = True):
+        """Times launch shapes on a fraction of the histories (a six-run coordinate search,
+        cc_batch_autotune); returns the fastest and the table (block, rounds_per_sync, pool,
+        histories/s) of the runs.  With cached, a workload tuned before in this process returns
+        its shape and table without timing."""
+        key = self.signature(fraction)
+        if cached and key in BatchedRun._tuned:
+            t, table = BatchedRun._tuned[key]
+            return Tuning(t.block, t.rounds_per_sync, t.pool, t.tally_copies), table.copy()
         best = N.CcBatchTuning()
         table = np.zeros((64, 4))
         rows = C.c_int32()
         N.check(self.eng.lib.cc_batch_autotune(self.eng.ctx, float(fraction), self.flags, C.byref(best),
                                                N.ptr(table, C.c_double), 64, C.byref(rows)))
-        return (Tuning(best.block, best.rounds_per_sync, best.pool, best.tally_copies),
-                table[:min(rows.value, 64)].copy())
+        tune = Tuning(best.block, best.rounds_per_sync, best.pool, best.tally_copies)
+        table = table[:min(rows.value, 64)].copy()
+        BatchedRun._tuned[key] = (tune, table)
+        return tune, table
 
     def histories(self, k: int) -> int:
         """Histories of scene k in the last run: ceil(fraction n_k), at least 1."""

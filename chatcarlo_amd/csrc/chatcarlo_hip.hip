@@ -1737,6 +1737,7 @@ __device__ __forceinline__ void lds_gemm_bf16_atomic(int M, int N, int K, const 
 // (16 floats each, with y) or from the record buffer through the affine permutation
 // idx = (a k + b) mod n_rec.
 constexpr int kGR = 32;
+constexpr int kGradBlocks = 256;
 __global__ __launch_bounds__(kBlock) void k_mlp_grad(const float* w, const uint16_t* wb, int n, const float* feat,
                                                      const float* yv, const float* rec, uint32_t n_rec,
                                                      uint64_t pa, uint64_t pb, uint32_t k0, MlpFrame frame,
@@ -1745,6 +1746,10 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(const float* w, const uint1
   __shared__ uint16_t sX[kGR * kMlpIn], sW1[kMlpIn * kMlpH], sW2[kMlpH * kMlpH], sH1[kGR * kMlpH];
   __shared__ uint16_t sD2[kGR * kMlpH], sD1[kGR * kMlpH];
   __shared__ float sZ1[kGR * kMlpH], sZ2[kGR * kMlpH], sOut[kGR], sG[kGR], sY[kGR];
+  // the block's gradient over all of its tiles, added into grad once at the end: one global atomic
+  // per parameter and block instead of one per parameter and tile
+  __shared__ float sGrad[kMlpParams];
+  for (int k = threadIdx.x; k < kMlpParams; k += blockDim.x) sGrad[k] = 0.f;
   MlpTile tile{sX, sW1, sW2, sZ1, sH1, sZ2};
   mlp_stage_weights(wb, wb + kMlpIn * kMlpH, tile);
   const float* b2 = w + kOffB2;
@@ -1793,13 +1798,13 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(const float* w, const uint1
         const float d2 = z > 0.f ? sG[rr] * w3[jj] : 0.f;
         s2 += d2;
       }
-      atomicAdd(grad + kOffW3 + jj, s3);
-      atomicAdd(grad + kOffB2 + jj, s2);
+      sGrad[kOffW3 + jj] += s3;   // one thread per hidden unit: no other writer
+      sGrad[kOffB2 + jj] += s2;
     }
     if (threadIdx.x == 64) {
       float s = 0.f;
       for (int rr = 0; rr < kGR; ++rr) s += sG[rr];
-      atomicAdd(grad + kOffB3, s);
+      sGrad[kOffB3] += s;
     }
     for (int k = threadIdx.x; k < kGR * kMlpH; k += blockDim.x) {
       const int rr = k / kMlpH, jj = k % kMlpH;
@@ -1807,16 +1812,18 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(const float* w, const uint1
     }
     __syncthreads();
     // dW2 += H1^T D2  (M = 64 hidden1, N = 64 hidden2, K = rows)
-    lds_gemm_bf16_atomic(kMlpH, kMlpH, kGR, sH1, 1, kMlpH, sD2, kMlpH, 1, grad + kOffW2, kMlpH);
+    lds_gemm_bf16_atomic(kMlpH, kMlpH, kGR, sH1, 1, kMlpH, sD2, kMlpH, 1, sGrad + kOffW2, kMlpH);
     // D1 = (D2 W2^T) 1[Z1 > 0]  (M = rows, N = 64 hidden1, K = 64 hidden2), into sZ1
     lds_gemm_bf16(kGR, kMlpH, kMlpH, sD2, kMlpH, 1, sW2, 1, kMlpH, sZ1, kMlpH, false);
     __syncthreads();
     for (int k = threadIdx.x; k < kGR * kMlpH; k += blockDim.x) sD1[k] = f2bf(bf2f(sH1[k]) > 0.f ? sZ1[k] : 0.f);
     __syncthreads();
     // dW1 += X^T D1  (M = 16 inputs, N = 64, K = rows)
-    lds_gemm_bf16_atomic(kMlpIn, kMlpH, kGR, sX, 1, kMlpIn, sD1, kMlpH, 1, grad + kOffW1, kMlpH);
+    lds_gemm_bf16_atomic(kMlpIn, kMlpH, kGR, sX, 1, kMlpIn, sD1, kMlpH, 1, sGrad + kOffW1, kMlpH);
     __syncthreads();
   }
+  for (int k = threadIdx.x; k < kMlpParams; k += blockDim.x)
+    if (sGrad[k] != 0.f) atomicAdd(grad + k, sGrad[k]);
   // block loss
   __shared__ double sL[kBlock];
   sL[threadIdx.x] = lsum;
@@ -3860,7 +3867,9 @@ int cc_mlp_train(void* ctx, int32_t net, const cc_train_cfg* cfg, cc_train_out* 
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   HIP_TRY(hipEventRecord(e0, c->stream));
-  const int gb = (int)std::min<uint32_t>((cfg->batch + kGR - 1) / kGR, 4096);
+  // at most kGradBlocks blocks: a larger minibatch gives each block more tiles, not more blocks,
+  // so the global gradient atomics per step stay at kGradBlocks x kMlpParams
+  const int gb = (int)std::min<uint32_t>((cfg->batch + kGR - 1) / kGR, kGradBlocks);
   const float gscale = 1.0f / (float)cfg->batch;
   double b1t = std::pow((double)cfg->beta1, (double)c->adam_t[net]);
   double b2t = std::pow((double)cfg->beta2, (double)c->adam_t[net]);
@@ -4050,8 +4059,18 @@ int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int
   return CC_OK;
 }
 
+// tally = false (autotuner timing runs): the scene tallies are neither cleared before nor folded
+// after the run; the transport kernels add into them as in a real run, so the timing is the same,
+// without the two passes over the tally copies (27 GB for the 64-scene workload) per candidate.
+static int batch_run(Ctx* c, const cc_batch_tuning* tune, double fraction, uint32_t flags, cc_batch_out* out,
+                     bool tally);
+
 int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32_t flags, cc_batch_out* out) {
-  Ctx* c = (Ctx*)ctx;
+  return batch_run((Ctx*)ctx, tune, fraction, flags, out, true);
+}
+
+static int batch_run(Ctx* c, const cc_batch_tuning* tune, double fraction, uint32_t flags, cc_batch_out* out,
+                     bool tally) {
   if (!c || !tune || !out) return fail(CC_ERR_ARG, "null argument");
   if (c->nscene <= 0) return fail(CC_ERR_STATE, "no scenes uploaded (cc_batch_upload)");
   if (tune->block < 64 || tune->block > kBlock || tune->block % kWave)
@@ -4097,8 +4116,9 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
   int r = ensure_queue(c, pool);
   if (r) return r;
   // fresh tallies, deposits, counters, free slots and queue
-  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks((size_t)ncopy * 2 * c->b_nvox)), dim3(kBlock), 0, c->stream, c->b_tally,
-                     (size_t)ncopy * 2 * c->b_nvox, 0.0);
+  if (tally)
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks((size_t)ncopy * 2 * c->b_nvox)), dim3(kBlock), 0, c->stream,
+                       c->b_tally, (size_t)ncopy * 2 * c->b_nvox, 0.0);
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(kAccCopies * (size_t)n * CC_MAX_MAT)), dim3(kBlock), 0, c->stream,
                      c->b_edep, kAccCopies * (size_t)n * CC_MAX_MAT, 0.0);
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, c->stats, c->edep, (int)C_ACT_A, 0u);
@@ -4176,7 +4196,7 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
   (void)hipGraphExecDestroy(exec);
   (void)hipGraphDestroy(graph);
   if (rc) return rc;
-  if (ncopy > 1) {
+  if (tally && ncopy > 1) {
     hipLaunchKernelGGL(k_fold_copies, dim3(grid_blocks(2 * c->b_nvox)), dim3(kBlock), 0, c->stream, c->b_tally,
                        2 * c->b_nvox, (int)ncopy, 0);
     HIP_TRY(hipGetLastError());
@@ -4230,29 +4250,59 @@ int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tunin
                       int32_t max_rows, int32_t* n_rows) {
   Ctx* c = (Ctx*)ctx;
   if (!c || !best) return fail(CC_ERR_ARG, "null argument");
+  // Coordinate search from (block 128, 8 rounds per sync, pool 2^20): the pool first (it moves
+  // the rate most), then the block size, then rounds per sync, each axis at the best of the
+  // axes before.  Six timing runs instead of all 24 shapes; timing runs skip the tally clear and
+  // fold, which do not depend on the shape.  The tallies are left cleared, as after an upload.
+  const uint32_t pools[] = {1u << 19, 1u << 20, 1u << 21};
   const int blocks[] = {128, 256};
-  const uint32_t pools[] = {1u << 18, 1u << 19, 1u << 20, 1u << 21};
-  const int rps[] = {4, 8, 16};
+  const int rps[] = {8, 4, 16};
+  cc_batch_tuning cur{128, 8, 1u << 20, 8u};
   double best_rate = -1.0;
   int rows = 0;
-  for (int b : blocks)
-    for (uint32_t p : pools)
-      for (int q : rps) {
-        cc_batch_tuning t{b, q, p, 8u};
-        cc_batch_out o;
-        int r = cc_batch_run(ctx, &t, fraction, flags, &o);
-        if (r) return r;
-        const double rate = o.device_ms > 0 ? (double)o.n_histories / (o.device_ms * 1e-3) : 0.0;
-        if (table && rows < max_rows) {
-          table[4 * rows] = b; table[4 * rows + 1] = q; table[4 * rows + 2] = p; table[4 * rows + 3] = rate;
-        }
-        ++rows;
-        if (rate > best_rate) {
-          best_rate = rate;
-          *best = t;
-        }
-      }
+  auto trial = [&](const cc_batch_tuning& t) -> int {
+    cc_batch_out o;
+    int r = batch_run(c, &t, fraction, flags, &o, false);
+    if (r) return r;
+    const double rate = o.device_ms > 0 ? (double)o.n_histories / (o.device_ms * 1e-3) : 0.0;
+    if (table && rows < max_rows) {
+      table[4 * rows] = t.block; table[4 * rows + 1] = t.rounds_per_sync; table[4 * rows + 2] = t.pool;
+      table[4 * rows + 3] = rate;
+    }
+    ++rows;
+    if (rate > best_rate) {
+      best_rate = rate;
+      *best = t;
+    }
+    return CC_OK;
+  };
+  int r;
+  for (uint32_t p : pools) {
+    cc_batch_tuning t = cur;
+    t.pool = p;
+    if ((r = trial(t))) return r;
+  }
+  cur = *best;
+  for (int b : blocks) {
+    if (b == cur.block) continue;
+    cc_batch_tuning t = cur;
+    t.block = b;
+    if ((r = trial(t))) return r;
+  }
+  cur = *best;
+  for (int q : rps) {
+    if (q == cur.rounds_per_sync) continue;
+    cc_batch_tuning t = cur;
+    t.rounds_per_sync = q;
+    if ((r = trial(t))) return r;
+  }
   if (n_rows) *n_rows = rows;
+  // leave the scene tallies as a fresh upload leaves them
+  const size_t nd = (size_t)(best->tally_copies ? best->tally_copies : 1u) * 2 * c->b_nvox;
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(std::min(nd, c->b_tally_cap))), dim3(kBlock), 0, c->stream,
+                     c->b_tally, std::min(nd, c->b_tally_cap), 0.0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return CC_OK;
 }
 

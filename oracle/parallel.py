@@ -37,11 +37,13 @@ def default_processes() -> int:
 
 
 def run_parallel(src, prims, world, pack, spec_e, spec_cdf, n_histories, seed, batch_size, grid_spec=None,
-                 fluorescence=True, processes=None):
+                 fluorescence=True, processes=None, per_process=False):
     """oracle.transport.run over `processes` processes (each a whole number of batches).
 
     grid_spec = (origin, shape, voxel) or None.  Returns (run summary, merged Grid or None); the
-    summary adds the counters, deposits and region sums of every process."""
+    summary adds the counters, deposits and region sums of every process.  per_process: also
+    returns the list of each process's own result (independent runs, for spreads of derived
+    quantities) as a third element."""
     procs = processes or default_processes()
     per = -(-int(n_histories) // procs)
     per = -(-per // batch_size) * batch_size
@@ -79,4 +81,6 @@ def run_parallel(src, prims, world, pack, spec_e, spec_cdf, n_histories, seed, b
             grid.hits += g["hits"]
             grid.n_batches += g["n_batches"]
             grid.n_histories += g["n_histories"]
+    if per_process:
+        return out, grid, res
     return out, grid

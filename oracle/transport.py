@@ -59,12 +59,18 @@ def _draw_gen(rng):
     return lambda idx, slot: rng.random(len(idx))
 
 
+EVENTS = ("boundary", "escape", "photoelectric", "fluorescence", "compton", "rayleigh")
+
+
 def transport_photons(pos, dirv, energy, geom, pack, rng, grid=None, fluorescence=True,
-                      exact_tally=True, tally_rng=None, k_trials=100000, region=None):
+                      exact_tally=True, tally_rng=None, k_trials=100000, region=None, recorder=None):
     """chatcarlo/transport.py:98-233 on the oracle's geometry and pack.
 
     region = (centre, radius): adds the per-history H*(10) track-length score
-    in that sphere, h*(10)(E) x chord / volume, as result["region_score"]."""
+    in that sphere, h*(10)(E) x chord / volume, as result["region_score"].
+    recorder: called once per round with (photon ids, starts, ends, energies, events, materials),
+    one segment per photon still alive, as the reference's loop records them
+    (chatcarlo/transport.py:219-229); events index EVENTS."""
     if grid is not None and not exact_tally and tally_rng is None:
         tally_rng = rng.spawn(1)[0]
     n = pos.shape[0]
@@ -107,6 +113,9 @@ def transport_photons(pos, dirv, energy, geom, pack, rng, grid=None, fluorescenc
             else:
                 core.stratified_tally(grid.kerma, grid.origin, grid.voxel, o, d, ds, wk, tally_rng)
                 core.stratified_tally(grid.h10, grid.origin, grid.voxel, o, d, ds, wh, tally_rng)
+        if recorder is not None:
+            ev = np.where(inter, 0, np.where(esc, 1, 0))   # interaction types filled in below
+            seg = (idx.copy(), o.copy(), ends.copy(), e.copy(), ev, mat.copy())
         pos[idx] = ends
         non = ~inter
         g = idx[non]
@@ -117,10 +126,14 @@ def transport_photons(pos, dirv, energy, geom, pack, rng, grid=None, fluorescenc
         escaped[en] = True
         ii = idx[inter]
         if len(ii) == 0:
+            if recorder is not None:
+                recorder(*seg)
             continue
         mat_i, e_i = mat[inter], e[inter]
         r = rng.random(len(ii))
         typ = core.interaction_type(xs["p_photo"][inter], xs["p_compt"][inter], r)
+        if recorder is not None:
+            seg[4][inter] = np.where(typ == 0, 2, np.where(typ == 1, 4, 5))
         ph = typ == 0
         if np.any(ph):
             pidx = ii[ph]
@@ -135,6 +148,9 @@ def transport_photons(pos, dirv, energy, geom, pack, rng, grid=None, fluorescenc
             n_scatter[pidx] += 1
             alive[pidx[~emit]] = False
             absorbed[pidx[~emit]] = True
+            if recorder is not None:
+                pos_in = np.flatnonzero(inter)[ph]
+                seg[4][pos_in[emit]] = 3
             ei = pidx[emit]
             if len(ei):
                 n_fl += len(ei)
@@ -161,6 +177,8 @@ def transport_photons(pos, dirv, energy, geom, pack, rng, grid=None, fluorescenc
             dirv[ridx] = core.rotate_u(dirv[ridx], cos, rng.random(len(ridx)))
             tau[ridx] = -np.log(rng.random(len(ridx)))
             n_scatter[ridx] += 1
+        if recorder is not None:
+            recorder(*seg)
     return dict(n_scatter=n_scatter, absorbed=absorbed, escaped=escaped, final_energy=energy,
                 energy_deposited=edep, n_fluorescence=n_fl, rounds=rounds, n_mat=n_mat,
                 region_score=rscore, n_segments=n_seg, n_voxel_pieces=n_vox)

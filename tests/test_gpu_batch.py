@@ -86,14 +86,27 @@ def test_fraction_and_rerun_start_from_zero(engine, single):
     np.testing.assert_allclose(res[0].grid.kerma_keV, ref[0]["kerma"], rtol=1e-9, atol=1e-12 * ref[0]["kerma"].max())
 
 
-def test_autotune_returns_a_candidate(engine):
-    from chatcarlo_amd.batch import BatchedRun
+def test_autotune_returns_a_candidate(engine, single):
+    from chatcarlo_amd.batch import BatchedRun, Tuning
     b = BatchedRun(_scenes(3, seed=5), engine)
-    t, table = b.autotune(fraction=0.2)
-    assert len(table) == 24 and np.all(table[:, 3] > 0)
+    t, table = b.autotune(fraction=0.2, cached=False)
+    # the coordinate search: three pools at (128, 8), the other block, two other rounds_per_sync
+    assert len(table) == 6 and np.all(table[:, 3] > 0)
+    assert sorted(int(p) for p in table[:3, 2]) == [1 << 19, 1 << 20, 1 << 21]
     assert (t.block, t.rounds_per_sync, t.pool) in {(int(a), int(q), int(p)) for a, q, p, _ in table}
     best = table[np.argmax(table[:, 3])]
     assert (t.block, t.rounds_per_sync, t.pool) == (int(best[0]), int(best[1]), int(best[2]))
+    # the same workload again: the cached shape, no timing
+    t2, table2 = b.autotune(fraction=0.2)
+    assert t2 == t and np.array_equal(table2, table)
+    # the timing runs leave no tallies behind: a run after tuning equals the single-scene runs
+    scenes, ref = single
+    b = BatchedRun(scenes, engine)
+    b.autotune(fraction=0.1, cached=False)
+    b.run(Tuning(128, 4, 8192))
+    for k, r in enumerate(ref):
+        kk, _, _ = b.read(k)
+        np.testing.assert_allclose(kk, r["kerma"], rtol=1e-9, atol=1e-12 * r["kerma"].max())
 
 
 def test_bad_tuning_is_rejected(engine):

@@ -59,3 +59,49 @@ def test_run_transport_calibrates_and_cli_writes_absolute_maps(tmp_path):
     assert "dose_Gy" in z and "h10_pSv" in z
     scale = z["dose_Gy"].max() / z["dose_per_history_Gy"].max()
     assert scale == pytest.approx(r.n_photons_real, rel=0.15)
+
+
+def test_ctdi_matches_the_oracle_procedure(dpack):
+    """CTDIw, D_center and D_periphery per history on the GPU against the same procedure on the
+    FP64 oracle's transport (the phantom, the 1 cm grid, the hole and ring voxels of ctdi.py),
+    within statistics: eight GPU runs of 4e5 histories and the oracle over 16 independent
+    processes give the spreads; the oracle's dose uses its own point location and the pack's
+    densities."""
+    from chatcarlo_amd.ctdi import (CHAMBER_LENGTH_CM, GRID_RESOLUTION_CM, PHANTOM_DIAMETER_CM,
+                                    PHANTOM_LENGTH_CM, ctdi_from_dose, ctdi_per_history_Gy,
+                                    ctdi_phantom_geometry)
+    from chatcarlo_amd.source import SourceRecord
+    from chatcarlo_amd.tally import VoxelGrid
+    from oracle import core
+    from oracle.parallel import run_parallel
+    assert CHAMBER_LENGTH_CM == 10.0
+    gpu = np.array([ctdi_per_history_Gy(_SRC, phantom="body", n_histories=400_000, seed=900 + k)
+                    for k in range(8)])
+    rot = _SRC["rotation"]
+    geo = ctdi_phantom_geometry(rot, "body")
+    radius = PHANTOM_DIAMETER_CM["body"] / 2.0
+    iso = np.asarray(rot["isocenter"], dtype=float)
+    ax = 1
+    lo, hi = iso - radius - 2.0, iso + radius + 2.0
+    lo[ax] = iso[ax] - PHANTOM_LENGTH_CM / 2.0 - 2.0
+    hi[ax] = iso[ax] + PHANTOM_LENGTH_CM / 2.0 + 2.0
+    grid = VoxelGrid.from_bbox(lo, hi, GRID_RESOLUTION_CM)
+    prims, world = geo.packed(dpack.mat_id)
+    og = core.OGeom.from_packed(prims, world)
+    rho = dpack.density[og.material_at(grid.voxel_centers())].reshape(grid.shape)
+    rec = SourceRecord(_SRC)
+    procs = 16
+    _, _, parts = run_parallel(_SRC, prims, world, dpack, rec.spec_e, rec.spec_cdf, procs * 25_000, 4321, 5_000,
+                               grid_spec=(grid.origin_cm, grid.shape, grid.voxel_size_cm), processes=procs,
+                               per_process=True)
+    orc = []
+    for p in parts:
+        dose = p["grid"]["kerma"] * 1.602176634e-13 / (rho * grid.voxel_volume_cm3()) / p["grid"]["n_histories"]
+        orc.append(ctdi_from_dose(dose, grid, iso, ax, radius))
+    orc = np.array(orc)
+    for j, name in enumerate(("CTDIw", "D_center", "D_periphery")):
+        m1, s1 = gpu[:, j].mean(), gpu[:, j].std(ddof=1) / np.sqrt(len(gpu))
+        m2, s2 = orc[:, j].mean(), orc[:, j].std(ddof=1) / np.sqrt(len(orc))
+        z = (m1 - m2) / np.hypot(s1, s2)
+        print(f"{name}: gpu {m1:.5g} +- {s1:.2g}, oracle {m2:.5g} +- {s2:.2g}, z={z:+.2f}")
+        assert abs(z) < 4.0, name

@@ -306,19 +306,21 @@ def test_adam_step_matches_oracle(engine):
 
 def test_learned_importance_is_unbiased_and_beats_analog(engine):
     """Trained map on the angio room: every learned run agrees with its analog run within 3.5 sigma,
-    and the median figure of merit 1/(R^2 t) over five seed pairs, at the same relative error, is
-    well above the analog one.  Single pairs spread by a factor of a few (heavy-tailed weights; one
-    run gave 10.9, 4.8, 4.8 on three pairs), so the bar is on the median of five."""
+    and the median figure of merit 1/(R^2 t) over nine seed pairs, at the same relative error, is at
+    least ten times the analog one (the learned-importance target, training excluded).  Single
+    pairs spread by a factor of a few (heavy-tailed weights: 4.8 to 17.6 in the pairs measured so
+    far, about one in seven below 10), so the bar is on the median of nine, which falls below 10
+    with a probability under 1 % at that spread."""
     li = _angio(engine)
     n_rec, _ = li.generate_records(10_000_000, 32_000_000, keep=0.3, seed=61)
-    t0 = li.train(0, steps=12000, batch=8192, lr=3e-3, lr_end=1e-4)
+    t0 = li.train(0, steps=1500, batch=65536, lr=8e-3, lr_end=1e-4)
     assert t0.loss_last < t0.loss_first
     li.configure()
     gains = []
-    for k in range(5):
+    for k in range(9):
         ana, lrn, chosen, z = li.compare(target_rel_err=0.02, seed_analog=71 + 2 * k, seed_learned=72 + 2 * k)
         print(f"pair {k}: analog R={ana.rel_err:.4f} t={ana.seconds:.3f}s FOM={ana.fom:.1f} | learned "
               f"R={lrn.rel_err:.4f} t={lrn.seconds:.3f}s FOM={lrn.fom:.1f} gain={lrn.fom / ana.fom:.2f} z={z:.2f}")
-        assert abs(z) < 3.5   # five pairs: 3.5 sigma keeps a false alarm near 0.2 %
+        assert abs(z) < 3.5   # nine pairs: 3.5 sigma keeps a false alarm near 0.4 %
         gains.append(lrn.fom / ana.fom)
-    assert sorted(gains)[2] > 4.0, gains
+    assert sorted(gains)[4] > 10.0, gains

@@ -145,3 +145,60 @@ def test_trace_cli_writes_json(tmp_path):
     assert main(["trace", str(scene), "-n", "40", "--seed", "3", "--out", str(out)]) == 0
     doc = json.loads(out.read_text())
     assert len(doc["trajectories"]) == 40 and sum(doc["categories"].values()) == 40
+
+
+def _trace_stats(ids_steps, n):
+    """Per-history statistics of a segment log given as a list of per-step (photon ids, energies,
+    event codes): segments per history, segments ending in each event per history, and the
+    energy of the k-th segment (k < 6) averaged over the histories that have one."""
+    from chatcarlo_amd.trajectory import EVENT_NAMES
+    seg = np.zeros(n)
+    per_ev = np.zeros((len(EVENT_NAMES), n))
+    e_k = [[] for _ in range(6)]
+    for k, (ids, en, ev) in enumerate(ids_steps):
+        np.add.at(seg, ids, 1)
+        for j in range(len(EVENT_NAMES)):
+            np.add.at(per_ev[j], ids[ev == j], 1)
+        if k < 6:
+            e_k[k].append(np.asarray(en, dtype=float))
+    rows = [("segments per history", seg)]
+    rows += [(f"{EVENT_NAMES[j]} per history", per_ev[j]) for j in range(len(EVENT_NAMES))]
+    stats = [(name, float(x.mean()), float(x.std(ddof=1)) / np.sqrt(n)) for name, x in rows]
+    for k in range(6):
+        x = np.concatenate(e_k[k]) if e_k[k] else np.zeros(0)
+        if len(x) > 100:
+            stats.append((f"energy of segment {k}", float(x.mean()), float(x.std(ddof=1)) / np.sqrt(len(x))))
+    return stats
+
+
+def test_segment_log_matches_the_oracle_transport(dpack):
+    """The GPU segment log against the oracle's round loop on the same slab and beam (80 keV pencil
+    on 10 cm of water): segments per history, the share of segments ending at a boundary, an
+    escape, photoabsorption, fluorescence, Compton and Rayleigh events, and the mean energy of
+    the first six segments, each within statistics (|z| < 4, sum of z^2 within 3 sigma)."""
+    from chatcarlo_amd.trajectory import EVENT_NAMES, TrajectoryRecorder
+    from oracle import core
+    from oracle import transport as OT
+    n = 40_000
+    rec = TrajectoryRecorder()
+    _run(n, 9, e=80.0, recorder=rec)
+    ev_index = {name: j for j, name in enumerate(EVENT_NAMES)}
+    gpu = [(ids, en, np.array([ev_index[x] for x in ev])) for ids, en, ev in zip(rec.photon_ids, rec.energies,
+                                                                              rec.events)]
+    prims, world = _slab().packed(dpack.mat_id)
+    og = core.OGeom.from_packed(prims, world)
+    steps = []
+    pos, dirv, en = _beam(n, 80.0)
+    OT.transport_photons(pos, dirv, en, og, dpack, np.random.default_rng(5),
+                         recorder=lambda ids, s, e, energy, ev, m: steps.append((ids, energy, ev)))
+    a, b = _trace_stats(gpu, n), _trace_stats(steps, n)
+    assert [x[0] for x in a] == [x[0] for x in b]
+    zs = []
+    for (name, m1, s1), (_, m2, s2) in zip(a, b):
+        z = (m1 - m2) / np.hypot(s1, s2) if s1 + s2 > 0 else 0.0
+        print(f"{name}: gpu {m1:.5g} oracle {m2:.5g} z={z:+.2f}")
+        zs.append(z)
+    zs = np.array(zs)
+    assert np.all(np.abs(zs) < 4.0), zs
+    assert np.sum(zs ** 2) < len(zs) + 3 * np.sqrt(2 * len(zs))
+    assert dict((x[0], x[1]) for x in a)["fluorescence per history"] == 0.0   # water: no K lines above 5 keV
