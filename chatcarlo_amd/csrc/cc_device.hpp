@@ -624,7 +624,7 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v) {
 // voxel owns goes straight to memory; the table is added into the grid once, when the
 // block ends.  (Probing further slots was measured slower: when the table misses, the
 // voxels are not shared anyway.)
-constexpr int kCacheSlots = 1536;   // a multiple of 4; 30 KB, so that three blocks fit a CU's LDS
+constexpr int kCacheSlots = 2304;   // a multiple of 4; 45 KB, so that three blocks fit a CU's LDS
 constexpr uint32_t kCacheEmpty = 0xffffffffu;
 struct TallyCache {
   uint32_t key[kCacheSlots];
@@ -805,21 +805,8 @@ __device__ __forceinline__ int dda_score(const GridDev& G, const float o[3], con
   return dda_walk(G, o, d, t0, t1, wk, wh, seg, -1, 0, cache);
 }
 
-// Per-wave staging of the segments of a balanced tally.
 constexpr int kLanes = 64;       // wavefront size of CDNA
 constexpr int kChunkSlabs = 4;   // dominant-axis slabs per chunk
-struct SegStage {
-  float o[3][kLanes], d[3][kLanes], t0[kLanes], t1[kLanes];
-  double wk[kLanes], wh[kLanes];
-  uint32_t start[kLanes], key[kLanes];
-  int kf[kLanes], m[kLanes], ax[kLanes];
-};
-
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // The tally of dda_score with the work of the wave's segments spread over its
 // lanes.  Each segment is cut at the planes of its dominant axis (largest |d|)
@@ -827,16 +814,17 @@ __device__ __forceinline__ void wave_lds_sync() {
 // cross at most one plane each, so a chunk holds at most 3 kChunkSlabs voxel
 // pieces.  The wave's chunks are dealt to its 64 lanes in passes, so the lanes
 // of one walk have similar trip counts; chunk ends lie on voxel faces, so the
-// pieces (and the sums) are those of dda_score.  grids: per-scene grids when
+// pieces (and the sums) are those of dda_score.  A lane reads its chunk's segment
+// from the owning lane's registers (lane shuffles), so the walk needs no LDS of its
+// own and the block's LDS goes to the tally cache.  grids: per-scene grids when
 // kScene (key = scene index), else unused.  All lanes of the wave must call it.
 template <bool kScene>
-__device__ __forceinline__ int dda_balanced(const GridDev& G0, const GridDev* grids, SegStage& st, const float o[3], const float d[3],
+__device__ __forceinline__ int dda_balanced(const GridDev& G0, const GridDev* grids, const float o[3], const float d[3],
                             float L, double wk, double wh, bool seg, uint32_t key, TallyCache* cache = nullptr,
                             size_t slot_stride = 0) {
   const int lane = __lane_id();
   float t0 = 0.f, t1 = 0.f;
   if (seg) seg = dda_clip(G0, o, d, L, t0, t1);
-  // dominant axis and the planes strictly inside (t0, t1)
   const float ad[3] = {fabsf(d[0]), fabsf(d[1]), fabsf(d[2])};
   const int a = (ad[0] >= ad[1]) ? ((ad[0] >= ad[2]) ? 0 : 2) : ((ad[1] >= ad[2]) ? 1 : 2);
   int kf = 0, m = 0;
@@ -868,43 +856,38 @@ __device__ __forceinline__ int dda_balanced(const GridDev& G0, const GridDev* gr
     total = __shfl(x, kLanes - 1);
     start = x - nch;
   }
-  wave_lds_sync();
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    st.o[k][lane] = o[k];
-    st.d[k][lane] = d[k];
-  }
-  st.t0[lane] = t0; st.t1[lane] = t1; st.wk[lane] = wk; st.wh[lane] = wh;
-  st.start[lane] = start; st.key[lane] = key; st.kf[lane] = kf; st.m[lane] = m; st.ax[lane] = a;
-  wave_lds_sync();
   int visits = 0;
   for (uint32_t base = 0; base < total; base += kLanes) {
     const uint32_t q = base + (uint32_t)lane;
-    bool act = q < total;
-    float so[3] = {0.f, 0.f, 0.f}, sd[3] = {0.f, 0.f, 1.f};
-    float ta = 0.f, tb = 0.f;
-    double swk = 0.0, swh = 0.0;
-    uint32_t skey = 0;
-    int sa = -1, fidx = 0;
-    if (act) {
-      // owner: the last lane whose first chunk is at or before q (lanes without
-      // chunks share the start of the next lane, so the last one has chunks)
-      int lo = 0, hi = kLanes - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (st.start[mid] <= q) lo = mid; else hi = mid - 1;
-      }
-      const int s = lo;
+    // owner of chunk q: the last lane whose first chunk is at or before q, by a six-step search
+    // that every lane runs whole, so that every shuffle reads an active lane
+    int lo = 0, hi = kLanes - 1;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        so[k] = st.o[k][s];
-        sd[k] = st.d[k][s];
+    for (int stp = 0; stp < 6; ++stp) {
+      const int mid = (lo + hi + 1) >> 1;
+      const uint32_t sm = (uint32_t)__shfl((int)start, mid);
+      if (lo < hi) {
+        if (sm <= q) lo = mid; else hi = mid - 1;
       }
-      const float s0 = st.t0[s], s1 = st.t1[s];
-      swk = st.wk[s]; swh = st.wh[s]; skey = st.key[s];
-      sa = st.ax[s];
-      const int skf = st.kf[s], sm = st.m[s];
-      const int j = (int)(q - st.start[s]);
+    }
+    const int s = lo;
+    float so[3], sd[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      so[k] = __shfl(o[k], s);
+      sd[k] = __shfl(d[k], s);
+    }
+    const float s0 = __shfl(t0, s), s1 = __shfl(t1, s);
+    const double swk = __shfl(wk, s), swh = __shfl(wh, s);
+    const uint32_t skey = (uint32_t)__shfl((int)key, s);
+    int sa = __shfl(a, s);
+    const int skf = __shfl(kf, s), sm = __shfl(m, s);
+    const uint32_t sst = (uint32_t)__shfl((int)start, s);
+    bool act = q < total;
+    float ta = 0.f, tb = 0.f;
+    int fidx = 0;
+    if (act) {
+      const int j = (int)(q - sst);
       const float da = sel3(sa, sd[0], sd[1], sd[2]), oa = sel3(sa, so[0], so[1], so[2]);
       const int sg = da > 0.f ? 1 : -1;
       const float inv_da = 1.0f / da;
@@ -917,14 +900,13 @@ __device__ __forceinline__ int dda_balanced(const GridDev& G0, const GridDev* gr
         ga = sel3(sa, G0.org[0], G0.org[1], G0.org[2]);
         gh = G0.h;
       }
-      // chunk j: slabs [j C, min((j + 1) C, m + 1)); plane P_i (i = 1..m) has index kf + sg (i - 1)
       const int i0 = j * kChunkSlabs, i1 = i0 + kChunkSlabs;
       ta = i0 == 0 ? s0 : fmaxf(plane_t(ga, gh, oa, inv_da, skf + sg * (i0 - 1)), s0);
       tb = i1 >= sm + 1 ? s1 : fminf(plane_t(ga, gh, oa, inv_da, skf + sg * (i1 - 1)), s1);
       if (i0 == 0) {
         sa = -1;
       } else {
-        const int kp = skf + sg * (i0 - 1);   // the chunk starts on this plane
+        const int kp = skf + sg * (i0 - 1);
         fidx = sg > 0 ? kp : kp - 1;
       }
       act = tb > ta;
@@ -933,7 +915,7 @@ __device__ __forceinline__ int dda_balanced(const GridDev& G0, const GridDev* gr
       const GridDev Gs = grids[act ? skey : 0];
       visits += dda_walk(Gs, so, sd, ta, tb, swk, swh, act, sa, fidx);
     } else {
-      if (slot_stride) {   // streamed batches: the owner's tally slot (key = slot)
+      if (slot_stride) {
         GridDev Gs = G0;
         const size_t off = (size_t)skey * slot_stride;
         Gs.t += 2 * off;

@@ -767,8 +767,6 @@ template <bool kRec, bool kMulti, bool kTrace = false, bool kStream = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_flight(Launch L, const int* act, const uint32_t* cnt_in, int* qp,
                                                    int* qc, int* qr, int* coll) {
   __shared__ Prim sp_lds[kMulti ? 1 : CC_MAX_PRIMS];
-  __shared__ SegStage seg_stage[kMulti ? 1 : kBlock / kWave];
-  SegStage& sst = seg_stage[kMulti ? 0 : threadIdx.x / kWave];
   __shared__ TallyCache tc_lds;
   __shared__ BlockQueues bq;
   TallyCache* tcache = &tc_lds;
@@ -897,7 +895,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         if constexpr (kMulti)
           visits += dda_score(G, pend_p, d, pend_len, pend_wk, wh, pend, tcache);
         else
-          visits += dda_balanced<false>(G, L.MS.grids, sst, pend_p, d, pend_len, pend_wk, wh, pend,
+          visits += dda_balanced<false>(G, L.MS.grids, pend_p, d, pend_len, pend_wk, wh, pend,
                                         kStream ? (uint32_t)scene : 0u, tcache, kStream ? L.St.slot_stride : 0);
         pend_p[0] = p[0]; pend_p[1] = p[1]; pend_p[2] = p[2];
         pend_len = ds;

@@ -184,7 +184,7 @@ class LearnedImportance:
         N.check(self.lib.cc_mlp_get_params(self.eng.ctx, int(map), N.ptr(w, C.c_float)))
         return w
 
-    def train(self, map: int = 0, steps: int = 1500, batch: int = 65536, lr: float = 8e-3, lr_end: float = 1e-4,
+    def train(self, map: int = 0, steps: int = 3000, batch: int = 65536, lr: float = 8e-3, lr_end: float = 1e-4,
               seed: int = 0, init_seed: int = 0, yscale: float = 0.0, power: int = 1,
               reinit: bool = True) -> TrainResult:
         if reinit:

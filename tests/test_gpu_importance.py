@@ -313,7 +313,7 @@ def test_learned_importance_is_unbiased_and_beats_analog(engine):
     with a probability under 1 % at that spread."""
     li = _angio(engine)
     n_rec, _ = li.generate_records(10_000_000, 32_000_000, keep=0.3, seed=61)
-    t0 = li.train(0, steps=1500, batch=65536, lr=8e-3, lr_end=1e-4)
+    t0 = li.train(0, steps=3000, batch=65536, lr=8e-3, lr_end=1e-4)
     assert t0.loss_last < t0.loss_first
     li.configure()
     gains = []
