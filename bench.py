@@ -306,8 +306,11 @@ def main():
         # (chatcarlo/transport.py:441-474) -- the PCIe-inclusive rate, reported beside the value
         eng.bind_grid(vg.origin_cm, vg.shape, vg.voxel_size_cm, True)
         o = eng.run(rec, n, 1000 + k, 0, args.batch, flags)
-        if readback:
-            eng.read_grid(vg.shape, moments=True)
+        if readback:   # into a fresh VoxelGrid, as run_transport returns it
+            g = VoxelGrid.from_bbox(geo.bbox_min, geo.bbox_max, 5.0, track_uncertainty=True)
+            eng.read_grid(vg.shape, moments=True, out={"kerma": g.kerma_keV, "h10": g.h10_track_pSv_cm3,
+                                                       "kerma_sum2": g.kerma_sum2, "h10_sum2": g.h10_sum2,
+                                                       "hits": g.n_batches_hit})
         return o
 
     for k in range(args.warmup):

@@ -146,7 +146,7 @@ class BatchedRun:
         return h.hexdigest()
 
     def autotune(self, fraction: float = 0.02, cached: bool = True):
-        """Times launch shapes on a fraction of the histories (a six-run coordinate search,
+        """Times launch shapes on a fraction of the histories (a four-run coordinate search,
         cc_batch_autotune); returns the fastest and the table (block, rounds_per_sync, pool,
         histories/s) of the runs.  With cached, a workload tuned before in this process returns
         its shape and table without timing."""

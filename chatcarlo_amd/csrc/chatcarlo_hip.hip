@@ -2263,7 +2263,39 @@ struct Ctx {
   unsigned long long* b_cursor = nullptr;
   unsigned long long* b_status = nullptr;
   unsigned long long* h_status = nullptr;   // pinned
+  // pinned staging of grid reads: two chunks, the DMA of one overlapping the copy of the other
+  unsigned char* h_stage = nullptr;
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
+
+constexpr size_t kStageChunk = 8u << 20;
+
+// Device -> caller (pageable) host memory through the pinned staging chunks.
+int stage_to_host(Ctx* c, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return CC_OK;
+  if (!c->h_stage) {
+    HIP_TRY(hipHostMalloc((void**)&c->h_stage, 2 * kStageChunk, hipHostMallocDefault));
+    for (auto& e : c->stage_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  const size_t nchunk = (bytes + kStageChunk - 1) / kStageChunk;
+  auto issue = [&](size_t i) -> int {
+    const size_t off = i * kStageChunk, len = std::min(kStageChunk, bytes - off);
+    HIP_TRY(hipMemcpyAsync(c->h_stage + (i & 1) * kStageChunk, (const unsigned char*)src + off, len,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipEventRecord(c->stage_ev[i & 1], c->stream));
+    return CC_OK;
+  };
+  int r;
+  if ((r = issue(0))) return r;
+  for (size_t i = 0; i < nchunk; ++i) {
+    // chunk i + 1 goes into the buffer whose chunk (i - 1) was copied out in the last iteration
+    if (i + 1 < nchunk && (r = issue(i + 1))) return r;
+    HIP_TRY(hipEventSynchronize(c->stage_ev[i & 1]));
+    const size_t off = i * kStageChunk, len = std::min(kStageChunk, bytes - off);
+    std::memcpy((unsigned char*)dst + off, c->h_stage + (i & 1) * kStageChunk, len);
+  }
+  return CC_OK;
+}
 
 void free_batch(Ctx* c) {
   void* ps[] = {c->b_prims, c->b_prim_off, c->b_worlds, c->b_grids, c->b_srcs, c->b_seeds, c->b_hist_start,
@@ -2750,6 +2782,9 @@ int cc_ctx_destroy(void* ctx) {
   for (void* p : bs)
     if (p) (void)hipFree(p);
   if (c->h_status) (void)hipHostFree(c->h_status);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
+  for (auto e : c->stage_ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
   for (auto e : c->ev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -2975,11 +3010,12 @@ int cc_read_grid(void* ctx, double* kerma, double* h10, double* ksum2, double* h
   if (!c || !c->have_grid) return fail(CC_ERR_STATE, "no grid bound");
   HIP_TRY(hipSetDevice(c->device));
   const size_t b = c->nvox * sizeof(double);
-  if (kerma) HIP_TRY(hipMemcpyAsync(kerma, c->tk, b, hipMemcpyDeviceToHost, c->stream));
-  if (h10) HIP_TRY(hipMemcpyAsync(h10, c->th, b, hipMemcpyDeviceToHost, c->stream));
-  if (ksum2) HIP_TRY(hipMemcpyAsync(ksum2, c->qk, b, hipMemcpyDeviceToHost, c->stream));
-  if (hsum2) HIP_TRY(hipMemcpyAsync(hsum2, c->qh, b, hipMemcpyDeviceToHost, c->stream));
-  if (hits) HIP_TRY(hipMemcpyAsync(hits, c->hits, c->nvox * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  int r;
+  if (kerma && (r = stage_to_host(c, kerma, c->tk, b))) return r;
+  if (h10 && (r = stage_to_host(c, h10, c->th, b))) return r;
+  if (ksum2 && (r = stage_to_host(c, ksum2, c->qk, b))) return r;
+  if (hsum2 && (r = stage_to_host(c, hsum2, c->qh, b))) return r;
+  if (hits && (r = stage_to_host(c, hits, c->hits, c->nvox * sizeof(int)))) return r;
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (n_batches) *n_batches = c->g_batches;
   if (n_hist) *n_hist = c->g_hist;
@@ -4248,14 +4284,14 @@ int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tunin
                       int32_t max_rows, int32_t* n_rows) {
   Ctx* c = (Ctx*)ctx;
   if (!c || !best) return fail(CC_ERR_ARG, "null argument");
-  // Coordinate search from (block 128, 8 rounds per sync, pool 2^20): the pool first (it moves
-  // the rate most), then the block size, then rounds per sync, each axis at the best of the
-  // axes before.  Six timing runs instead of all 24 shapes; timing runs skip the tally clear and
-  // fold, which do not depend on the shape.  The tallies are left cleared, as after an upload.
+  // Coordinate search from (block 128, 4 rounds per sync, pool 2^20): the pool first (it moves
+  // the rate most), then the block size at the best pool.  Four timing runs instead of all 24
+  // shapes (rounds per sync moved the rate by 1 % over 4..16 on the MI355X, so it is not
+  // searched); timing runs skip the tally clear and fold, which do not depend on the shape.  The
+  // tallies are left cleared, as after an upload.
   const uint32_t pools[] = {1u << 19, 1u << 20, 1u << 21};
   const int blocks[] = {128, 256};
-  const int rps[] = {8, 4, 16};
-  cc_batch_tuning cur{128, 8, 1u << 20, 8u};
+  cc_batch_tuning cur{128, 4, 1u << 20, 8u};
   double best_rate = -1.0;
   int rows = 0;
   auto trial = [&](const cc_batch_tuning& t) -> int {
@@ -4285,13 +4321,6 @@ int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tunin
     if (b == cur.block) continue;
     cc_batch_tuning t = cur;
     t.block = b;
-    if ((r = trial(t))) return r;
-  }
-  cur = *best;
-  for (int q : rps) {
-    if (q == cur.rounds_per_sync) continue;
-    cc_batch_tuning t = cur;
-    t.rounds_per_sync = q;
     if ((r = trial(t))) return r;
   }
   if (n_rows) *n_rows = rows;

@@ -180,13 +180,25 @@ class Engine:
         else:
             N.check(self.lib.cc_reset_grid(self.ctx))
 
-    def read_grid(self, shape, moments=True):
+    def read_grid(self, shape, moments=True, out=None):
+        """Totals (and with moments the batch moments and hit counts) of the bound grid.  out:
+        optional dict of C-contiguous arrays of the grid's shape (kerma, h10, kerma_sum2, h10_sum2
+        float64, hits int32), filled in place instead of new arrays, e.g. a VoxelGrid's own."""
         nv = int(np.prod(shape))
-        k = np.zeros(nv)
-        h = np.zeros(nv)
-        q1 = np.zeros(nv) if moments else None
-        q2 = np.zeros(nv) if moments else None
-        hits = np.zeros(nv, dtype=np.int32) if moments else None
+        out = out or {}
+
+        def target(key, dtype):
+            a = out.get(key)
+            if a is None:
+                return np.empty(nv, dtype=dtype)
+            if a.dtype != dtype or a.size != nv or not a.flags.c_contiguous:
+                raise ValueError(f"read_grid: {key} must be a C-contiguous {dtype} array of {nv} voxels")
+            return a.reshape(nv)
+        k = target("kerma", np.float64)
+        h = target("h10", np.float64)
+        q1 = target("kerma_sum2", np.float64) if moments else None
+        q2 = target("h10_sum2", np.float64) if moments else None
+        hits = target("hits", np.int32) if moments else None
         nb = C.c_int64()
         nh = C.c_int64()
         nul = C.POINTER(C.c_double)()
@@ -349,13 +361,12 @@ def run_transport(scene, n_histories: int = 100_000, seed: int | None = None,
         moments.n_batches = int(out.n_batches)
         moments.n_histories = int(n_histories)
     if grid is not None:
-        g = eng.read_grid(grid.shape, moments=track_uncertainty)
-        grid.kerma_keV[...] = g["kerma"]
-        grid.h10_track_pSv_cm3[...] = g["h10"]
+        # straight into the grid's own arrays (no intermediate copies)
+        dst = {"kerma": grid.kerma_keV, "h10": grid.h10_track_pSv_cm3}
         if track_uncertainty:
-            grid.kerma_sum2[...] = g["kerma_sum2"]
-            grid.h10_sum2[...] = g["h10_sum2"]
-            grid.n_batches_hit[...] = g["hits"]
+            dst.update(kerma_sum2=grid.kerma_sum2, h10_sum2=grid.h10_sum2, hits=grid.n_batches_hit)
+        g = eng.read_grid(grid.shape, moments=track_uncertainty, out=dst)
+        if track_uncertainty:
             grid.n_batches = int(g["n_batches"])
             grid.n_histories = int(g["n_histories"])
     n_real = None

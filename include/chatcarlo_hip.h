@@ -333,8 +333,8 @@ int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int
 int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32_t flags, cc_batch_out* out);
 int cc_batch_read(void* ctx, int32_t scene, double* kerma_keV, double* h10_pSv_cm3, double* edep_keV);
 /* Times launch shapes, with 8 tally copies, on a fraction of the histories and returns the
- * fastest: a coordinate search from (128, 8, 2^20) over pool {2^19, 2^20, 2^21}, then block
- * {128, 256}, then rounds_per_sync {4, 8, 16}, six timing runs.  table (4 doubles per row: block,
+ * fastest: a coordinate search from (block 128, rounds_per_sync 4, pool 2^20) over pool {2^19,
+ * 2^20, 2^21}, then block {128, 256}, four timing runs.  table (4 doubles per row: block,
  * rounds_per_sync, pool, histories/s; one row per run) may be NULL.  The scene tallies are
  * cleared afterwards. */
 int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tuning* best, double* table,

@@ -90,8 +90,8 @@ def test_autotune_returns_a_candidate(engine, single):
     from chatcarlo_amd.batch import BatchedRun, Tuning
     b = BatchedRun(_scenes(3, seed=5), engine)
     t, table = b.autotune(fraction=0.2, cached=False)
-    # the coordinate search: three pools at (128, 8), the other block, two other rounds_per_sync
-    assert len(table) == 6 and np.all(table[:, 3] > 0)
+    # the coordinate search: three pools at (128, 4), then the other block at the best pool
+    assert len(table) == 4 and np.all(table[:, 3] > 0)
     assert sorted(int(p) for p in table[:3, 2]) == [1 << 19, 1 << 20, 1 << 21]
     assert (t.block, t.rounds_per_sync, t.pool) in {(int(a), int(q), int(p)) for a, q, p, _ in table}
     best = table[np.argmax(table[:, 3])]
