@@ -181,11 +181,12 @@ def learned_fom(eng, target=0.02, pairs=5):
 
 def batched_sweep(eng, autotune_fraction=0.01, sequential=True):
     """64 random scenes (1-40 primitives each, 50-150 kVp, 1e5-1e7 histories, 2 cm grids) in one
-    particle pool: launch shape autotuned on a fraction of the histories, then the whole sweep;
+    batched run: form and launch shape autotuned on a fraction of the histories, then the whole
+    sweep; the whole sweep again in each form (one persistent launch, replayed event-based rounds);
     beside it the same sweep scene by scene (one single-scene run per scene, as a caller of
     run_transport would loop), whose tallies it must equal."""
     from chatcarlo_amd import _native as N
-    from chatcarlo_amd.batch import BatchedRun, SceneSpec
+    from chatcarlo_amd.batch import BatchedRun, SceneSpec, Tuning
     from chatcarlo_amd.geometry import Geometry
     from chatcarlo_amd.source import SourceRecord
     from chatcarlo_amd.workloads import batched_scenes
@@ -193,7 +194,8 @@ def batched_sweep(eng, autotune_fraction=0.01, sequential=True):
     t0 = time.perf_counter()
     b = BatchedRun(scenes, eng)
     t_up = time.perf_counter() - t0
-    b.run(fraction=0.005)   # warm-up (code objects, pool allocation)
+    b.run(fraction=0.005)   # warm-up of both forms (code objects, pool allocation)
+    b.run(Tuning(128, 4, 1 << 19, 16), fraction=0.005)
     t0 = time.perf_counter()
     tune, table = b.autotune(autotune_fraction, cached=False)
     t_tune = time.perf_counter() - t0
@@ -201,9 +203,10 @@ def batched_sweep(eng, autotune_fraction=0.01, sequential=True):
     st = b.run(tune)
     wall = time.perf_counter() - t0
     out = {"workload": "64_random_scenes_300cm_air_room_2cm_grids", "scenes": len(scenes),
-           "histories": st.n_histories, "tuning": {"block": tune.block, "rounds_per_sync": tune.rounds_per_sync,
-                                                   "pool": tune.pool,
-                                                   "tally_copies": tune.tally_copies},
+           "histories": st.n_histories,
+           "tuning": {"form": "persistent" if tune.persistent else "event_rounds", "block": tune.block,
+                      "epoch": tune.epoch, "rounds_per_sync": tune.rounds_per_sync, "pool": tune.pool,
+                      "tally_copies": tune.tally_copies},
            "histories_per_s": st.n_histories / wall, "device_ms": st.device_ms, "wall_s": wall,
            "rounds": st.rounds, "graph_launches": st.graph_launches,
            "voxel_visits_per_history": st.n_voxel_visits / st.n_histories,
@@ -211,7 +214,20 @@ def batched_sweep(eng, autotune_fraction=0.01, sequential=True):
            "upload_s": t_up, "autotune_s": t_tune, "autotune_fraction": autotune_fraction,
            "autotune_share_of_sweep": t_tune / wall, "wall_s_with_autotune": wall + t_tune,
            "histories_per_s_with_autotune": st.n_histories / (wall + t_tune),
-           "autotune_table": [[int(r[0]), int(r[1]), int(r[2]), round(float(r[3]))] for r in table]}
+           "autotune_columns": ["persistent", "block", "epoch", "rounds_per_sync", "pool", "tally_copies",
+                                "histories_per_s"],
+           "autotune_table": [[int(x) for x in r[:6]] + [round(float(r[6]))] for r in table]}
+    # the two forms on the whole sweep: one persistent launch and the replayed event-based rounds
+    forms = {}
+    for name, t in (("persistent", Tuning(512, tally_copies=16, persistent=True, epoch=2)),
+                    ("event_rounds", Tuning(128, 4, 1 << 19, 16))):
+        s_ = b.run(t)
+        forms[name] = {"histories_per_s_device": s_.n_histories / (s_.device_ms * 1e-3), "device_ms": s_.device_ms,
+                       "graph_launches": s_.graph_launches}
+    forms["persistent_over_event_rounds"] = (forms["persistent"]["histories_per_s_device"]
+                                             / forms["event_rounds"]["histories_per_s_device"])
+    out["forms"] = forms
+    b.run(tune)   # the tuned run's tallies again, for the comparison below
     if sequential:
         kerma_b = [float(b.read(k)[0].sum()) for k in range(len(scenes))]
         t0 = time.perf_counter()

@@ -108,7 +108,7 @@ class CcTraceRecord(C.Structure):
 
 class CcBatchTuning(C.Structure):
     _fields_ = [("block", C.c_int32), ("rounds_per_sync", C.c_int32), ("pool", C.c_uint32),
-                ("tally_copies", C.c_uint32)]
+                ("tally_copies", C.c_uint32), ("persistent", C.c_int32), ("epoch", C.c_int32)]
 
 
 class CcBatchOut(C.Structure):

@@ -16,7 +16,7 @@ material); batch moments (uncertainties) stay with ``run_transport``.
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -40,15 +40,25 @@ class SceneSpec:
 @dataclass
 class Tuning:
     """Launch shape of a batched run: threads per block of the transport kernels, rounds
-    between host checks of the queue (even), photons in flight, and copies of the scene
-    tallies (workgroup b adds into copy b mod copies)."""
+    between host checks of the queue (even), photons in flight, copies of the scene tallies
+    (workgroup b adds into copy b mod copies), and the form: persistent = True runs one
+    persistent history-based launch (rounds_per_sync and pool unused) whose workgroups flush
+    their LDS tally cache every `epoch` wave iterations."""
     block: int = 256
     rounds_per_sync: int = 8
     pool: int = 1 << 20
     tally_copies: int = 8
+    persistent: bool = False
+    epoch: int = 32
 
     def struct(self):
-        return N.CcBatchTuning(int(self.block), int(self.rounds_per_sync), int(self.pool), int(self.tally_copies))
+        return N.CcBatchTuning(int(self.block), int(self.rounds_per_sync), int(self.pool), int(self.tally_copies),
+                               int(bool(self.persistent)), int(self.epoch))
+
+
+# the shape of a batched run without a tuning: the persistent form, eight waves per workgroup
+# sharing one LDS tally cache, flushed every second iteration, 16 tally copies
+PERSISTENT = Tuning(512, tally_copies=16, persistent=True, epoch=2)
 
 
 @dataclass
@@ -118,8 +128,9 @@ class BatchedRun:
         self._shapes = [tuple(int(x) for x in s.grid[1]) for s in self.scenes]
 
     def run(self, tuning: Tuning | None = None, fraction: float = 1.0) -> BatchStats:
-        """Transports ceil(fraction n_s) histories of every scene; tallies start from zero."""
-        t = tuning or Tuning()
+        """Transports ceil(fraction n_s) histories of every scene; tallies start from zero.  Without
+        a tuning: the persistent form (PERSISTENT)."""
+        t = tuning or PERSISTENT
         self._frac = float(fraction)
         out = N.CcBatchOut()
         N.check(self.eng.lib.cc_batch_run(self.eng.ctx, C.byref(t.struct()), float(fraction), self.flags,
@@ -146,20 +157,22 @@ class BatchedRun:
         return h.hexdigest()
 
     def autotune(self, fraction: float = 0.02, cached: bool = True):
-        """Times launch shapes on a fraction of the histories (a four-run coordinate search,
-        cc_batch_autotune); returns the fastest and the table (block, rounds_per_sync, pool,
-        histories/s) of the runs.  With cached, a workload tuned before in this process returns
-        its shape and table without timing."""
+        """Times launch shapes on a fraction of the histories (a four-run coordinate search over
+        the persistent and event-based forms, cc_batch_autotune); returns the fastest and the
+        table (persistent, block, epoch, rounds_per_sync, pool, tally_copies, histories/s) of the
+        runs.  With cached, a workload tuned before in this process returns its shape and table
+        without timing."""
         key = self.signature(fraction)
         if cached and key in BatchedRun._tuned:
             t, table = BatchedRun._tuned[key]
-            return Tuning(t.block, t.rounds_per_sync, t.pool, t.tally_copies), table.copy()
+            return replace(t), table.copy()
         best = N.CcBatchTuning()
-        table = np.zeros((64, 4))
+        table = np.zeros((64, 7))
         rows = C.c_int32()
         N.check(self.eng.lib.cc_batch_autotune(self.eng.ctx, float(fraction), self.flags, C.byref(best),
                                                N.ptr(table, C.c_double), 64, C.byref(rows)))
-        tune = Tuning(best.block, best.rounds_per_sync, best.pool, best.tally_copies)
+        tune = Tuning(best.block, best.rounds_per_sync, best.pool, best.tally_copies, bool(best.persistent),
+                      int(best.epoch) or 32)
         table = table[:min(rows.value, 64)].copy()
         BatchedRun._tuned[key] = (tune, table)
         return tune, table

@@ -626,13 +626,18 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v) {
 // voxels are not shared anyway.)
 constexpr int kCacheSlots = 2304;   // a multiple of 4; 45 KB, so that three blocks fit a CU's LDS
 constexpr uint32_t kCacheEmpty = 0xffffffffu;
-struct TallyCache {
-  uint32_t key[kCacheSlots];
-  double val[kCacheSlots][2];
+template <int kSlots>
+struct TallyCacheT {
+  static_assert(kSlots % 4 == 0, "slots come in groups of four");
+  static constexpr int kN = kSlots;
+  uint32_t key[kSlots];
+  double val[kSlots][2];
 };
+using TallyCache = TallyCacheT<kCacheSlots>;
 
-__device__ __forceinline__ void cache_init(TallyCache* c) {
-  for (int k = threadIdx.x; k < kCacheSlots; k += blockDim.x) {
+template <class C>
+__device__ __forceinline__ void cache_init(C* c) {
+  for (int k = threadIdx.x; k < C::kN; k += blockDim.x) {
     c->key[k] = kCacheEmpty;
     c->val[k][0] = 0.0;
     c->val[k][1] = 0.0;
@@ -640,11 +645,12 @@ __device__ __forceinline__ void cache_init(TallyCache* c) {
   __syncthreads();
 }
 
-__device__ __forceinline__ bool cache_add(TallyCache* c, uint32_t v, double a, double b) {
+template <class C>
+__device__ __forceinline__ bool cache_add(C* c, uint32_t v, double a, double b) {
   // The four voxels whose tally pairs share a 64-byte segment (keys 4k .. 4k + 3) take four
   // consecutive slots, so the block-end flush adds them from neighbouring lanes of one
   // instruction, which the memory system merges into one request; groups are hashed.
-  const uint32_t g = (uint32_t)(((uint64_t)((v >> 2) * 2654435761u) * (uint32_t)(kCacheSlots / 4)) >> 32);
+  const uint32_t g = (uint32_t)(((uint64_t)((v >> 2) * 2654435761u) * (uint32_t)(C::kN / 4)) >> 32);
   const uint32_t s = (g << 2) | (v & 3u);
   uint32_t cur = c->key[s];
   if (cur == kCacheEmpty) cur = atomicCAS(&c->key[s], kCacheEmpty, v);
@@ -689,8 +695,9 @@ __device__ __forceinline__ void pair_add(bool pred, double* p, double a, double 
 // (pair_add).  Lanes that share a voxel meet in the cache; matching them across the wave
 // first (ballots and lane shuffles at every DDA step) was measured 14 % slower once the
 // cache was there.  Every lane of the wave must call this (converged control flow).
+template <class C = TallyCache>
 __device__ __forceinline__ void wave_tally(const GridDev& G, bool has, uint32_t v, double ak, double ah,
-                                           TallyCache* tc = nullptr) {
+                                           C* tc = nullptr) {
   double* const t = G.t + (size_t)(blockIdx.x & G.cmask) * G.cstride;
   const bool direct = has && !(tc && cache_add(tc, G.base + v, ak, ah));
   pair_add(direct, t + 2 * (size_t)v, ak, ah);
@@ -733,9 +740,10 @@ __device__ __forceinline__ float plane_t(float org_a, float h, float o_a, float 
 // overlapped.  fa >= 0 fixes the start index on axis fa to fidx (a walk that
 // starts on a plane of that axis).  All lanes of the wave must call it.
 // Returns the number of voxel visits of the calling lane.
+template <class C = TallyCache>
 __device__ __forceinline__ int dda_walk(const GridDev& G, const float o[3], const float d[3], float t0, float t1,
                                         double wk, double wh, bool act, int fa = -1, int fidx = 0,
-                                        TallyCache* cache = nullptr) {
+                                        C* cache = nullptr) {
   const int n[3] = {G.nx, G.ny, G.nz};
   int idx[3] = {0, 0, 0}, step[3] = {0, 0, 0};
   float inv_d[3] = {0.f, 0.f, 0.f};
@@ -798,8 +806,9 @@ __device__ __forceinline__ int dda_walk(const GridDev& G, const float o[3], cons
 // Scores wk*len into the kerma and wh*len into the H*(10) tally of every voxel the segment
 // o + t d, t in [0, L], overlaps, for lanes with seg = true; each lane walks
 // its own segment.  All lanes of the wave must call it.
+template <class C = TallyCache>
 __device__ __forceinline__ int dda_score(const GridDev& G, const float o[3], const float d[3], float L,
-                                         double wk, double wh, bool seg, TallyCache* cache = nullptr) {
+                                         double wk, double wh, bool seg, C* cache = nullptr) {
   float t0 = 0.f, t1 = 0.f;
   if (seg) seg = dda_clip(G, o, d, L, t0, t1);
   return dda_walk(G, o, d, t0, t1, wk, wh, seg, -1, 0, cache);
@@ -818,9 +827,9 @@ constexpr int kChunkSlabs = 4;   // dominant-axis slabs per chunk
 // from the owning lane's registers (lane shuffles), so the walk needs no LDS of its
 // own and the block's LDS goes to the tally cache.  grids: per-scene grids when
 // kScene (key = scene index), else unused.  All lanes of the wave must call it.
-template <bool kScene>
+template <bool kScene, class C = TallyCache>
 __device__ __forceinline__ int dda_balanced(const GridDev& G0, const GridDev* grids, const float o[3], const float d[3],
-                            float L, double wk, double wh, bool seg, uint32_t key, TallyCache* cache = nullptr,
+                            float L, double wk, double wh, bool seg, uint32_t key, C* cache = nullptr,
                             size_t slot_stride = 0) {
   const int lane = __lane_id();
   float t0 = 0.f, t1 = 0.f;

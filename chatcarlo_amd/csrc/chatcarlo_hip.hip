@@ -1206,6 +1206,288 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
 }
 
 // ---------------------------------------------------------------------------
+// batched runs in one persistent launch
+// ---------------------------------------------------------------------------
+// History-based transport of the batched index space: each lane carries one history from its
+// source draw to its end (escape, absorption or a device error) and then starts the next, so
+// no photon record goes through memory and no round waits on its slowest photon.  A wave
+// claims kPersistChunk consecutive histories from the device cursor at a time and deals them
+// to its idle lanes.  One iteration of a wave runs, for its lanes in turn: the interactions
+// that ended the previous iteration's flights, the source draws of idle lanes, one boundary
+// step of every flight, and the tally of the previous iteration's segments (one step late, as
+// in k_flight, so that the walk's atomics follow the step's loads instead of holding them up).
+// A history draws the stream (scene seed, history, tag 0) in the order of the event-based
+// rounds and every step is the arithmetic of k_refill, k_flight and k_interact, so the event
+// counters equal those of the rounds and the tallies and deposits differ only in the order of
+// their FP64 sums.  Every `epoch` iterations the block adds its LDS tally cache into the grid
+// copies and clears it; a block leaves once the cursor has passed the end and every lane of
+// every wave is idle.
+constexpr uint32_t kPersistChunk = 128;   // histories a wave claims at once
+constexpr int kMaxFlights = 1 << 12;      // flights of one history (CC_DEV_ROUND_CAP beyond)
+// Two waves per SIMD (every value in registers) and the CU's LDS as tally cache: 4000 slots per
+// block of 256 threads (two blocks per CU) or 8000 per block of 512 (one).  Three waves with 2304
+// slots were 7 % slower than the first; the second, whose eight waves share one cache, is 16 %
+// faster again.
+template <int kThreads>
+struct PersistShape {
+  static_assert(kThreads == 256 || kThreads == 512, "blocks of 256 or 512 threads");
+  static constexpr int kSlots = 4000 * (kThreads / 256);
+};
+enum { PH_IDLE = 0, PH_FLY = 1, PH_HIT = 2 };
+
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_batch_persistent(Launch L,
+                                                                                                     int epoch) {
+  constexpr int kPersistSlots = PersistShape<kThreads>::kSlots;
+  __shared__ TallyCacheT<kPersistSlots> tc_lds;
+  TallyCacheT<kPersistSlots>* const tcache = &tc_lds;
+  cache_init(tcache);
+  const MultiScene& M = L.MS;
+  const Tables& T = L.T;
+  const int lane = __lane_id();
+  // the lane's history
+  int phase = PH_IDLE, scene = -1, type = 0, mat = 0, nfl = 0;
+  uint32_t hid = 0, ctr = 0;
+  uint64_t key = 0;
+  float E = 1.f, tau = 0.f;
+  float p[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 1.f};
+  const Prim* sp = M.prims;
+  World Wd = L.W;
+  // the current flight
+  bool fresh = false;
+  float o0[3] = {0.f, 0.f, 0.f};
+  float eps0 = 0.f, tc = 0.f, mu = 0.f, muen = 0.f;
+  int mc = -1, it = 0;
+  EPos ep{0, 0.f, 0.f};
+  double wh = 0.0;
+  // the segment waiting for its tally, and the grid of its scene
+  float pend_p[3] = {0.f, 0.f, 0.f}, pend_d[3] = {0.f, 0.f, 1.f}, pend_len = 0.f;
+  double pend_wk = 0.0, pend_wh = 0.0;
+  bool pend = false;
+  int pend_sc = 0, gscene = -1;
+  GridDev G = L.G;
+  // event counts since the last epoch end
+  uint32_t n_evt = 0, n_vox = 0, n_fl = 0, n_esc = 0, n_abs = 0, n_fluor = 0, n_scat = 0;
+  // the wave's claimed histories [wnext, wend)
+  unsigned long long wnext = 0, wend = 0;
+  bool exhausted = false, done = false;
+  for (;;) {
+    for (int itn = 0; itn < epoch && !done; ++itn) {
+      // 1. interactions (photo_one, compton_one, rayleigh_one on registers)
+      double dep = 0.0;
+      bool dep_on = false;
+      if (phase == PH_HIT) {
+        PhiloxRegRng rng(key, (uint64_t)hid, 0u, ctr);
+        const EPos epi = energy_pos(T, E);
+        bool fly = true;
+        if (type == 0) {
+          float eline = 0.f;
+          bool emit = false;
+          if (L.fluor) {
+            int el;
+            emit = fluor_sample(T, mat, E, epi, rng, el, eline);
+          }
+          dep = (double)(emit ? E - eline : E);
+          if (emit) {
+            isotropic_dir(d, rng.next(), rng.next());
+            tau = -logf(rng.next());
+            E = eline;
+            ++n_fluor;
+          } else {
+            fly = false;
+            ++n_abs;
+          }
+        } else if (type == 1) {
+          float cs, sn;
+          int el, trials;
+          const float eps = compton_sample(T, mat, E, epi, rng, kTrialCap, cs, sn, el, trials);
+          if (trials == 0) raise_err(L.err, CC_DEV_ITER_CAP);
+          const float enew = E * eps;
+          dep = (double)E - (double)enew;
+          rotate_dir(d, cs, sn, rng.next());
+          tau = -logf(rng.next());
+          E = enew;
+        } else {
+          int el, trials;
+          bool qerr;
+          float sn;
+          const float cs = rayleigh_sample(T, mat, E, epi, rng, kTrialCap, sn, el, trials, qerr);
+          if (trials == 0) raise_err(L.err, CC_DEV_ITER_CAP);
+          if (qerr) raise_err(L.err, CC_DEV_Q_RANGE);
+          rotate_dir(d, cs, sn, rng.next());
+          tau = -logf(rng.next());
+        }
+        dep_on = type <= 1;
+        ctr = rng.ctr;
+        ++n_scat;
+        phase = fly ? PH_FLY : PH_IDLE;
+        fresh = fly;
+      }
+      wave_add_by_key(dep_on, scene * CC_MAX_MAT + mat, dep, dep_copy(L));
+      // 2. idle lanes start the next histories of the wave's claim (k_refill)
+      for (int pass = 0; pass < 3; ++pass) {
+        const unsigned long long need = __ballot(phase == PH_IDLE);
+        if (need == 0ull || exhausted) break;
+        if (wnext >= wend) {
+          unsigned long long base = 0ull;
+          if (lane == 0) base = atomicAdd(M.cursor, (unsigned long long)kPersistChunk);
+          base = __shfl(base, 0);
+          if (base >= M.total) {
+            exhausted = true;
+            break;
+          }
+          wnext = base;
+          wend = min(base + (unsigned long long)kPersistChunk, M.total);
+        }
+        const uint32_t avail = (uint32_t)(wend - wnext);
+        const uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+        if (phase == PH_IDLE && rank < avail) {
+          const unsigned long long g = wnext + rank;
+          int lo = 0, hi = M.n - 1;   // last scene whose first history is <= g
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (M.hist_start[mid] <= g) lo = mid; else hi = mid - 1;
+          }
+          if (lo != scene) {
+            scene = lo;
+            sp = M.prims + M.prim_off[lo];
+            Wd = M.worlds[lo];
+            key = M.seeds[lo];
+          }
+          const unsigned long long h = g - M.hist_start[lo];
+          PhiloxRng rng(key, (uint64_t)h, 0u, 0u);
+          sample_source(M.srcs[lo], rng, p, d, E);
+          tau = -logf(rng.next());
+          ctr = rng.ctr;
+          hid = (uint32_t)h;
+          nfl = 0;
+          phase = PH_FLY;
+          fresh = true;
+        }
+        wnext += min((uint32_t)__popcll(need), avail);
+      }
+      // 3. one boundary step of every flight (k_flight)
+      float q[3] = {0.f, 0.f, 0.f}, ds = 0.f;
+      double wk = 0.0;
+      bool seg = false;
+      if (phase == PH_FLY && fresh) {
+        fresh = false;
+        ++n_fl;
+        if (++nfl > kMaxFlights) {
+          raise_err(L.err, CC_DEV_ROUND_CAP);
+          phase = PH_IDLE;
+        } else if (!energy_ok(E)) {
+          raise_err(L.err, CC_DEV_ENERGY_RANGE);
+          phase = PH_IDLE;
+        } else {
+          ep = energy_pos(T, E);
+          wh = (double)h10_coeff(T.h10, T.n_h10, E);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) o0[k] = p[k];
+          eps0 = 1e-6f + 1.2e-7f * fmaxf(fabsf(p[0]), fmaxf(fabsf(p[1]), fabsf(p[2])));
+          tc = 0.f;
+          it = 0;
+          mc = -1;
+        }
+      }
+      if (phase == PH_FLY) {
+        ++n_evt;
+        bool esc;
+        const float t_next = next_boundary(sp, Wd.n_prim, Wd, o0, d, it == 0 ? eps0 : tc, esc);
+        if (!(t_next < INFINITY) || t_next < 0.f) {   // outside the world box
+          ++n_esc;
+          phase = PH_IDLE;
+        } else {
+          const float tb = t_next - tc;
+          const float th = tc + 0.5f * tb;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) q[k] = o0[k] + d[k] * tc;
+          const float mid[3] = {o0[0] + d[0] * th, o0[1] + d[1] * th, o0[2] + d[2] * th};
+          const int m = locate(sp, Wd.n_prim, Wd.background, mid);
+          if (m != mc) {
+            mc = m;
+            const float rho = T.density[m];
+            mu = xs_part(T, m, P_TOTAL, E, ep) * rho;
+            muen = xs_part(T, m, P_MUEN, E, ep) * rho;
+          }
+          const float taub = mu * tb;
+          const bool inter = tau < taub;
+          ds = inter ? tau / fmaxf(mu, 1e-30f) : tb;
+          wk = (double)E * (double)muen;
+          seg = true;
+          if (inter) {
+            tc += ds;
+            PhiloxRng rng(key, (uint64_t)hid, 0u, ctr);
+            type = sample_type(T, m, E, ep, rng.next());
+            ctr = rng.ctr;
+            mat = m;
+            tau = 0.f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) p[k] = o0[k] + d[k] * tc;
+            phase = PH_HIT;
+          } else {
+            tau -= taub;
+            tc = t_next;
+            if (esc) {
+              ++n_esc;
+              phase = PH_IDLE;
+            } else if (++it >= kMaxCrossings) {
+              raise_err(L.err, CC_DEV_STEP_CAP);
+              phase = PH_IDLE;
+            }
+          }
+        }
+      }
+      // 4. the tally of the previous iteration's segments; this step's wait for the next (each
+      // lane walks its own: the wave's segments walked work-balanced were measured 8 % slower here)
+      if (pend && pend_sc != gscene) {
+        G = M.grids[pend_sc];
+        gscene = pend_sc;
+      }
+      n_vox += (uint32_t)dda_score(G, pend_p, pend_d, pend_len, pend_wk, pend_wh, pend, tcache);
+      pend = seg;
+      if (seg) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          pend_p[k] = q[k];
+          pend_d[k] = d[k];
+        }
+        pend_len = ds;
+        pend_wk = wk;
+        pend_wh = wh;
+        pend_sc = scene;
+      }
+      if (exhausted && !__ballot(phase != PH_IDLE || pend)) done = true;
+    }
+    wave_sum_small(n_evt, stat_copy(L) + S_EVT);
+    wave_sum_small(n_vox, stat_copy(L) + S_VOX);
+    wave_sum_small(n_fl, stat_copy(L) + S_FLIGHT);
+    wave_sum_small(n_esc, stat_copy(L) + S_ESC);
+    wave_sum_small(n_abs, stat_copy(L) + S_ABS);
+    wave_sum_small(n_fluor, stat_copy(L) + S_FLUOR);
+    wave_sum_small(n_scat, stat_copy(L) + S_SCAT);
+    n_evt = n_vox = n_fl = n_esc = n_abs = n_fluor = n_scat = 0;
+    // the block's cached tallies into its copy of the grids, then a clear cache
+    __syncthreads();
+    {
+      double* const t = M.t0 + (size_t)(blockIdx.x & M.cmask) * M.cstride;
+      for (int e = threadIdx.x; e < 2 * kPersistSlots; e += blockDim.x) {
+        const int k = e >> 1, c = e & 1;
+        const uint32_t v = tcache->key[k];
+        if (v != kCacheEmpty) atomic_add_f64(t + 2 * (size_t)v + c, tcache->val[k][c]);
+      }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kPersistSlots; k += blockDim.x) {
+      tcache->key[k] = kCacheEmpty;
+      tcache->val[k][0] = 0.0;
+      tcache->val[k][1] = 0.0;
+    }
+    if (!__syncthreads_or(done ? 0 : 1)) break;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // learned importance: weight windows, splitting and Russian roulette
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t& total) {
@@ -4107,11 +4389,18 @@ static int batch_run(Ctx* c, const cc_batch_tuning* tune, double fraction, uint3
                      bool tally) {
   if (!c || !tune || !out) return fail(CC_ERR_ARG, "null argument");
   if (c->nscene <= 0) return fail(CC_ERR_STATE, "no scenes uploaded (cc_batch_upload)");
-  if (tune->block < 64 || tune->block > kBlock || tune->block % kWave)
-    return fail(CC_ERR_ARG, "block %d must be a multiple of 64 in [64, %d]", tune->block, kBlock);
-  if (tune->rounds_per_sync < 2 || tune->rounds_per_sync > 256 || tune->rounds_per_sync % 2)
-    return fail(CC_ERR_ARG, "rounds_per_sync must be even, 2..256");
-  if (tune->pool < 1024 || tune->pool > (1u << 28)) return fail(CC_ERR_ARG, "pool %u out of range", tune->pool);
+  const bool persistent = tune->persistent != 0;
+  if (persistent) {
+    if (tune->block != 256 && tune->block != 512)
+      return fail(CC_ERR_ARG, "block %d: the persistent form takes 256 or 512", tune->block);
+    if (tune->epoch < 1 || tune->epoch > (1 << 16)) return fail(CC_ERR_ARG, "epoch %d out of range 1..65536", tune->epoch);
+  } else {
+    if (tune->block < 64 || tune->block > kBlock || tune->block % kWave)
+      return fail(CC_ERR_ARG, "block %d must be a multiple of 64 in [64, %d]", tune->block, kBlock);
+    if (tune->rounds_per_sync < 2 || tune->rounds_per_sync > 256 || tune->rounds_per_sync % 2)
+      return fail(CC_ERR_ARG, "rounds_per_sync must be even, 2..256");
+    if (tune->pool < 1024 || tune->pool > (1u << 28)) return fail(CC_ERR_ARG, "pool %u out of range", tune->pool);
+  }
   if (!(fraction > 0.0 && fraction <= 1.0)) return fail(CC_ERR_ARG, "fraction must be in (0, 1]");
   const uint32_t ncopy = tune->tally_copies ? tune->tally_copies : 1u;
   if (ncopy > 16 || (ncopy & (ncopy - 1))) return fail(CC_ERR_ARG, "tally_copies %u must be 0 or a power of two <= 16", ncopy);
@@ -4146,7 +4435,7 @@ static int batch_run(Ctx* c, const cc_batch_tuning* tune, double fraction, uint3
   }
   const uint64_t total = start[n];
   HIP_TRY(hipMemcpy(c->b_hist_start, start.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice));
-  const uint32_t pool = tune->pool;
+  const uint32_t pool = persistent ? 1024u : tune->pool;
   int r = ensure_queue(c, pool);
   if (r) return r;
   // fresh tallies, deposits, counters, free slots and queue
@@ -4172,58 +4461,94 @@ static int batch_run(Ctx* c, const cc_batch_tuning* tune, double fraction, uint3
   L.MS.cmask = ncopy - 1;
   L.MS.cstride = 2 * c->b_nvox;
   const int bs = tune->block;
-  const int nb = (int)std::min<size_t>((pool + bs - 1) / bs, 8192);
-  // two rounds (list A -> B -> A) and the status copy, captured once and replayed
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  for (int half = 0; half < 2; ++half) {
-    const int cur = half == 0 ? C_ACT_A : C_ACT_B, nxt = half == 0 ? C_ACT_B : C_ACT_A;
-    int* act_cur = half == 0 ? c->act0 : c->act1;
-    int* act_nxt = half == 0 ? c->act1 : c->act0;
-    hipLaunchKernelGGL(k_refill, dim3(nb), dim3(bs), 0, c->stream, L, act_cur, c->cnt + cur, pool);
-    hipLaunchKernelGGL(k_commit_and_reset, dim3(1), dim3(64), 0, c->stream, L, pool, nxt);
-    hipLaunchKernelGGL((k_flight<false, true>), dim3(nb), dim3(bs), 0, c->stream, L, (const int*)act_cur,
-                       (const uint32_t*)(c->cnt + cur), c->qp, c->qc, c->qr, (int*)nullptr);
-    hipLaunchKernelGGL(k_interact, dim3(nb), dim3(bs), 0, c->stream, L, (const int*)c->qp, (const int*)c->qc,
-                       (const int*)c->qr, act_nxt, c->cnt + nxt);
-  }
-  hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(64), 0, c->stream, L, (const uint32_t*)(c->cnt + C_ACT_A),
-                     c->b_status);
-  hipError_t ce = hipMemcpyAsync(c->h_status, c->b_status, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                 c->stream);
-  hipError_t ee = hipStreamEndCapture(c->stream, &graph);
-  if (ce != hipSuccess || ee != hipSuccess) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return fail(CC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce != hipSuccess ? ce : ee));
-  }
-  hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  if (ie != hipSuccess) {
-    (void)hipGraphDestroy(graph);
-    return fail(CC_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
-  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(hipEventRecord(e0, c->stream));
   uint64_t launches = 0;
   int rc = CC_OK;
-  const uint64_t max_launches = (uint64_t)kMaxRounds;
-  for (;;) {
-    for (int k = 0; k < tune->rounds_per_sync / 2; ++k) {
-      hipError_t le = hipGraphLaunch(exec, c->stream);
-      if (le != hipSuccess) { rc = fail(CC_ERR_HIP, "graph launch failed: %s", hipGetErrorString(le)); break; }
-      ++launches;
-    }
-    if (rc) break;
-    hipError_t se = hipStreamSynchronize(c->stream);
-    if (se != hipSuccess) { rc = fail(CC_ERR_HIP, "stream sync: %s", hipGetErrorString(se)); break; }
-    if (c->h_status[0] >= total && c->h_status[1] == 0) break;
-    if (launches >= max_launches) { rc = fail(CC_ERR_DEVICE, "batched run did not drain"); break; }
-  }
-  (void)hipEventRecord(e1, c->stream);
-  (void)hipStreamSynchronize(c->stream);
   float ms = 0.f;
+  if (persistent) {
+    // every block resident at once: the occupancy of the kernel at this block size on every CU
+    int per_cu = 0, ncu = 0;
+    if (bs == 512)
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_persistent<512>, bs, 0));
+    else
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_persistent<256>, bs, 0));
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    const int nbp = std::max(1, per_cu) * std::max(1, ncu);
+    // the launch, captured as the graph's one node and launched once
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    if (bs == 512)
+      hipLaunchKernelGGL(k_batch_persistent<512>, dim3(nbp), dim3(bs), 0, c->stream, L, (int)tune->epoch);
+    else
+      hipLaunchKernelGGL(k_batch_persistent<256>, dim3(nbp), dim3(bs), 0, c->stream, L, (int)tune->epoch);
+    hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+    if (ee != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return fail(CC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ee));
+    }
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(CC_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+    }
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    hipError_t le = hipGraphLaunch(exec, c->stream);
+    if (le != hipSuccess) rc = fail(CC_ERR_HIP, "graph launch failed: %s", hipGetErrorString(le));
+    launches = rc ? 0 : 1;
+    (void)hipEventRecord(e1, c->stream);
+    hipError_t se = hipStreamSynchronize(c->stream);
+    if (!rc && se != hipSuccess) rc = fail(CC_ERR_HIP, "stream sync: %s", hipGetErrorString(se));
+  } else {
+    const int nb = (int)std::min<size_t>((pool + bs - 1) / bs, 8192);
+    // two rounds (list A -> B -> A) and the status copy, captured once and replayed
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int half = 0; half < 2; ++half) {
+      const int cur = half == 0 ? C_ACT_A : C_ACT_B, nxt = half == 0 ? C_ACT_B : C_ACT_A;
+      int* act_cur = half == 0 ? c->act0 : c->act1;
+      int* act_nxt = half == 0 ? c->act1 : c->act0;
+      hipLaunchKernelGGL(k_refill, dim3(nb), dim3(bs), 0, c->stream, L, act_cur, c->cnt + cur, pool);
+      hipLaunchKernelGGL(k_commit_and_reset, dim3(1), dim3(64), 0, c->stream, L, pool, nxt);
+      hipLaunchKernelGGL((k_flight<false, true>), dim3(nb), dim3(bs), 0, c->stream, L, (const int*)act_cur,
+                         (const uint32_t*)(c->cnt + cur), c->qp, c->qc, c->qr, (int*)nullptr);
+      hipLaunchKernelGGL(k_interact, dim3(nb), dim3(bs), 0, c->stream, L, (const int*)c->qp, (const int*)c->qc,
+                         (const int*)c->qr, act_nxt, c->cnt + nxt);
+    }
+    hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(64), 0, c->stream, L, (const uint32_t*)(c->cnt + C_ACT_A),
+                       c->b_status);
+    hipError_t ce = hipMemcpyAsync(c->h_status, c->b_status, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                   c->stream);
+    hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+    if (ce != hipSuccess || ee != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return fail(CC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce != hipSuccess ? ce : ee));
+    }
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(CC_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+    }
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    const uint64_t max_launches = (uint64_t)kMaxRounds;
+    for (;;) {
+      for (int k = 0; k < tune->rounds_per_sync / 2; ++k) {
+        hipError_t le = hipGraphLaunch(exec, c->stream);
+        if (le != hipSuccess) { rc = fail(CC_ERR_HIP, "graph launch failed: %s", hipGetErrorString(le)); break; }
+        ++launches;
+      }
+      if (rc) break;
+      hipError_t se = hipStreamSynchronize(c->stream);
+      if (se != hipSuccess) { rc = fail(CC_ERR_HIP, "stream sync: %s", hipGetErrorString(se)); break; }
+      if (c->h_status[0] >= total && c->h_status[1] == 0) break;
+      if (launches >= max_launches) { rc = fail(CC_ERR_DEVICE, "batched run did not drain"); break; }
+    }
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+  }
   (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -4251,7 +4576,7 @@ static int batch_run(Ctx* c, const cc_batch_tuning* tune, double fraction, uint3
   out->n_events = st[S_EVT];
   out->n_flights = st[S_FLIGHT];
   out->n_voxel_visits = st[S_VOX];
-  out->rounds = 2 * launches;
+  out->rounds = persistent ? 0 : 2 * launches;
   out->graph_launches = launches;
   out->device_ms = ms;
   out->error_bits = (uint32_t)e;
@@ -4280,18 +4605,29 @@ int cc_batch_read(void* ctx, int32_t scene, double* kerma_keV, double* h10_pSv_c
   return CC_OK;
 }
 
+// Tally copies for the autotuner: the largest power of two up to 16 whose copies of every scene's
+// grid fit in half the free device memory (the present tally buffer counted as free).
+static uint32_t fit_copies(Ctx* c) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 8u;
+  const size_t per = 2 * c->b_nvox * sizeof(double);
+  const size_t avail = fr / 2 + c->b_tally_cap * sizeof(double);
+  uint32_t k = 16u;
+  while (k > 1u && (size_t)k * per > avail) k >>= 1;
+  return k;
+}
+
 int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tuning* best, double* table,
                       int32_t max_rows, int32_t* n_rows) {
   Ctx* c = (Ctx*)ctx;
   if (!c || !best) return fail(CC_ERR_ARG, "null argument");
-  // Coordinate search from (block 128, 4 rounds per sync, pool 2^20): the pool first (it moves
-  // the rate most), then the block size at the best pool.  Four timing runs instead of all 24
-  // shapes (rounds per sync moved the rate by 1 % over 4..16 on the MI355X, so it is not
-  // searched); timing runs skip the tally clear and fold, which do not depend on the shape.  The
-  // tallies are left cleared, as after an upload.
-  const uint32_t pools[] = {1u << 19, 1u << 20, 1u << 21};
-  const int blocks[] = {128, 256};
-  cc_batch_tuning cur{128, 4, 1u << 20, 8u};
+  if (c->nscene <= 0) return fail(CC_ERR_STATE, "no scenes uploaded (cc_batch_upload)");
+  // Four timing runs, a coordinate search from the persistent form at (block 512, epoch 2): block
+  // 256, then epoch 8 at the better block, then the event-based form at the shape an earlier
+  // search over pools {2^19, 2^20, 2^21} and blocks {128, 256} found best on the MI355X (block
+  // 128, 4 rounds per sync, pool 2^19).  Timing runs skip the tally clear and fold, which do not
+  // depend on the shape.  The tallies are left cleared, as after an upload.
+  const uint32_t copies = fit_copies(c);
   double best_rate = -1.0;
   int rows = 0;
   auto trial = [&](const cc_batch_tuning& t) -> int {
@@ -4300,8 +4636,9 @@ int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tunin
     if (r) return r;
     const double rate = o.device_ms > 0 ? (double)o.n_histories / (o.device_ms * 1e-3) : 0.0;
     if (table && rows < max_rows) {
-      table[4 * rows] = t.block; table[4 * rows + 1] = t.rounds_per_sync; table[4 * rows + 2] = t.pool;
-      table[4 * rows + 3] = rate;
+      double* const row = table + 7 * rows;
+      row[0] = t.persistent; row[1] = t.block; row[2] = t.epoch; row[3] = t.rounds_per_sync; row[4] = t.pool;
+      row[5] = t.tally_copies; row[6] = rate;
     }
     ++rows;
     if (rate > best_rate) {
@@ -4311,18 +4648,13 @@ int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tunin
     return CC_OK;
   };
   int r;
-  for (uint32_t p : pools) {
-    cc_batch_tuning t = cur;
-    t.pool = p;
-    if ((r = trial(t))) return r;
-  }
-  cur = *best;
-  for (int b : blocks) {
-    if (b == cur.block) continue;
-    cc_batch_tuning t = cur;
-    t.block = b;
-    if ((r = trial(t))) return r;
-  }
+  const cc_batch_tuning p512{512, 4, 1u << 19, copies, 1, 2}, p256{256, 4, 1u << 19, copies, 1, 2};
+  if ((r = trial(p512)) || (r = trial(p256))) return r;
+  cc_batch_tuning t = *best;
+  t.epoch = 8;
+  if ((r = trial(t))) return r;
+  const cc_batch_tuning ev{128, 4, 1u << 19, copies, 0, 2};
+  if ((r = trial(ev))) return r;
   if (n_rows) *n_rows = rows;
   // leave the scene tallies as a fresh upload leaves them
   const size_t nd = (size_t)(best->tally_copies ? best->tally_copies : 1u) * 2 * c->b_nvox;

@@ -304,15 +304,22 @@ int cc_trace_read(void* ctx, uint64_t max_records, cc_trace_record* out, uint64_
 /* Batched runs (no counterpart in the reference, which runs one scene per call): many scenes in
  * one particle pool.  Scene s's histories 0 .. n_s - 1 draw from the streams keyed by (seed_s, h),
  * so each scene's tallies are those of cc_run(seed_s, first 0) up to FP64 summation order.  A
- * device cursor hands histories to free pool slots; two transport rounds and the status copy
- * are captured in one hipGraph and replayed until the queue drains.  Tallies: kerma and H*(10)
- * totals per scene and per-material deposits (no batch moments). */
+ * device cursor hands out the histories, in one of two forms: (persistent = 0) to free slots of a
+ * particle pool, with two event-based transport rounds and the status copy captured in one hipGraph
+ * and replayed until the queue drains; (persistent = 1) to the lanes of one persistent
+ * history-based kernel, captured as a one-node hipGraph and launched once, each wave claiming
+ * histories from the cursor as its lanes go idle.  Both give the same event counters; tallies and
+ * deposits agree up to FP64 summation order.  Tallies: kerma and H*(10) totals per scene and
+ * per-material deposits (no batch moments). */
 typedef struct {
   int32_t block;                /* threads per block of the transport kernels (64, 128, 192 or 256) */
-  int32_t rounds_per_sync;      /* rounds between host checks of the queue (even) */
-  uint32_t pool;                /* photons in flight */
+  int32_t rounds_per_sync;      /* rounds between host checks of the queue (even; event-based form) */
+  uint32_t pool;                /* photons in flight (event-based form) */
   uint32_t tally_copies;        /* copies of the scene tallies (0 or a power of two <= 16; 0 = 1): workgroup
                                    b adds into copy b mod copies, folded once the queue has drained */
+  int32_t persistent;           /* 1: one persistent launch; 0: replayed event-based rounds */
+  int32_t epoch;                /* persistent form: wave iterations between flushes of a workgroup's
+                                   LDS tally cache into the grid (1 .. 65536) */
 } cc_batch_tuning;
 
 typedef struct {
@@ -332,11 +339,12 @@ int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int
 /* Runs ceil(fraction n_s) histories of every scene (fraction in (0, 1]); tallies start from zero. */
 int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32_t flags, cc_batch_out* out);
 int cc_batch_read(void* ctx, int32_t scene, double* kerma_keV, double* h10_pSv_cm3, double* edep_keV);
-/* Times launch shapes, with 8 tally copies, on a fraction of the histories and returns the
- * fastest: a coordinate search from (block 128, rounds_per_sync 4, pool 2^20) over pool {2^19,
- * 2^20, 2^21}, then block {128, 256}, four timing runs.  table (4 doubles per row: block,
- * rounds_per_sync, pool, histories/s; one row per run) may be NULL.  The scene tallies are
- * cleared afterwards. */
+/* Times launch shapes on a fraction of the histories and returns the fastest, four timing runs:
+ * the persistent form at block 512 and 256 (epoch 2), epoch 8 at the better block, and the
+ * event-based form at (block 128, rounds_per_sync 4, pool 2^19); tally copies: the largest power
+ * of two up to 16 whose copies fit in half the free device memory.  table (7 doubles per row:
+ * persistent, block, epoch, rounds_per_sync, pool, tally_copies, histories/s; one row per run)
+ * may be NULL.  The scene tallies are cleared afterwards. */
 int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tuning* best, double* table,
                       int32_t max_rows, int32_t* n_rows);
 

@@ -57,6 +57,7 @@ def test_ctypes_layouts_match_the_header(tmp_path):
               ("CC_EV_RAYLEIGH", N.EV_RAYLEIGH),
               ("sizeof(cc_batch_tuning)", C.sizeof(N.CcBatchTuning)),
               ("offsetof(cc_batch_tuning, pool)", N.CcBatchTuning.pool.offset),
+              ("offsetof(cc_batch_tuning, epoch)", N.CcBatchTuning.epoch.offset),
               ("sizeof(cc_batch_out)", C.sizeof(N.CcBatchOut)),
               ("offsetof(cc_batch_out, device_ms)", N.CcBatchOut.device_ms.offset),
               ("offsetof(cc_batch_out, error_bits)", N.CcBatchOut.error_bits.offset),

@@ -1,12 +1,14 @@
-"""Batched transport (many scenes in one pool, hipGraph replay, device work queue).
+"""Batched transport (many scenes fed by one device work queue, in two forms: one persistent
+history-based launch, and event-based rounds in one particle pool replayed as a hipGraph).
 
 Every scene's history h uses the Philox stream keyed by (seed_s, h), exactly as
 history h of a single-scene run with seed_s, so a batched run must reproduce,
 scene by scene, the kerma and H*(10) grids and the deposits of the single-scene
 runs up to the order of FP64 additions, and the event counters exactly.  The
-launch shapes below include pools far smaller than the history count, so slots
-are recycled through the free-slot stack many times and the queue crosses scene
-boundaries inside one refill.
+event-based launch shapes below include pools far smaller than the history count, so
+slots are recycled through the free-slot stack many times and the queue crosses scene
+boundaries inside one refill; the persistent ones flush the LDS tally cache from every
+iteration to about once per workgroup.
 """
 import numpy as np
 import pytest
@@ -71,6 +73,32 @@ def test_batched_equals_single_scene_runs(engine, single, tune):
         np.testing.assert_allclose(e, r["edep"], rtol=1e-9, atol=1e-9)
 
 
+@pytest.mark.parametrize("block,epoch", [(512, 2), (256, 1), (512, 1000), (256, 7)])
+def test_persistent_equals_single_scene_runs(engine, single, block, epoch):
+    """The one persistent launch: the same counters as the single-scene runs, the same grids
+    and deposits up to FP64 summation order.  epoch 1 flushes the LDS tally cache after every
+    iteration, epoch 1000 about once per workgroup."""
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd.batch import BatchedRun, Tuning
+    scenes, ref = single
+    b = BatchedRun(scenes, engine)
+    st = b.run(Tuning(block, tally_copies=4, persistent=True, epoch=epoch))
+    print(f"persistent block {block} epoch {epoch}: {st.n_histories} histories, {st.device_ms:.2f} ms", flush=True)
+    assert st.graph_launches == 1 and st.rounds == 0
+    got = np.array([st.n_absorbed, st.n_escaped, st.scatter_sum, st.n_fluorescence, st.n_events, st.n_flights,
+                    st.n_voxel_visits], dtype=np.int64)
+    np.testing.assert_array_equal(got, np.sum([r["cnt"] for r in ref], axis=0))
+    names = engine.pack.materials
+    for k, r in enumerate(ref):
+        kk, hh, edep = b.read(k)
+        np.testing.assert_allclose(kk, r["kerma"], rtol=1e-9, atol=1e-12 * r["kerma"].max())
+        np.testing.assert_allclose(hh, r["h10"], rtol=1e-9, atol=1e-12 * r["h10"].max())
+        e = np.zeros(N.CC_MAX_MAT)
+        for m, v in edep.items():
+            e[names.index(m)] = v
+        np.testing.assert_allclose(e, r["edep"], rtol=1e-9, atol=1e-9)
+
+
 def test_fraction_and_rerun_start_from_zero(engine, single):
     from chatcarlo_amd.batch import BatchedRun, Tuning
     scenes, ref = single
@@ -90,12 +118,16 @@ def test_autotune_returns_a_candidate(engine, single):
     from chatcarlo_amd.batch import BatchedRun, Tuning
     b = BatchedRun(_scenes(3, seed=5), engine)
     t, table = b.autotune(fraction=0.2, cached=False)
-    # the coordinate search: three pools at (128, 4), then the other block at the best pool
-    assert len(table) == 4 and np.all(table[:, 3] > 0)
-    assert sorted(int(p) for p in table[:3, 2]) == [1 << 19, 1 << 20, 1 << 21]
-    assert (t.block, t.rounds_per_sync, t.pool) in {(int(a), int(q), int(p)) for a, q, p, _ in table}
-    best = table[np.argmax(table[:, 3])]
-    assert (t.block, t.rounds_per_sync, t.pool) == (int(best[0]), int(best[1]), int(best[2]))
+    # the coordinate search: the persistent form at blocks 512 and 256, epoch 8 at the better
+    # block, then the event-based form
+    assert table.shape == (4, 7) and np.all(table[:, 6] > 0)
+    assert [int(x) for x in table[:, 0]] == [1, 1, 1, 0]
+    assert [int(x) for x in table[:2, 1]] == [512, 256] and [int(x) for x in table[:, 2]] == [2, 2, 8, 2]
+    assert int(table[2, 1]) == int(table[np.argmax(table[:2, 6]), 1])
+    assert (int(table[3, 1]), int(table[3, 3]), int(table[3, 4])) == (128, 4, 1 << 19)
+    best = table[np.argmax(table[:, 6])]
+    assert (int(t.persistent), t.block, t.epoch, t.rounds_per_sync, t.pool, t.tally_copies) == \
+        tuple(int(x) for x in best[:6])
     # the same workload again: the cached shape, no timing
     t2, table2 = b.autotune(fraction=0.2)
     assert t2 == t and np.array_equal(table2, table)
@@ -112,7 +144,8 @@ def test_autotune_returns_a_candidate(engine, single):
 def test_bad_tuning_is_rejected(engine):
     from chatcarlo_amd.batch import BatchedRun, Tuning
     b = BatchedRun(_scenes(2, seed=3), engine)
-    for t in (Tuning(100, 8, 4096), Tuning(256, 3, 4096), Tuning(256, 8, 10), Tuning(512, 8, 4096)):
+    for t in (Tuning(100, 8, 4096), Tuning(256, 3, 4096), Tuning(256, 8, 10), Tuning(512, 8, 4096),
+              Tuning(256, persistent=True, epoch=0), Tuning(320, persistent=True), Tuning(128, persistent=True)):
         with pytest.raises(ValueError):
             b.run(t)
     with pytest.raises(ValueError):
