@@ -16,7 +16,7 @@ CC_OK, CC_ERR_ARG, CC_ERR_HIP, CC_ERR_STATE, CC_ERR_DEVICE = 0, -1, -2, -3, -4
 CC_MAX_PRIMS, CC_MAX_MAT = 64, 32
 RUN_FLUORESCENCE, RUN_GRID, RUN_TRACK_UNCERTAINTY, RUN_TIMING = 1, 2, 4, 8
 RUN_REGION, RUN_RECORD, RUN_IMPORTANCE, RUN_SOURCE_BIAS = 16, 32, 64, 128
-RUN_TRACE, RUN_PER_BATCH = 256, 512
+RUN_TRACE, RUN_PER_BATCH, RUN_PERSISTENT = 256, 512, 1024
 NOALLOC_MAX_PHOTONS = 1 << 16   # photons per cc_transport_noalloc call
 EV_BOUNDARY, EV_ESCAPE, EV_PHOTOELECTRIC, EV_FLUORESCENCE, EV_COMPTON, EV_RAYLEIGH = range(6)
 TRACE_DTYPE = np.dtype([("photon", np.uint32), ("flight", np.int32), ("step", np.int32), ("material", np.int32),

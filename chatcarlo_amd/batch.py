@@ -157,8 +157,8 @@ class BatchedRun:
         return h.hexdigest()
 
     def autotune(self, fraction: float = 0.02, cached: bool = True):
-        """Times launch shapes on a fraction of the histories (a four-run coordinate search over
-        the persistent and event-based forms, cc_batch_autotune); returns the fastest and the
+        """Times launch shapes on a fraction of the histories (three runs over the persistent and
+        event-based forms, cc_batch_autotune); returns the fastest and the
         table (persistent, block, epoch, rounds_per_sync, pool, tally_copies, histories/s) of the
         runs.  With cached, a workload tuned before in this process returns its shape and table
         without timing."""

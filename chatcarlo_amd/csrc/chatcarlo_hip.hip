@@ -58,7 +58,7 @@ constexpr int kTrialCap = 100000;
 enum { F_ABSORBED = 1, F_ESCAPED = 2, F_DEAD = 4 };
 // device counter slots (uint32)
 enum { C_ACT_A = 0, C_ACT_B = 1, C_QP = 2, C_QC = 3, C_QR = 4, C_IMP = 5, C_POOL = 6, C_COLL = 7, C_RESV = 8,
-       C_NCNT = 16 };
+       C_DRAIN = 9, C_NCNT = 16 };
 // device stats slots (u64)
 enum { S_ABS = 0, S_ESC = 1, S_SCAT = 2, S_FLUOR = 3, S_EVT = 4, S_FLIGHT = 5, S_VOX = 6, S_SPLIT = 7, S_ROUL = 8,
        S_NSTAT = 10 };
@@ -1206,52 +1206,61 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
 }
 
 // ---------------------------------------------------------------------------
-// batched runs in one persistent launch
+// history-based persistent transport: batched scenes, and the drain of a streamed run
 // ---------------------------------------------------------------------------
-// History-based transport of the batched index space: each lane carries one history from its
-// source draw to its end (escape, absorption or a device error) and then starts the next, so
-// no photon record goes through memory and no round waits on its slowest photon.  A wave
-// claims kPersistChunk consecutive histories from the device cursor at a time and deals them
-// to its idle lanes.  One iteration of a wave runs, for its lanes in turn: the interactions
-// that ended the previous iteration's flights, the source draws of idle lanes, one boundary
-// step of every flight, and the tally of the previous iteration's segments (one step late, as
-// in k_flight, so that the walk's atomics follow the step's loads instead of holding them up).
-// A history draws the stream (scene seed, history, tag 0) in the order of the event-based
-// rounds and every step is the arithmetic of k_refill, k_flight and k_interact, so the event
-// counters equal those of the rounds and the tallies and deposits differ only in the order of
-// their FP64 sums.  Every `epoch` iterations the block adds its LDS tally cache into the grid
-// copies and clears it; a block leaves once the cursor has passed the end and every lane of
-// every wave is idle.
-constexpr uint32_t kPersistChunk = 128;   // histories a wave claims at once
+// Each lane carries one history from its start to its end (escape, absorption or a device
+// error) and then takes the next, so no photon record goes through memory and no round waits on
+// its slowest photon.  A wave claims kPersistChunk consecutive histories from the device cursor
+// at a time and deals them to its idle lanes, which draw their source photons:
+//  - batched scenes (kStream = false): histories of the batched index space, drawn as k_refill
+//    draws them, on their scenes' primitives (global memory) and grids;
+//  - a streamed run (kStream = true): histories of the run's group of batches, drawn as
+//    k_refill_stream draws them, on the scene's primitives staged in LDS, each tallied into the
+//    slot of its batch.  Photons already in flight come first: the entries of an active list
+//    (cnt_act of them; a wave claims entries for its idle lanes through `claim`), so that the
+//    launch can also carry the last photons of event-based rounds to their ends.
+// One iteration of a wave runs, for its lanes in turn: the interactions that ended the previous
+// iteration's flights, the starts of idle lanes, one boundary step of every flight, and the tally
+// of the previous iteration's segments (one step late, as in k_flight, so that the walk's atomics
+// follow the step's loads instead of holding them up).  A history draws its stream (seed,
+// history, tag) in the order of the event-based rounds and every step is the arithmetic of
+// k_refill, k_flight and k_interact, so the event counters equal those of the rounds and the
+// tallies and deposits differ only in the order of their FP64 sums.  Every `epoch` iterations the
+// block adds its LDS tally cache into the grid copies and clears it; a block leaves once its work
+// source is exhausted and every lane of every wave is idle.
+constexpr uint32_t kPersistChunk = 128;   // histories a batched-scene wave claims at once
 constexpr int kMaxFlights = 1 << 12;      // flights of one history (CC_DEV_ROUND_CAP beyond)
 // Two waves per SIMD (every value in registers) and the CU's LDS as tally cache: 4000 slots per
-// block of 256 threads (two blocks per CU) or 8000 per block of 512 (one).  Three waves with 2304
-// slots were 7 % slower than the first; the second, whose eight waves share one cache, is 16 %
-// faster again.
-template <int kThreads>
+// block of 256 threads (two blocks per CU) or 8000 per block of 512 (one).  On the 64-scene sweep
+// three waves with 2304 slots were 7 % slower than the first; the second, whose eight waves share
+// one cache, is 16 % faster again.
+template <int kThreads, bool kStream>
 struct PersistShape {
   static_assert(kThreads == 256 || kThreads == 512, "blocks of 256 or 512 threads");
-  static constexpr int kSlots = 4000 * (kThreads / 256);
+  // the drain stages the scene's primitives in LDS as well
+  static constexpr int kSlots = 4000 * (kThreads / 256) - (kStream ? 160 * (kThreads / 256) : 0);
 };
 enum { PH_IDLE = 0, PH_FLY = 1, PH_HIT = 2 };
 
-template <int kThreads>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_batch_persistent(Launch L,
-                                                                                                     int epoch) {
-  constexpr int kPersistSlots = PersistShape<kThreads>::kSlots;
+template <int kThreads, bool kStream>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_persistent(
+    Launch L, Src src, int epoch, const int* act, const uint32_t* cnt_act, uint32_t* claim) {
+  constexpr int kPersistSlots = PersistShape<kThreads, kStream>::kSlots;
   __shared__ TallyCacheT<kPersistSlots> tc_lds;
+  __shared__ Prim sp_lds[kStream ? CC_MAX_PRIMS : 1];
   TallyCacheT<kPersistSlots>* const tcache = &tc_lds;
   cache_init(tcache);
+  if (kStream) stage_prims(L.prims, sp_lds, L.W.n_prim);
   const MultiScene& M = L.MS;
   const Tables& T = L.T;
   const int lane = __lane_id();
-  // the lane's history
+  // the lane's history: scene (batched) or batch slot (drain), counter, tag, weight
   int phase = PH_IDLE, scene = -1, type = 0, mat = 0, nfl = 0;
-  uint32_t hid = 0, ctr = 0;
-  uint64_t key = 0;
-  float E = 1.f, tau = 0.f;
+  uint32_t hid = 0, ctr = 0, tag = 0;
+  uint64_t key = kStream ? L.seed : 0;
+  float E = 1.f, tau = 0.f, w = 1.f;
   float p[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 1.f};
-  const Prim* sp = M.prims;
+  const Prim* sp = kStream ? sp_lds : M.prims;
   World Wd = L.W;
   // the current flight
   bool fresh = false;
@@ -1260,7 +1269,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   int mc = -1, it = 0;
   EPos ep{0, 0.f, 0.f};
   double wh = 0.0;
-  // the segment waiting for its tally, and the grid of its scene
+  // the segment waiting for its tally, and the grid of its scene (batched scenes)
   float pend_p[3] = {0.f, 0.f, 0.f}, pend_d[3] = {0.f, 0.f, 1.f}, pend_len = 0.f;
   double pend_wk = 0.0, pend_wh = 0.0;
   bool pend = false;
@@ -1268,16 +1277,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   GridDev G = L.G;
   // event counts since the last epoch end
   uint32_t n_evt = 0, n_vox = 0, n_fl = 0, n_esc = 0, n_abs = 0, n_fluor = 0, n_scat = 0;
-  // the wave's claimed histories [wnext, wend)
+  // batched scenes: the wave's claimed histories [wnext, wend)
   unsigned long long wnext = 0, wend = 0;
-  bool exhausted = false, done = false;
+  const uint32_t n_act = kStream && cnt_act ? *cnt_act : 0u;
+  bool act_done = n_act == 0u, exhausted = false, done = false;
   for (;;) {
     for (int itn = 0; itn < epoch && !done; ++itn) {
       // 1. interactions (photo_one, compton_one, rayleigh_one on registers)
       double dep = 0.0;
       bool dep_on = false;
       if (phase == PH_HIT) {
-        PhiloxRegRng rng(key, (uint64_t)hid, 0u, ctr);
+        PhiloxRegRng rng(key, L.hist0 + (uint64_t)hid, tag, ctr);
         const EPos epi = energy_pos(T, E);
         bool fly = true;
         if (type == 0) {
@@ -1287,7 +1297,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             int el;
             emit = fluor_sample(T, mat, E, epi, rng, el, eline);
           }
-          dep = (double)(emit ? E - eline : E);
+          dep = (double)w * (double)(emit ? E - eline : E);
           if (emit) {
             isotropic_dir(d, rng.next(), rng.next());
             tau = -logf(rng.next());
@@ -1303,7 +1313,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
           const float eps = compton_sample(T, mat, E, epi, rng, kTrialCap, cs, sn, el, trials);
           if (trials == 0) raise_err(L.err, CC_DEV_ITER_CAP);
           const float enew = E * eps;
-          dep = (double)E - (double)enew;
+          dep = (double)w * ((double)E - (double)enew);
           rotate_dir(d, cs, sn, rng.next());
           tau = -logf(rng.next());
           E = enew;
@@ -1324,10 +1334,36 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         fresh = fly;
       }
       wave_add_by_key(dep_on, scene * CC_MAX_MAT + mat, dep, dep_copy(L));
-      // 2. idle lanes start the next histories of the wave's claim (k_refill)
-      for (int pass = 0; pass < 3; ++pass) {
+      // 2. idle lanes take their next work
+      for (int pass = 0; pass < 4; ++pass) {
         const unsigned long long need = __ballot(phase == PH_IDLE);
         if (need == 0ull || exhausted) break;
+        const uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+        if (kStream && !act_done) {
+          // entries of the active list, one claim per wave
+          uint32_t base = 0u;
+          if (lane == 0) base = atomicAdd(claim, (uint32_t)__popcll(need));
+          base = __shfl(base, 0);
+          if (base + (uint32_t)__popcll(need) >= n_act) act_done = true;
+          const uint32_t j = base + rank;
+          if (phase == PH_IDLE && j < n_act) {
+            const PhotonWords P = load_photon(L.S, act[j]);
+            p[0] = P.f(W_X); p[1] = P.f(W_Y); p[2] = P.f(W_Z);
+            d[0] = P.f(W_DX); d[1] = P.f(W_DY); d[2] = P.f(W_DZ);
+            E = P.f(W_E);
+            tau = P.f(W_TAU);
+            w = P.f(W_WT);
+            ctr = P.u(W_CTR);
+            hid = P.u(W_HID);
+            tag = P.u(W_TAG);
+            scene = (int)P.u(W_SC);
+            nfl = 0;
+            phase = PH_FLY;
+            fresh = true;
+          }
+          continue;
+        }
+        // the histories of the wave's claim
         if (wnext >= wend) {
           unsigned long long base = 0ull;
           if (lane == 0) base = atomicAdd(M.cursor, (unsigned long long)kPersistChunk);
@@ -1340,29 +1376,47 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
           wend = min(base + (unsigned long long)kPersistChunk, M.total);
         }
         const uint32_t avail = (uint32_t)(wend - wnext);
-        const uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
-        if (phase == PH_IDLE && rank < avail) {
-          const unsigned long long g = wnext + rank;
-          int lo = 0, hi = M.n - 1;   // last scene whose first history is <= g
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (M.hist_start[mid] <= g) lo = mid; else hi = mid - 1;
+        if constexpr (kStream) {
+          // history h of the run (k_refill_stream): stream (seed, first + h), its batch's slot
+          if (phase == PH_IDLE && rank < avail) {
+            const uint32_t h = (uint32_t)(wnext + rank);   // a streamed run has fewer than 2^32 histories
+            PhiloxRng rng(L.seed, L.hist0 + (uint64_t)h, 0u, 0u);
+            sample_source(src, rng, p, d, E);
+            tau = -logf(rng.next());
+            ctr = rng.ctr;
+            hid = h;
+            tag = 0u;
+            w = 1.0f;
+            scene = (int)(h / (uint32_t)L.St.batch - (uint32_t)L.St.g0);
+            nfl = 0;
+            phase = PH_FLY;
+            fresh = true;
           }
-          if (lo != scene) {
-            scene = lo;
-            sp = M.prims + M.prim_off[lo];
-            Wd = M.worlds[lo];
-            key = M.seeds[lo];
+        } else {
+          // history g of the batched index space (k_refill)
+          if (phase == PH_IDLE && rank < avail) {
+            const unsigned long long g = wnext + rank;
+            int lo = 0, hi = M.n - 1;   // last scene whose first history is <= g
+            while (lo < hi) {
+              const int mid = (lo + hi + 1) >> 1;
+              if (M.hist_start[mid] <= g) lo = mid; else hi = mid - 1;
+            }
+            if (lo != scene) {
+              scene = lo;
+              sp = M.prims + M.prim_off[lo];
+              Wd = M.worlds[lo];
+              key = M.seeds[lo];
+            }
+            const unsigned long long h = g - M.hist_start[lo];
+            PhiloxRng rng(key, (uint64_t)h, 0u, 0u);
+            sample_source(M.srcs[lo], rng, p, d, E);
+            tau = -logf(rng.next());
+            ctr = rng.ctr;
+            hid = (uint32_t)h;
+            nfl = 0;
+            phase = PH_FLY;
+            fresh = true;
           }
-          const unsigned long long h = g - M.hist_start[lo];
-          PhiloxRng rng(key, (uint64_t)h, 0u, 0u);
-          sample_source(M.srcs[lo], rng, p, d, E);
-          tau = -logf(rng.next());
-          ctr = rng.ctr;
-          hid = (uint32_t)h;
-          nfl = 0;
-          phase = PH_FLY;
-          fresh = true;
         }
         wnext += min((uint32_t)__popcll(need), avail);
       }
@@ -1381,7 +1435,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
           phase = PH_IDLE;
         } else {
           ep = energy_pos(T, E);
-          wh = (double)h10_coeff(T.h10, T.n_h10, E);
+          wh = (double)w * (double)h10_coeff(T.h10, T.n_h10, E);
 #pragma unroll
           for (int k = 0; k < 3; ++k) o0[k] = p[k];
           eps0 = 1e-6f + 1.2e-7f * fmaxf(fabsf(p[0]), fmaxf(fabsf(p[1]), fabsf(p[2])));
@@ -1413,11 +1467,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
           const float taub = mu * tb;
           const bool inter = tau < taub;
           ds = inter ? tau / fmaxf(mu, 1e-30f) : tb;
-          wk = (double)E * (double)muen;
-          seg = true;
+          wk = (double)w * (double)E * (double)muen;
+          seg = L.grid_on != 0;   // a streamed run may tally no grid
           if (inter) {
             tc += ds;
-            PhiloxRng rng(key, (uint64_t)hid, 0u, ctr);
+            PhiloxRng rng(key, L.hist0 + (uint64_t)hid, tag, ctr);
             type = sample_type(T, m, E, ep, rng.next());
             ctr = rng.ctr;
             mat = m;
@@ -1438,13 +1492,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
           }
         }
       }
-      // 4. the tally of the previous iteration's segments; this step's wait for the next (each
-      // lane walks its own: the wave's segments walked work-balanced were measured 8 % slower here)
-      if (pend && pend_sc != gscene) {
-        G = M.grids[pend_sc];
-        gscene = pend_sc;
+      // 4. the tally of the previous iteration's segments; this step's wait for the next
+      if constexpr (kStream) {
+        // one scene: the wave's segments walked work-balanced, each into its batch slot
+        n_vox += (uint32_t)dda_balanced<false>(L.G, M.grids, pend_p, pend_d, pend_len, pend_wk, pend_wh, pend,
+                                               (uint32_t)pend_sc, tcache, L.St.slot_stride);
+      } else {
+        // each lane walks its own (the wave's segments walked work-balanced, when they share a
+        // scene, were measured 8 % slower here)
+        if (pend && pend_sc != gscene) {
+          G = M.grids[pend_sc];
+          gscene = pend_sc;
+        }
+        n_vox += (uint32_t)dda_score(G, pend_p, pend_d, pend_len, pend_wk, pend_wh, pend, tcache);
       }
-      n_vox += (uint32_t)dda_score(G, pend_p, pend_d, pend_len, pend_wk, pend_wh, pend, tcache);
       pend = seg;
       if (seg) {
 #pragma unroll
@@ -1467,10 +1528,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     wave_sum_small(n_fluor, stat_copy(L) + S_FLUOR);
     wave_sum_small(n_scat, stat_copy(L) + S_SCAT);
     n_evt = n_vox = n_fl = n_esc = n_abs = n_fluor = n_scat = 0;
-    // the block's cached tallies into its copy of the grids, then a clear cache
+    // the block's cached tallies into its copy of the grid(s), then a clear cache
     __syncthreads();
-    {
-      double* const t = M.t0 + (size_t)(blockIdx.x & M.cmask) * M.cstride;
+    if (L.grid_on) {
+      double* const t = (kStream ? L.G.t + (size_t)(blockIdx.x & L.G.cmask) * L.G.cstride
+                                 : M.t0 + (size_t)(blockIdx.x & M.cmask) * M.cstride);
       for (int e = threadIdx.x; e < 2 * kPersistSlots; e += blockDim.x) {
         const int k = e >> 1, c = e & 1;
         const uint32_t v = tcache->key[k];
@@ -3334,12 +3396,22 @@ int cc_load_grid(void* ctx, const double* kerma, const double* h10, const double
 constexpr uint32_t kStreamPool = 1u << 21;      // photons in flight in a streamed run
 constexpr int kStreamSlots = 64;                 // batches in flight at most (one group)
 constexpr size_t kStreamTallyBytes = 8ull << 30; // cap on the group's tally slots (all copies)
+// The drain: once every history of a group has started, the rounds go on until at most
+// kDrainPerLane photons per resident lane of the drain kernel are left in flight (the status is
+// read after every round from then on), and one persistent launch carries those to their ends.
+constexpr int kDrainPerLane = 4;
+constexpr int kDrainEpoch = 32;    // wave iterations between tally-cache flushes of the drain
+// CC_RUN_PERSISTENT: each group of batches in one persistent launch of 256-thread blocks, flushing
+// every 8 iterations (the fastest of 512 / 256 threads and epochs 4 to 128 on the chest room, 4 %
+// slower there than the rounds and the drain: 25.4 against 24.4 ms per 1e7 histories)
+constexpr int kStreamEpoch = 8;
 
 // A plain analog run with the pool fed from the device history cursor (see Stream).
 int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t first, uint32_t batch,
                uint32_t flags, cc_run_out* out) {
   const bool grid_on = (flags & CC_RUN_GRID) != 0;
   const bool track = c->track && (flags & CC_RUN_TRACK_UNCERTAINTY);
+  const bool persistent = (flags & CC_RUN_PERSISTENT) != 0;
   const uint64_t nbt = (n_hist + batch - 1) / batch;
   const uint32_t pool = (uint32_t)std::max<uint64_t>(1024, std::min<uint64_t>(kStreamPool, n_hist));
   int r = ensure_queue(c, pool);
@@ -3375,6 +3447,16 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
   Timer tm{c, (flags & CC_RUN_TIMING) != 0};
   const int nbk = grid_blocks(pool);
   int err_bits = 0;
+  int drain_blocks = 0, stream_blocks = 0;   // every block of the persistent launches resident at once
+  {
+    int per_cu = 0, per_cu2 = 0, ncu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent<512, true>, 512, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, k_persistent<256, true>, 256, 0));
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    drain_blocks = std::max(1, per_cu) * std::max(1, ncu);
+    stream_blocks = std::max(1, per_cu2) * std::max(1, ncu);
+  }
+  const unsigned long long drain_below = (unsigned long long)drain_blocks * 512ull * kDrainPerLane;
   for (uint64_t g0 = 0; g0 < nbt; g0 += (uint64_t)slots) {
     const int ns = (int)std::min<uint64_t>((uint64_t)slots, nbt - g0);
     const uint64_t h_begin = g0 * batch, h_end = std::min<uint64_t>(n_hist, (g0 + ns) * (uint64_t)batch);
@@ -3406,8 +3488,16 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
     int* act_cur = c->act0;
     int* act_nxt = c->act1;
     uint32_t rounds = 0;
-    for (;;) {
-      for (int k = 0; k < kRoundChunk; ++k) {
+    int chunk = kRoundChunk;
+    if (persistent) {   // the group's histories in one persistent launch
+      tm.begin(K_OTHER);
+      hipLaunchKernelGGL((k_persistent<256, true>), dim3(stream_blocks), dim3(256), 0, c->stream, L, s, kStreamEpoch,
+                         (const int*)nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr);
+      tm.end();
+      HIP_TRY(hipGetLastError());
+    }
+    for (; !persistent;) {
+      for (int k = 0; k < chunk; ++k) {
         tm.begin(K_SOURCE);
         hipLaunchKernelGGL(k_refill_stream, dim3(nbk), dim3(kBlock), 0, c->stream, L, s, act_cur, c->cnt + cur, pool);
         hipLaunchKernelGGL(k_commit_and_reset, dim3(1), dim3(64), 0, c->stream, L, pool, nxt);
@@ -3423,7 +3513,7 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
         std::swap(cur, nxt);
         std::swap(act_cur, act_nxt);
       }
-      rounds += kRoundChunk;
+      rounds += chunk;
       hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(64), 0, c->stream, L, (const uint32_t*)(c->cnt + cur),
                          c->b_status);
       HIP_TRY(hipMemcpyAsync(c->h_status, c->b_status, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -3432,6 +3522,18 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
       HIP_TRY(hipStreamSynchronize(c->stream));
       tm.collect(out->kernel_ms, out->kernel_launches);
       if (c->h_status[0] >= h_end && c->h_status[1] == 0) break;
+      if (c->h_status[0] >= h_end) {
+        if (c->h_status[1] <= drain_below) {   // the last photons in flight, to their ends in one launch
+          hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, (int*)(c->cnt + C_DRAIN), (size_t)1, 0);
+          tm.begin(K_OTHER);
+          hipLaunchKernelGGL((k_persistent<512, true>), dim3(drain_blocks), dim3(512), 0, c->stream, L, s, kDrainEpoch,
+                             (const int*)act_cur, (const uint32_t*)(c->cnt + cur), c->cnt + C_DRAIN);
+          tm.end();
+          HIP_TRY(hipGetLastError());
+          break;
+        }
+        chunk = 1;
+      }
       if (rounds >= (uint32_t)kMaxRounds) return fail(CC_ERR_DEVICE, "streamed run did not drain");
     }
     if (rounds > out->rounds_max) out->rounds_max = rounds;
@@ -4471,17 +4573,20 @@ static int batch_run(Ctx* c, const cc_batch_tuning* tune, double fraction, uint3
     // every block resident at once: the occupancy of the kernel at this block size on every CU
     int per_cu = 0, ncu = 0;
     if (bs == 512)
-      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_persistent<512>, bs, 0));
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent<512, false>, bs, 0));
     else
-      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_persistent<256>, bs, 0));
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent<256, false>, bs, 0));
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
     const int nbp = std::max(1, per_cu) * std::max(1, ncu);
     // the launch, captured as the graph's one node and launched once
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const Src none{};
     if (bs == 512)
-      hipLaunchKernelGGL(k_batch_persistent<512>, dim3(nbp), dim3(bs), 0, c->stream, L, (int)tune->epoch);
+      hipLaunchKernelGGL((k_persistent<512, false>), dim3(nbp), dim3(bs), 0, c->stream, L, none, (int)tune->epoch,
+                         (const int*)nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr);
     else
-      hipLaunchKernelGGL(k_batch_persistent<256>, dim3(nbp), dim3(bs), 0, c->stream, L, (int)tune->epoch);
+      hipLaunchKernelGGL((k_persistent<256, false>), dim3(nbp), dim3(bs), 0, c->stream, L, none, (int)tune->epoch,
+                         (const int*)nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr);
     hipError_t ee = hipStreamEndCapture(c->stream, &graph);
     if (ee != hipSuccess) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -4622,11 +4727,11 @@ int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tunin
   Ctx* c = (Ctx*)ctx;
   if (!c || !best) return fail(CC_ERR_ARG, "null argument");
   if (c->nscene <= 0) return fail(CC_ERR_STATE, "no scenes uploaded (cc_batch_upload)");
-  // Four timing runs, a coordinate search from the persistent form at (block 512, epoch 2): block
-  // 256, then epoch 8 at the better block, then the event-based form at the shape an earlier
-  // search over pools {2^19, 2^20, 2^21} and blocks {128, 256} found best on the MI355X (block
-  // 128, 4 rounds per sync, pool 2^19).  Timing runs skip the tally clear and fold, which do not
-  // depend on the shape.  The tallies are left cleared, as after an upload.
+  // Three timing runs: the persistent form at blocks 512 and 256 (epoch 2; epochs 1 to 4 were
+  // within 2 % of each other on the 64-scene sweep and 8 was 4 % slower), and the event-based form
+  // at the shape a search over pools {2^19, 2^20, 2^21} and blocks {128, 256} found best on the
+  // MI355X (block 128, 4 rounds per sync, pool 2^19).  Timing runs skip the tally clear and fold,
+  // which do not depend on the shape.  The tallies are left cleared, as after an upload.
   const uint32_t copies = fit_copies(c);
   double best_rate = -1.0;
   int rows = 0;
@@ -4650,9 +4755,6 @@ int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tunin
   int r;
   const cc_batch_tuning p512{512, 4, 1u << 19, copies, 1, 2}, p256{256, 4, 1u << 19, copies, 1, 2};
   if ((r = trial(p512)) || (r = trial(p256))) return r;
-  cc_batch_tuning t = *best;
-  t.epoch = 8;
-  if ((r = trial(t))) return r;
   const cc_batch_tuning ev{128, 4, 1u << 19, copies, 0, 2};
   if ((r = trial(ev))) return r;
   if (n_rows) *n_rows = rows;

@@ -112,7 +112,11 @@ enum {
   CC_RUN_IMPORTANCE = 64,       /* weight windows from the learned importance map */
   CC_RUN_SOURCE_BIAS = 128,     /* source cells drawn by learned importance (cc_set_importance) */
   CC_RUN_TRACE = 256,           /* cc_transport: log every flight segment (cc_trace_alloc, cc_trace_read) */
-  CC_RUN_PER_BATCH = 512        /* cc_run: one batch after the other instead of streaming them (same results) */
+  CC_RUN_PER_BATCH = 512,       /* cc_run: one batch after the other instead of streaming them (same results) */
+  CC_RUN_PERSISTENT = 1024      /* cc_run, streamed: one persistent history-based launch per group of
+                                   batches instead of event-based rounds (refill, flight, interactions) in a
+                                   particle pool whose last photons one persistent launch carries to their
+                                   ends; same counters, tallies up to FP64 summation order */
 };
 
 typedef struct {
@@ -124,7 +128,8 @@ typedef struct {
   double edep_keV[CC_MAX_MAT];  /* collision estimator per material */
   double edep_sum2[CC_MAX_MAT]; /* sum over batches of S_b^2 / n_b */
   double kernel_ms[8];          /* source, flight, photo, compton, rayleigh, end_batch, other, interact
-                                   (photo + Compton + Rayleigh in one launch: streamed runs) */
+                                   (photo + Compton + Rayleigh in one launch: streamed runs; other:
+                                   the drain of streamed runs, one launch per group of batches) */
   uint64_t kernel_launches[8];  /* launches timed per kernel kind */
   /* region tally (CC_RUN_REGION), pSv per history summed: per-history sum and sum of squares,
    * and the batch form sum over batches of S_b^2 / n_b */
@@ -339,9 +344,9 @@ int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int
 /* Runs ceil(fraction n_s) histories of every scene (fraction in (0, 1]); tallies start from zero. */
 int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32_t flags, cc_batch_out* out);
 int cc_batch_read(void* ctx, int32_t scene, double* kerma_keV, double* h10_pSv_cm3, double* edep_keV);
-/* Times launch shapes on a fraction of the histories and returns the fastest, four timing runs:
- * the persistent form at block 512 and 256 (epoch 2), epoch 8 at the better block, and the
- * event-based form at (block 128, rounds_per_sync 4, pool 2^19); tally copies: the largest power
+/* Times launch shapes on a fraction of the histories and returns the fastest, three timing runs:
+ * the persistent form at block 512 and 256 (epoch 2) and the event-based form at (block 128,
+ * rounds_per_sync 4, pool 2^19); tally copies: the largest power
  * of two up to 16 whose copies fit in half the free device memory.  table (7 doubles per row:
  * persistent, block, epoch, rounds_per_sync, pool, tally_copies, histories/s; one row per run)
  * may be NULL.  The scene tallies are cleared afterwards. */

@@ -53,6 +53,7 @@ def test_ctypes_layouts_match_the_header(tmp_path):
               ("offsetof(cc_trace_record, event)", N.CcTraceRecord.event.offset),
               ("offsetof(cc_trace_record, event)", N.TRACE_DTYPE.fields["event"][1]),
               ("CC_RUN_TRACE", N.RUN_TRACE), ("CC_EV_FLUORESCENCE", N.EV_FLUORESCENCE),
+              ("CC_RUN_PERSISTENT", N.RUN_PERSISTENT),
               ("CC_NOALLOC_MAX_PHOTONS", N.NOALLOC_MAX_PHOTONS),
               ("CC_EV_RAYLEIGH", N.EV_RAYLEIGH),
               ("sizeof(cc_batch_tuning)", C.sizeof(N.CcBatchTuning)),

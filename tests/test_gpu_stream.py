@@ -1,20 +1,23 @@
-"""Streamed runs (the batches of cc_run fed to one pool from a device history cursor) against
-the batch-by-batch loop (CC_RUN_PER_BATCH): the same histories on the same Philox streams, so
-tallies, deposits, batch moments and counters agree up to FP64 summation order (counters and
-hit counts exactly).  Small batches make several groups of batches in flight."""
+"""Streamed runs (the batches of cc_run fed from a device history cursor, in both forms:
+event-based rounds in one pool whose last photons one persistent launch carries to their ends,
+and one persistent history-based launch per group of batches) against the batch-by-batch loop
+(CC_RUN_PER_BATCH): the same histories on the same Philox streams, so tallies, deposits, batch
+moments and counters agree up to FP64 summation order (counters and hit counts exactly).  Small
+batches make several groups of batches in flight."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-def _run(engine, raw, grid, n, batch, seed, per_batch, first=0):
+def _run(engine, raw, grid, n, batch, seed, form, first=0):
     from chatcarlo_amd import _native as N
     from chatcarlo_amd.geometry import Geometry
     from chatcarlo_amd.source import SourceRecord
     engine.set_geometry(Geometry(raw["geometry"]))
     engine.bind_grid(grid[0], grid[1], grid[2], True)
-    flags = N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY | (N.RUN_PER_BATCH if per_batch else 0)
+    flags = N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY | {
+        "rounds": 0, "persistent": N.RUN_PERSISTENT, "per_batch": N.RUN_PER_BATCH}[form]
     out = engine.run(SourceRecord(raw["source"]), n, seed, first, batch, flags)
     g = engine.read_grid(grid[1], moments=True)
     cnt = np.array([out.n_absorbed, out.n_escaped, out.scatter_sum, out.n_fluorescence, out.n_events,
@@ -22,8 +25,9 @@ def _run(engine, raw, grid, n, batch, seed, per_batch, first=0):
     return out, g, cnt
 
 
+@pytest.mark.parametrize("form", ["persistent", "rounds"])
 @pytest.mark.parametrize("case", ["slab_many_groups", "chest_room", "ct_helical_offset"])
-def test_streamed_equals_batch_by_batch(engine, case):
+def test_streamed_equals_batch_by_batch(engine, case, form):
     from chatcarlo_amd import workloads as W
     from chatcarlo_amd.geometry import Geometry
     from chatcarlo_amd.tally import VoxelGrid
@@ -39,8 +43,8 @@ def test_streamed_equals_batch_by_batch(engine, case):
     else:
         raw, grid = W.ct_helical()
         n, batch, first = 300_000, 30_000, 7_777_777
-    a, ga, ca = _run(engine, raw, grid, n, batch, 4242, False, first)
-    b, gb, cb = _run(engine, raw, grid, n, batch, 4242, True, first)
+    a, ga, ca = _run(engine, raw, grid, n, batch, 4242, form, first)
+    b, gb, cb = _run(engine, raw, grid, n, batch, 4242, "per_batch", first)
     np.testing.assert_array_equal(ca, cb)
     np.testing.assert_allclose(np.array(a.edep_keV[:]), np.array(b.edep_keV[:]), rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(np.array(a.edep_sum2[:]), np.array(b.edep_sum2[:]), rtol=1e-9, atol=1e-12)
@@ -59,8 +63,9 @@ def test_streamed_run_without_grid(engine):
     raw, _ = W.chest_room()
     engine.set_geometry(Geometry(raw["geometry"]))
     rec = SourceRecord(raw["source"])
-    a = engine.run(rec, 500_000, 9, 0, 50_000, N.RUN_FLUORESCENCE)
     b = engine.run(rec, 500_000, 9, 0, 50_000, N.RUN_FLUORESCENCE | N.RUN_PER_BATCH)
-    assert (a.n_absorbed, a.n_escaped, a.scatter_sum, a.n_events) == (b.n_absorbed, b.n_escaped, b.scatter_sum,
-                                                                      b.n_events)
-    np.testing.assert_allclose(np.array(a.edep_keV[:]), np.array(b.edep_keV[:]), rtol=1e-9, atol=1e-12)
+    for extra in (0, N.RUN_PERSISTENT):
+        a = engine.run(rec, 500_000, 9, 0, 50_000, N.RUN_FLUORESCENCE | extra)
+        assert (a.n_absorbed, a.n_escaped, a.scatter_sum, a.n_events) == (b.n_absorbed, b.n_escaped, b.scatter_sum,
+                                                                          b.n_events)
+        np.testing.assert_allclose(np.array(a.edep_keV[:]), np.array(b.edep_keV[:]), rtol=1e-9, atol=1e-12)
