@@ -192,10 +192,11 @@ int cc_upload_scene(void* ctx, const cc_prim* prims, int32_t n_prim, const cc_wo
 /* Grid: origin (3), shape (nx, ny, nz), cubic voxel size; flags bit 1 = track_uncertainty.
  * Replaces VoxelGrid.from_bbox / __post_init__ (chatcarlo/tally.py:42-57). */
 int cc_bind_grid(void* ctx, const double* origin, const int32_t* shape, double voxel_cm, int32_t flags);
-/* Batch-tally copies used by the next cc_bind_grid (power of two, 1..64; default 8). Workgroup b
- * adds into copy b mod copies: with 8, each XCD of the MI355X adds into its own copy, so atomics on
- * the hot voxels of a beam are not contended across XCDs. No counterpart in the reference (its NumPy
- * tally has no concurrent writers); results do not depend on it beyond FP64 summation order. */
+/* Batch-tally copies used by the next cc_bind_grid (power of two, 1..64; default 4). Workgroup b
+ * adds into copy b mod copies, and workgroups are dealt to the MI355X's 8 XCDs round-robin: with 4,
+ * two XCDs share a copy, so atomics on the hot voxels of a beam are contended by two XCDs, not 8.
+ * No counterpart in the reference (its NumPy tally has no concurrent writers); results do not
+ * depend on it beyond FP64 summation order. */
 int cc_set_grid_copies(void* ctx, int32_t copies);
 /* Calibration of the tally path's bound, with no counterpart in the reference: the rate of
  * no-return FP64 atomic adds (the tally adds) at uniformly random addresses of an n_slots-double
