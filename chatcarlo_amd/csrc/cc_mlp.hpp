@@ -8,10 +8,10 @@
 // 7 is the first-layer bias) and zeros up to the MFMA depth of 16.  exp(f)
 // estimates the expected future H*(10) contribution of the photon to the region.
 //
-// The GEMMs of forward and backward passes run on v_mfma_f32_16x16x16_bf16
-// with operands staged in LDS.  Fragment layout of the 16x16x16 BF16 MFMA
-// (wave64): lane l holds A[l % 16][4 (l / 16) + e], B[4 (l / 16) + e][l % 16]
-// for e = 0..3 and returns D[4 (l / 16) + e][l % 16].
+// The GEMMs of forward and backward passes run on v_mfma_f32_16x16x16_bf16 with
+// operands in registers, a wave's 64 photons or rows at a time.  Fragment layout of
+// the 16x16x16 BF16 MFMA (wave64): lane l holds A[l % 16][4 (l / 16) + e],
+// B[4 (l / 16) + e][l % 16] for e = 0..3 and returns D[4 (l / 16) + e][l % 16].
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,45 +33,11 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // float -> bf16 bits, round to nearest even (finite inputs)
 __device__ __forceinline__ uint16_t f2bf(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+  // gfx950's v_cvt_pk_bf16_f32: round to nearest even, denormals kept (the f32 mode of HIP code),
+  // the same bits as adding 0x7FFF plus the kept lsb to the float's bits for every finite input
+  return __builtin_bit_cast(uint16_t, (__bf16)x);
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
-
-// C[M][N] (fp32, row stride ldc) = sum_k A(i, k) B(k, j) over K, A and B bf16 in LDS with
-// explicit element strides (so transposes are free).  M, N, K multiples of 16; the tiles
-// are shared round-robin by the waves of the block.  accumulate: C += instead of =.
-__device__ __forceinline__ void lds_gemm_bf16(int M, int N, int K, const uint16_t* A, int a_si, int a_sk,
-                                              const uint16_t* B, int b_sk, int b_sj, float* C, int ldc,
-                                              bool accumulate) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int nwaves = blockDim.x >> 6;
-  const int tiles_n = N / 16;
-  const int tiles = (M / 16) * tiles_n;
-  const int r = lane & 15;
-  const int g = lane >> 4;
-  for (int t = wave; t < tiles; t += nwaves) {
-    const int tm = t / tiles_n, tn = t % tiles_n;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int kb = 0; kb < K; kb += 16) {
-      bf16x4 a, b;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int k = kb + 4 * g + e;
-        a[e] = (short)A[(tm * 16 + r) * a_si + k * a_sk];
-        b[e] = (short)B[k * b_sk + (tn * 16 + r) * b_sj];
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      float* c = C + (tm * 16 + 4 * g + e) * ldc + tn * 16 + r;
-      *c = accumulate ? *c + acc[e] : acc[e];
-    }
-  }
-}
 
 // Feature row of one photon state (7 features, bias 1, zeros).
 struct MlpFrame {
@@ -106,42 +72,6 @@ __device__ __forceinline__ void mlp_features(const MlpFrame& F, const float p[3]
   x[7] = 1.0f;
 #pragma unroll
   for (int k = 8; k < kMlpIn; ++k) x[k] = 0.0f;
-}
-
-// Shared-memory workspace for a tile of R samples (R multiple of 16, block-wide).
-struct MlpTile {
-  uint16_t* X;    // [R][16]  bf16
-  uint16_t* W1;   // [16][64] bf16
-  uint16_t* W2;   // [64][64] bf16
-  float* Z1;      // [R][64]  fp32 pre-activation of layer 1
-  uint16_t* H1;   // [R][64]  bf16 relu(Z1)
-  float* Z2;      // [R][64]  fp32 pre-activation of layer 2 (+ b2)
-};
-
-// Loads the bf16 weights of W1 and W2 into the tile (block-wide).
-__device__ __forceinline__ void mlp_stage_weights(const uint16_t* w1, const uint16_t* w2, MlpTile& T) {
-  for (int k = threadIdx.x; k < kMlpIn * kMlpH; k += blockDim.x) T.W1[k] = w1[k];
-  for (int k = threadIdx.x; k < kMlpH * kMlpH; k += blockDim.x) T.W2[k] = w2[k];
-}
-
-// Forward pass of R staged rows (T.X filled).  out[r] = f(x_r).  Block-wide.
-__device__ __forceinline__ void mlp_forward_tile(int R, MlpTile& T, const float* b2, const float* w3, float b3,
-                                                 float* out) {
-  __syncthreads();
-  lds_gemm_bf16(R, kMlpH, kMlpIn, T.X, kMlpIn, 1, T.W1, kMlpH, 1, T.Z1, kMlpH, false);
-  __syncthreads();
-  for (int k = threadIdx.x; k < R * kMlpH; k += blockDim.x) T.H1[k] = f2bf(fmaxf(T.Z1[k], 0.0f));
-  __syncthreads();
-  lds_gemm_bf16(R, kMlpH, kMlpH, T.H1, kMlpH, 1, T.W2, kMlpH, 1, T.Z2, kMlpH, false);
-  __syncthreads();
-  for (int k = threadIdx.x; k < R * kMlpH; k += blockDim.x) T.Z2[k] += b2[k % kMlpH];
-  __syncthreads();
-  for (int r = threadIdx.x; r < R; r += blockDim.x) {
-    float s = b3;
-    for (int j = 0; j < kMlpH; ++j) s += fmaxf(T.Z2[r * kMlpH + j], 0.0f) * w3[j];
-    out[r] = s;
-  }
-  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------

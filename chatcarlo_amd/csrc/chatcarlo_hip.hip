@@ -2048,140 +2048,271 @@ __global__ __launch_bounds__(kBlock) void k_mlp_forward(const float* w, const ui
   }
 }
 
-// C (global fp32) += A B with A, B bf16 in LDS (atomic, for gradient accumulation).
-__device__ __forceinline__ void lds_gemm_bf16_atomic(int M, int N, int K, const uint16_t* A, int a_si, int a_sk,
-                                                     const uint16_t* B, int b_sk, int b_sj, float* C, int ldc) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int nwaves = blockDim.x >> 6;
-  const int tiles_n = N / 16;
-  const int tiles = (M / 16) * tiles_n;
-  const int r = lane & 15;
-  const int g = lane >> 4;
-  for (int t = wave; t < tiles; t += nwaves) {
-    const int tm = t / tiles_n, tn = t % tiles_n;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int kb = 0; kb < K; kb += 16) {
-      bf16x4 a, b;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int k = kb + 4 * g + e;
-        a[e] = (short)A[(tm * 16 + r) * a_si + k * a_sk];
-        b[e] = (short)B[k * b_sk + (tn * 16 + r) * b_sj];
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int e = 0; e < 4; ++e) atomicAdd(C + (tm * 16 + 4 * g + e) * ldc + tn * 16 + r, acc[e]);
-  }
-}
-
 // Gradient of the Poisson deviance  sum_r [exp(f_r) - t_r f_r] * gscale  over rows, added
 // into grad (fp32, kMlpParams), with target t = yscale y (power 1) or (yscale y)^2
 // (power 2: exp(f) then estimates the second moment).  Rows come either from `feat`
 // (16 floats each, with y) or from the record buffer through the affine permutation
 // idx = (a k + b) mod n_rec.
-constexpr int kGR = 32;
+//
+// Each wave carries 64 rows (one per lane) through forward and backward passes in registers,
+// 16 rows at a time on 16x16x16 MFMAs, as mlp_wave_eval does the forward pass: with weights as
+// the A operand and rows as the B operand, Z1^T = W1^T X^T and Z2^T = W2^T relu(Z1)^T chain in
+// the MFMA layouts, and so does the back-propagated dH1^T = W2 dZ2^T.  The weight gradients
+// dW2 = H1^T dZ2 and dW1 = X^T dZ1 contract over rows, which sit in the column (lane) index of
+// those layouts, so each 16x16 block of H1^T, dZ2^T and dZ1^T is transposed once through the
+// wave's own LDS area; the products then accumulate in registers over all the wave's rows.  At
+// the end the waves of a block add their gradients into LDS and the block adds that into grad
+// with one atomic per parameter.  The rounding points are those of the FP64 oracle: X, W1, W2,
+// relu(Z1), dZ2 and dZ1 in BF16, everything else in FP32.
+constexpr int kGradRows = kBlock;   // rows per block and pass: one per lane
 constexpr int kGradBlocks = 256;
+struct GradWaveLds {
+  uint16_t xs[kWave][kMlpIn];          // the wave's rows, [row][feature]
+  uint16_t tr[3][4][16][16];           // H1^T, dZ2^T, dZ1^T blocks: [which][block][row][col]
+};
+__device__ __forceinline__ void wave_lds_order() {
+  // LDS operations of one wave are performed in order; this keeps the compiler from moving them
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ bf16x4 ld_bf16x4(const uint16_t* p) {   // 4 consecutive bf16, 8-byte aligned
+  const uint2 v = *reinterpret_cast<const uint2*>(p);
+  bf16x4 r;
+  r[0] = (short)(v.x & 0xFFFFu); r[1] = (short)(v.x >> 16);
+  r[2] = (short)(v.y & 0xFFFFu); r[3] = (short)(v.y >> 16);
+  return r;
+}
 __global__ __launch_bounds__(kBlock) void k_mlp_grad(const float* w, const uint16_t* wb, int n, const float* feat,
                                                      const float* yv, const float* rec, uint32_t n_rec,
                                                      uint64_t pa, uint64_t pb, uint32_t k0, MlpFrame frame,
                                                      float yscale, int power, float gscale, float* grad,
                                                      double* loss) {
-  __shared__ uint16_t sX[kGR * kMlpIn], sW1[kMlpIn * kMlpH], sW2[kMlpH * kMlpH], sH1[kGR * kMlpH];
-  __shared__ uint16_t sD2[kGR * kMlpH], sD1[kGR * kMlpH];
-  __shared__ float sZ1[kGR * kMlpH], sZ2[kGR * kMlpH], sOut[kGR], sG[kGR], sY[kGR];
-  // the block's gradient over all of its tiles, added into grad once at the end: one global atomic
-  // per parameter and block instead of one per parameter and tile
+  __shared__ __align__(16) uint16_t sW2[kMlpH * kMlpH];   // W2 [i][j]
+  __shared__ float sbw[2 * kMlpH];                        // b2, w3
+  __shared__ __align__(16) GradWaveLds swl[kBlock / kWave];
   __shared__ float sGrad[kMlpParams];
+  __shared__ double sL[kBlock / kWave];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, g = lane >> 4;
   for (int k = threadIdx.x; k < kMlpParams; k += blockDim.x) sGrad[k] = 0.f;
-  MlpTile tile{sX, sW1, sW2, sZ1, sH1, sZ2};
-  mlp_stage_weights(wb, wb + kMlpIn * kMlpH, tile);
-  const float* b2 = w + kOffB2;
-  const float* w3 = w + kOffW3;
+  for (int k = threadIdx.x; k < kMlpH * kMlpH; k += blockDim.x) sW2[k] = wb[kOffW2 + k];
+  for (int k = threadIdx.x; k < 2 * kMlpH; k += blockDim.x) sbw[k] = w[kOffB2 + k];
+  // forward weights in registers (mlp_wave_load's layout): a1[tm][e] = W1[4g + e][16 tm + c],
+  // a2[tm][kb][e] = W2[16 kb + 4g + e][16 tm + c]
+  bf16x4 a1[4], a2[4][4];
+#pragma unroll
+  for (int tm = 0; tm < 4; ++tm) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      a1[tm][e] = (short)wb[(4 * g + e) * kMlpH + 16 * tm + c];
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) a2[tm][kb][e] = (short)wb[kOffW2 + (16 * kb + 4 * g + e) * kMlpH + 16 * tm + c];
+    }
+  }
+  const float b3 = w[kOffB3];
+  __syncthreads();
+  GradWaveLds& L = swl[wave];
+  // gradient accumulators: acc2[ti][tj][e] = dW2[16 ti + 4g + e][16 tj + c], acc1[ti][e] =
+  // dW1[4g + e][16 ti + c]; dw3, db2 of hidden unit 16 tm + 4g + e over this lane's column rows
+  f32x4 acc2[4][4], acc1[4];
+  float dw3[4][4], db2[4][4], db3 = 0.f;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    acc1[a] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      acc2[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+      dw3[a][b] = 0.f;
+      db2[a][b] = 0.f;
+    }
+  }
   double lsum = 0.0;
-  for (int base = blockIdx.x * kGR; base < n; base += gridDim.x * kGR) {
-    const int r = threadIdx.x;
-    if (r < kGR) {
-      const int row = base + r;
-      float x[kMlpIn];
-      float y = 0.f;
-      if (row < n) {
-        if (feat) {
-          for (int k = 0; k < kMlpIn; ++k) x[k] = feat[(size_t)row * kMlpIn + k];
-          y = yv[row];
-        } else {
-          const uint64_t q = (pa * (uint64_t)(k0 + row) + pb) % (uint64_t)n_rec;
-          const float* rr = rec + q * 12;
-          const float p[3] = {rr[0], rr[1], rr[2]}, d[3] = {rr[3], rr[4], rr[5]};
-          mlp_features(frame, p, d, rr[6], x);
-          y = rr[8];
-        }
+  for (int base = blockIdx.x * kGradRows; base < n; base += gridDim.x * kGradRows) {
+    // this lane's row: features (bf16, into the wave's LDS rows), target and validity
+    const int row = base + wave * kWave + lane;
+    const bool ok = row < n;
+    float x[kMlpIn];
+    float t = 0.f;
+    if (ok) {
+      float y;
+      if (feat) {
+#pragma unroll
+        for (int k = 0; k < kMlpIn; ++k) x[k] = feat[(size_t)row * kMlpIn + k];
+        y = yv[row];
       } else {
-        for (int k = 0; k < kMlpIn; ++k) x[k] = 0.f;
+        const uint64_t q = (pa * (uint64_t)(k0 + row) + pb) % (uint64_t)n_rec;
+        const float* rr = rec + q * 12;
+        const float p[3] = {rr[0], rr[1], rr[2]}, d[3] = {rr[3], rr[4], rr[5]};
+        mlp_features(frame, p, d, rr[6], x);
+        y = rr[8];
       }
-      for (int k = 0; k < kMlpIn; ++k) sX[r * kMlpIn + k] = f2bf(x[k]);
-      const float t = y * yscale;
-      sY[r] = power == 2 ? t * t : t;
+      const float ty = y * yscale;
+      t = power == 2 ? ty * ty : ty;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kMlpIn; ++k) x[k] = 0.f;
     }
-    mlp_forward_tile(kGR, tile, b2, w3, w[kOffB3], sOut);
-    if (r < kGR) {
-      const bool ok = base + r < n;
-      const float f = sOut[r];
+    {
+      uint32_t pk[kMlpIn / 2];
+#pragma unroll
+      for (int k = 0; k < kMlpIn / 2; ++k) pk[k] = (uint32_t)f2bf(x[2 * k]) | ((uint32_t)f2bf(x[2 * k + 1]) << 16);
+      uint4* const dst = reinterpret_cast<uint4*>(&L.xs[lane][0]);
+      dst[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+      dst[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
+    }
+    wave_lds_order();
+#pragma unroll 1
+    for (int tn = 0; tn < 4; ++tn) {
+      // forward for rows 16 tn .. 16 tn + 15; this lane's column is row 16 tn + c
+      const bf16x4 bx = ld_bf16x4(&L.xs[16 * tn + c][4 * g]);   // X^T[4g + e][row]
+      bf16x4 h[4];
+      uint32_t mask1 = 0u;   // bit 4 tm + e: Z1^T[16 tm + 4g + e][row] > 0
+#pragma unroll
+      for (int tm = 0; tm < 4; ++tm) {
+        const f32x4 z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[tm], bx, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          h[tm][e] = (short)f2bf(fmaxf(z[e], 0.0f));
+          if (z[e] > 0.f) mask1 |= 1u << (4 * tm + e);
+        }
+      }
+      float a2v[4][4];
+      float fs = 0.f;
+#pragma unroll
+      for (int tm = 0; tm < 4; ++tm) {
+        f32x4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[tm][kb], h[kb], z, 0, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int h2 = 16 * tm + 4 * g + e;
+          a2v[tm][e] = z[e] + sbw[h2];
+          fs += fmaxf(a2v[tm][e], 0.0f) * sbw[kMlpH + h2];
+        }
+      }
+      fs += __shfl_xor(fs, 16);
+      fs += __shfl_xor(fs, 32);
+      const float f = fs + b3;
+      // the row's target and validity from the lane that holds it
+      const int src = 16 * tn + c;
+      const float tr_ = __shfl(t, src);
+      const bool okr = __shfl((int)ok, src) != 0;
       const float ef = expf(fminf(f, 80.f));
-      sG[r] = ok ? (ef - sY[r]) * gscale : 0.f;
-      if (ok) lsum += (double)(ef - sY[r] * f) * gscale;
-    }
-    __syncthreads();
-    // output layer: dw3 = sum_r g relu(Z2), db3 = sum_r g; D2 = g w3 1[Z2 > 0]; db2 = sum_r D2
-    if (threadIdx.x < kMlpH) {
-      const int jj = threadIdx.x;
-      float s3 = 0.f, s2 = 0.f;
-      for (int rr = 0; rr < kGR; ++rr) {
-        const float z = sZ2[rr * kMlpH + jj];
-        s3 += sG[rr] * fmaxf(z, 0.f);
-        const float d2 = z > 0.f ? sG[rr] * w3[jj] : 0.f;
-        s2 += d2;
+      const float gr = okr ? (ef - tr_) * gscale : 0.f;
+      if (okr && g == 0) lsum += (double)(ef - tr_ * f) * gscale;
+      if (g == 0) db3 += gr;
+      // output layer and dZ2^T (this lane: hidden 16 tm + 4g + e of its row)
+      bf16x4 d2[4];
+#pragma unroll
+      for (int tm = 0; tm < 4; ++tm) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int h2 = 16 * tm + 4 * g + e;
+          const float v = a2v[tm][e];
+          dw3[tm][e] += gr * fmaxf(v, 0.0f);
+          const float dz = v > 0.f ? gr * sbw[kMlpH + h2] : 0.f;
+          db2[tm][e] += dz;
+          d2[tm][e] = (short)f2bf(dz);
+        }
       }
-      sGrad[kOffW3 + jj] += s3;   // one thread per hidden unit: no other writer
-      sGrad[kOffB2 + jj] += s2;
+      // dH1^T = W2 dZ2^T: A[i][k] = W2[16 ti + c][16 kb + 4g + e]; then dZ1^T = dH1^T 1[Z1 > 0]
+      bf16x4 d1[4];
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) {
+        f32x4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+          z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_bf16x4(&sW2[(16 * ti + c) * kMlpH + 16 * kb + 4 * g]), d2[kb],
+                                                        z, 0, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) d1[ti][e] = (short)f2bf((mask1 >> (4 * ti + e)) & 1u ? z[e] : 0.f);
+      }
+      // transposes: block b of H1^T, dZ2^T, dZ1^T with rows 4g + e at column c
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          L.tr[0][b][4 * g + e][c] = (uint16_t)h[b][e];
+          L.tr[1][b][4 * g + e][c] = (uint16_t)d2[b][e];
+          L.tr[2][b][4 * g + e][c] = (uint16_t)d1[b][e];
+        }
+      }
+      wave_lds_order();
+      // dW2 += H1^T dZ2: A[i][k] = H1^T[16 ti + c][row 4g + e], B[k][j] = dZ2^T[16 tj + c][row 4g + e]
+      bf16x4 bd2[4];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) bd2[tj] = ld_bf16x4(&L.tr[1][tj][c][4 * g]);
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) {
+        const bf16x4 ah = ld_bf16x4(&L.tr[0][ti][c][4 * g]);
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) acc2[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bd2[tj], acc2[ti][tj], 0, 0, 0);
+      }
+      // dW1 += X^T dZ1: A[f][k] = X[row 16 tn + 4g + e][feature c], B[k][i] = dZ1^T[16 ti + c][row 4g + e]
+      bf16x4 ax;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) ax[e] = (short)L.xs[16 * tn + 4 * g + e][c];
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+        acc1[ti] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ax, ld_bf16x4(&L.tr[2][ti][c][4 * g]), acc1[ti], 0, 0, 0);
+      wave_lds_order();
     }
-    if (threadIdx.x == 64) {
-      float s = 0.f;
-      for (int rr = 0; rr < kGR; ++rr) s += sG[rr];
-      sGrad[kOffB3] += s;
+  }
+  // the waves' gradients into the block's, one wave after the other with plain LDS adds (LDS
+  // atomics from all four waves at once were measured 24 us of a 62 us step), then into grad
+  for (int wv = 0; wv < (int)(blockDim.x / kWave); ++wv) {
+    if (wave == wv) {
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          sGrad[kOffW1 + (4 * g + e) * kMlpH + 16 * ti + c] += acc1[ti][e];
+#pragma unroll
+          for (int tj = 0; tj < 4; ++tj) sGrad[kOffW2 + (16 * ti + 4 * g + e) * kMlpH + 16 * tj + c] += acc2[ti][tj][e];
+        }
+      }
     }
-    for (int k = threadIdx.x; k < kGR * kMlpH; k += blockDim.x) {
-      const int rr = k / kMlpH, jj = k % kMlpH;
-      sD2[k] = f2bf(sZ2[k] > 0.f ? sG[rr] * w3[jj] : 0.f);
-    }
-    __syncthreads();
-    // dW2 += H1^T D2  (M = 64 hidden1, N = 64 hidden2, K = rows)
-    lds_gemm_bf16_atomic(kMlpH, kMlpH, kGR, sH1, 1, kMlpH, sD2, kMlpH, 1, sGrad + kOffW2, kMlpH);
-    // D1 = (D2 W2^T) 1[Z1 > 0]  (M = rows, N = 64 hidden1, K = 64 hidden2), into sZ1
-    lds_gemm_bf16(kGR, kMlpH, kMlpH, sD2, kMlpH, 1, sW2, 1, kMlpH, sZ1, kMlpH, false);
-    __syncthreads();
-    for (int k = threadIdx.x; k < kGR * kMlpH; k += blockDim.x) sD1[k] = f2bf(bf2f(sH1[k]) > 0.f ? sZ1[k] : 0.f);
-    __syncthreads();
-    // dW1 += X^T D1  (M = 16 inputs, N = 64, K = rows)
-    lds_gemm_bf16_atomic(kMlpIn, kMlpH, kGR, sX, 1, kMlpIn, sD1, kMlpH, 1, sGrad + kOffW1, kMlpH);
     __syncthreads();
   }
+#pragma unroll
+  for (int tm = 0; tm < 4; ++tm) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      float a = dw3[tm][e], b = db2[tm][e];
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+      }
+      if (c == 0) {
+        atomicAdd(&sGrad[kOffW3 + 16 * tm + 4 * g + e], a);
+        atomicAdd(&sGrad[kOffB2 + 16 * tm + 4 * g + e], b);
+      }
+    }
+  }
+  {
+    float s = db3;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) atomicAdd(&sGrad[kOffB3], s);
+    double l = lsum;
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+    if (lane == 0) sL[wave] = l;
+  }
+  __syncthreads();
   for (int k = threadIdx.x; k < kMlpParams; k += blockDim.x)
     if (sGrad[k] != 0.f) atomicAdd(grad + k, sGrad[k]);
-  // block loss
-  __shared__ double sL[kBlock];
-  sL[threadIdx.x] = lsum;
-  __syncthreads();
   if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int k = 0; k < (int)blockDim.x; ++k) t += sL[k];
-    atomic_add_f64(loss, t);
+    double l = 0.0;
+    for (int k = 0; k < (int)(blockDim.x / kWave); ++k) l += sL[k];
+    atomic_add_f64(loss, l);
   }
 }
 
+
+
 // Adam on the fp32 master weights; refreshes the bf16 copies and clears grad.
-// Rows 8..15 of W1 (padding columns of the input) stay at zero.
+// Rows 8..15 of W1 (padding columns of the input) stay at zero.  (Taken by the last block of
+// k_mlp_grad instead, the step ran 61 us against 39 + 5 us: one block's dependent loads.)
 __global__ void k_adam(float* w, float* m, float* v, float* grad, uint16_t* wb, float lr, float b1, float b2,
                        float c1, float c2, float eps) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < kMlpParams; k += gridDim.x * blockDim.x) {
@@ -4224,7 +4355,7 @@ int cc_mlp_grad(void* ctx, int32_t net, int64_t n, const float* feat, const floa
   hipLaunchKernelGGL(k_fill_f32, dim3(32), dim3(kBlock), 0, c->stream, dg, (size_t)kMlpParams, 0.f);
   hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, c->stream, dl, (size_t)1, 0.0);
   if (n > 0) {
-    const int nb = (int)std::min<int64_t>((n + kGR - 1) / kGR, 2048);
+    const int nb = (int)std::min<int64_t>((n + kGradRows - 1) / kGradRows, 2048);
     hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(kBlock), 0, c->stream, c->mw[net], c->mwb[net], (int)n, df, dy,
                        (const float*)nullptr, 0u, (uint64_t)0, (uint64_t)0, 0u, c->frame, yscale, 1, 1.0f, dg, dl);
   }
@@ -4290,7 +4421,7 @@ int cc_mlp_train(void* ctx, int32_t net, const cc_train_cfg* cfg, cc_train_out* 
   HIP_TRY(hipEventRecord(e0, c->stream));
   // at most kGradBlocks blocks: a larger minibatch gives each block more tiles, not more blocks,
   // so the global gradient atomics per step stay at kGradBlocks x kMlpParams
-  const int gb = (int)std::min<uint32_t>((cfg->batch + kGR - 1) / kGR, kGradBlocks);
+  const int gb = (int)std::min<uint32_t>((cfg->batch + kGradRows - 1) / kGradRows, kGradBlocks);
   const float gscale = 1.0f / (float)cfg->batch;
   double b1t = std::pow((double)cfg->beta1, (double)c->adam_t[net]);
   double b2t = std::pow((double)cfg->beta2, (double)c->adam_t[net]);
