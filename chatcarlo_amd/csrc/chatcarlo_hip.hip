@@ -243,9 +243,11 @@ __device__ __forceinline__ PhotonWords load_photon(const PState& S, int i) {
   return PhotonWords{{r[0], r[1], r[2], r[3]}};
 }
 // The Philox stream of a loaded photon: keyed by the run's seed, or in a batched run by its
-// scene's; counter (history, tag, event counter).
+// scene's (a streamed run, whose W_SC is a batch slot, has no seed table); counter (history, tag,
+// event counter).
 __device__ __forceinline__ PhiloxRegRng photon_rng(const Launch& L, const PhotonWords& P) {
-  return PhiloxRegRng(L.multi ? L.MS.seeds[P.u(W_SC)] : L.seed, L.hist0 + (uint64_t)P.u(W_HID), P.u(W_TAG), P.u(W_CTR));
+  return PhiloxRegRng(L.multi && L.MS.seeds ? L.MS.seeds[P.u(W_SC)] : L.seed, L.hist0 + (uint64_t)P.u(W_HID),
+                      P.u(W_TAG), P.u(W_CTR));
 }
 
 __device__ __forceinline__ uint4* photon_record_w(const PState& S, int i) {
@@ -2675,8 +2677,8 @@ struct Ctx {
   // streamed runs: tally slots of a group of batches, their deposits, per-slot seeds
   double* st_tally = nullptr;
   size_t st_tally_cap = 0;
+  bool st_clean = false;   // st_tally all zero (a streamed run's group ends clear every slot it used)
   double* st_edep = nullptr;
-  uint64_t* st_seeds = nullptr;
   // segment trace of cc_transport
   float* tr_rec = nullptr;
   uint32_t* tr_cnt = nullptr;
@@ -2929,43 +2931,43 @@ Launch make_launch(Ctx* c, uint64_t seed, uint64_t hist0, int grid_on, int fluor
   return L;
 }
 
+// Per-kernel-kind HIP-event timing on the context's stream.  Events mark the boundaries of
+// segments: mark(kind) ends the open segment (if any) and opens one of `kind`, mark(-1) just ends
+// it, so back-to-back kernels share one event per boundary (every event adds a few microseconds
+// between kernels).  begin(kind) / end() bracket a single segment.
 struct Timer {
   Ctx* c;
   bool on;
-  std::vector<std::pair<int, int>> marks;   // (kind, event index of start)
-  int used = 0;
+  std::vector<int> kinds;   // kinds[i]: kind of the segment that event i opens (-1: none)
   hipEvent_t next() {
-    if (used >= (int)c->ev.size()) {
+    if (kinds.size() >= c->ev.size()) {
       hipEvent_t e;
       (void)hipEventCreate(&e);
       c->ev.push_back(e);
     }
-    return c->ev[used++];
+    return c->ev[kinds.size()];
+  }
+  void mark(int kind) {
+    if (!on) return;
+    (void)hipEventRecord(next(), c->stream);
+    kinds.push_back(kind);
   }
   int begin(int kind) {
-    if (!on) return -1;
-    int idx = used;
-    hipEvent_t e = next();
-    (void)hipEventRecord(e, c->stream);
-    marks.push_back({kind, idx});
-    return idx;
+    mark(kind);
+    return 0;
   }
-  void end() {
-    if (!on) return;
-    hipEvent_t e = next();
-    (void)hipEventRecord(e, c->stream);
-  }
-  // after a stream sync
+  void end() { mark(-1); }
+  // after a stream sync: every segment closed by a later event is added to its kind
   void collect(double* ms, uint64_t* launches) {
     if (!on) return;
-    for (auto& m : marks) {
+    for (size_t i = 0; i + 1 < kinds.size(); ++i) {
+      if (kinds[i] < 0) continue;
       float t = 0.f;
-      (void)hipEventElapsedTime(&t, c->ev[m.second], c->ev[m.second + 1]);
-      ms[m.first] += t;
-      if (launches) launches[m.first] += 1;
+      (void)hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]);
+      ms[kinds[i]] += t;
+      if (launches) launches[kinds[i]] += 1;
     }
-    marks.clear();
-    used = 0;
+    kinds.clear();
   }
 };
 
@@ -3250,7 +3252,7 @@ int cc_ctx_destroy(void* ctx) {
   (void)hipStreamSynchronize(c->stream);
   free_pool(c);
   for (void* p : c->tbufs) (void)hipFree(p);
-  void* ps[] = {c->st_tally, c->st_edep, c->st_seeds, c->tr_rec, c->tr_cnt, c->tr_last, c->prims, c->tk, c->th, c->qk, c->qh, c->hits, c->G.t, c->cnt, c->stats, c->edep, c->err,
+  void* ps[] = {c->st_tally, c->st_edep, c->tr_rec, c->tr_cnt, c->tr_last, c->prims, c->tk, c->th, c->qk, c->qh, c->hits, c->G.t, c->cnt, c->stats, c->edep, c->err,
                 c->spec, c->rec, c->rcnt, c->mw[0], c->mm[0], c->mv[0], c->mgrad[0], c->mwb[0],
                 c->mw[1], c->mm[1], c->mv[1], c->mgrad[1], c->mwb[1], c->bcdf, c->bwt, c->stage};
   for (void* p : ps)
@@ -3348,15 +3350,19 @@ int cc_bind_grid(void* ctx, const double* origin, const int32_t* shape, double v
   if (shape[0] <= 0 || shape[1] <= 0 || shape[2] <= 0 || !(voxel_cm > 0))
     return fail(CC_ERR_ARG, "invalid grid shape or voxel size");
   HIP_TRY(hipSetDevice(c->device));
-  cc_unbind_grid(ctx);
   const size_t nv = (size_t)shape[0] * shape[1] * shape[2];
   if (nv >= 0xffffffffull) return fail(CC_ERR_ARG, "grid of %zu voxels: at most 2^32 - 2", nv);
   const size_t cs = 2 * ((nv + 31) & ~(size_t)31);   // (kerma, H*(10)) pairs; copies start on 512 B boundaries
   const size_t nc = (size_t)c->grid_copies;
-  int r;
-  if ((r = dalloc(&c->G.t, nc * cs)) || (r = dalloc(&c->tk, nv)) ||
-      (r = dalloc(&c->th, nv)) || (r = dalloc(&c->qk, nv)) || (r = dalloc(&c->qh, nv)) || (r = dalloc(&c->hits, nv)))
-    return r;
+  // a grid of the same voxel count and copies keeps its buffers (rebinding every run would free and
+  // allocate ~0.1 GB each time); they are cleared below either way
+  if (!(c->have_grid && c->nvox == nv && (size_t)c->G.cmask + 1 == nc && c->G.cstride == cs)) {
+    cc_unbind_grid(ctx);
+    int r;
+    if ((r = dalloc(&c->G.t, nc * cs)) || (r = dalloc(&c->tk, nv)) ||
+        (r = dalloc(&c->th, nv)) || (r = dalloc(&c->qk, nv)) || (r = dalloc(&c->qh, nv)) || (r = dalloc(&c->hits, nv)))
+      return r;
+  }
   c->nvox = nv;
   c->G.cmask = (uint32_t)(nc - 1);
   c->G.cstride = cs;
@@ -3566,17 +3572,16 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
       c->st_tally_cap = 0;
       if ((r = dalloc(&c->st_tally, need))) return r;
       c->st_tally_cap = need;
+      c->st_clean = false;
     }
-    hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(need)), dim3(kBlock), 0, c->stream, c->st_tally, need, 0.0);
+    // the folds at the end of every group clear the slots; a run that failed may have left some
+    if (!c->st_clean)
+      hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(c->st_tally_cap)), dim3(kBlock), 0, c->stream, c->st_tally,
+                         c->st_tally_cap, 0.0);
+    c->st_clean = false;
   }
   constexpr size_t kStreamDep = (size_t)kStreamSlots * CC_MAX_MAT;   // deposit sums per copy: [slot][material]
-  if (!c->st_edep && ((r = dalloc(&c->st_edep, kAccCopies * kStreamDep)) ||
-                      (r = dalloc(&c->st_seeds, (size_t)kStreamSlots))))
-    return r;
-  {
-    std::vector<uint64_t> sd(kStreamSlots, seed);
-    HIP_TRY(hipMemcpy(c->st_seeds, sd.data(), sizeof(uint64_t) * kStreamSlots, hipMemcpyHostToDevice));
-  }
+  if (!c->st_edep && (r = dalloc(&c->st_edep, kAccCopies * kStreamDep))) return r;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->err, (size_t)1, 0);
   Timer tm{c, (flags & CC_RUN_TIMING) != 0};
   const int nbk = grid_blocks(pool);
@@ -3604,7 +3609,7 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
     L.multi = 1;   // deposits and RNG keys per batch slot, freed slots to the stack
     L.edep = c->st_edep;
     L.dep_stride = kStreamDep;
-    L.MS.seeds = c->st_seeds;
+    L.MS.seeds = nullptr;   // every batch slot of a streamed run draws from the run's seed
     L.MS.free_stack = c->b_free;
     L.MS.free_cnt = c->b_free_cnt;
     L.MS.cursor = c->b_cursor;
@@ -3632,21 +3637,19 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
     }
     for (; !persistent;) {
       for (int k = 0; k < chunk; ++k) {
-        tm.begin(K_SOURCE);
+        tm.mark(K_SOURCE);
         hipLaunchKernelGGL(k_refill_stream, dim3(nbk), dim3(kBlock), 0, c->stream, L, s, act_cur, c->cnt + cur, pool);
         hipLaunchKernelGGL(k_commit_and_reset, dim3(1), dim3(64), 0, c->stream, L, pool, nxt);
-        tm.end();
-        tm.begin(K_FLIGHT);
+        tm.mark(K_FLIGHT);
         hipLaunchKernelGGL((k_flight<false, false, false, true>), dim3(nbk), dim3(kBlock), 0, c->stream, L,
                            (const int*)act_cur, (const uint32_t*)(c->cnt + cur), c->qp, c->qc, c->qr, (int*)nullptr);
-        tm.end();
-        tm.begin(K_INTERACT);
+        tm.mark(K_INTERACT);
         hipLaunchKernelGGL(k_interact, dim3(nbk), dim3(kBlock), 0, c->stream, L, (const int*)c->qp,
                            (const int*)c->qc, (const int*)c->qr, act_nxt, c->cnt + nxt);
-        tm.end();
         std::swap(cur, nxt);
         std::swap(act_cur, act_nxt);
       }
+      tm.mark(-1);
       rounds += chunk;
       hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(64), 0, c->stream, L, (const uint32_t*)(c->cnt + cur),
                          c->b_status);
@@ -3715,6 +3718,7 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
   out->n_histories = n_hist;
   out->error_bits = (uint32_t)err_bits;
   if (err_bits) return fail(CC_ERR_DEVICE, "device error bits 0x%x", err_bits);
+  if (grid_on) c->st_clean = true;   // every group's fold cleared its slots (nothing else writes them)
   return CC_OK;
 }
 
