@@ -4884,7 +4884,10 @@ int cc_batch_autotune(void* ctx, double fraction, uint32_t flags, cc_batch_tunin
       row[5] = t.tally_copies; row[6] = rate;
     }
     ++rows;
-    if (rate > best_rate) {
+    // a later candidate must beat the first (the persistent form at block 512) by 5 %: on short
+    // timing runs the tail of the last histories hides most of the difference (the 64-scene sweep
+    // at 1 %: 144 against 144 M/s for blocks 512 and 256; in full: 228 against 197)
+    if (best_rate < 0.0 || rate > 1.05 * best_rate) {
       best_rate = rate;
       *best = t;
     }

@@ -347,7 +347,8 @@ int cc_batch_run(void* ctx, const cc_batch_tuning* tune, double fraction, uint32
 int cc_batch_read(void* ctx, int32_t scene, double* kerma_keV, double* h10_pSv_cm3, double* edep_keV);
 /* Times launch shapes on a fraction of the histories and returns the fastest, three timing runs:
  * the persistent form at block 512 and 256 (epoch 2) and the event-based form at (block 128,
- * rounds_per_sync 4, pool 2^19); tally copies: the largest power
+ * rounds_per_sync 4, pool 2^19), a later one taken only when it beats the best so far by 5 %;
+ * tally copies: the largest power
  * of two up to 16 whose copies fit in half the free device memory.  table (7 doubles per row:
  * persistent, block, epoch, rounds_per_sync, pool, tally_copies, histories/s; one row per run)
  * may be NULL.  The scene tallies are cleared afterwards. */

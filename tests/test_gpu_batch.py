@@ -152,7 +152,11 @@ def test_autotune_returns_a_candidate(engine, single):
     assert [int(x) for x in table[:, 0]] == [1, 1, 0]
     assert [int(x) for x in table[:2, 1]] == [512, 256] and [int(x) for x in table[:, 2]] == [2, 2, 2]
     assert (int(table[2, 1]), int(table[2, 3]), int(table[2, 4])) == (128, 4, 1 << 19)
-    best = table[np.argmax(table[:, 6])]
+    # the first run's shape unless a later one beats the best so far by 5 %
+    best = table[0]
+    for row in table[1:]:
+        if row[6] > 1.05 * best[6]:
+            best = row
     assert (int(t.persistent), t.block, t.epoch, t.rounds_per_sync, t.pool, t.tally_copies) == \
         tuple(int(x) for x in best[:6])
     # the same workload again: the cached shape, no timing
