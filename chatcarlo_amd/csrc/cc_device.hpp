@@ -29,6 +29,7 @@ constexpr float kMEC2 = 511.0f;         // chatcarlo/physics.py:34
 constexpr float kFluorCut = 5.0f;       // chatcarlo/physics.py:38
 constexpr float kTwoPi = 6.28318530717958647692f;
 constexpr int kMaxEdges = 16;
+constexpr int kRayGuide = 1024;   // guide-table buckets of each element's Rayleigh cumulative A(x)
 constexpr int kFlStride = 32;   // edge, omega, 1/dlnE, ln edge, lines[8], cdf[8], K share[12]
 constexpr int kKFracNodes = 12;
 constexpr int kNParts = 5;
@@ -62,6 +63,8 @@ struct Tables {
   const float* q_s;
   const float* ray_x;
   const float* ray_a;        // [n_el][n_q]
+  const int* ray_guide;      // [n_el][kRayGuide]: the search start of ray_a for a target in bucket k
+  const float* ray_gscale;   // [n_el]: kRayGuide / ray_a[el][n_q - 1] (bucket = floor(target scale))
   const float* fl;           // [n_el][20]
   const float* h10;          // [4][n_h10]
 };
@@ -502,15 +505,15 @@ __device__ __forceinline__ float rayleigh_sample(const Tables& T, int m, float E
   qerr = x_max > xg[nq - 1];
   const float dqr = 20.0f / (float)(nq - 1);
   float a_cut = interp_guess(xg, ag, nq, x_max, (int)(qm / dqr));
+  const int* gd = T.ray_guide + (size_t)el * kRayGuide;
+  const float gs = T.ray_gscale[el];
   for (int t = 0; t < kmax; ++t) {
     float x1 = rng.next(), x2 = rng.next();
     float target = x1 * a_cut;
-    // largest j with A[j] <= target (np.interp bracket)
-    int lo = 0, hi = nq - 1;
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (ag[mid] <= target) lo = mid; else hi = mid;
-    }
+    // largest j <= nq - 2 with A[j] <= target (np.interp bracket; A[0] = 0): from the guide entry of
+    // the target's bucket, which no later bracket precedes, forward over A
+    int lo = gd[min((int)(target * gs), kRayGuide - 1)];
+    while (lo < nq - 2 && ag[lo + 1] <= target) ++lo;
     float x;
     if (target >= ag[nq - 1]) x = xg[nq - 1];
     else x = xg[lo] + (xg[lo + 1] - xg[lo]) / (ag[lo + 1] - ag[lo]) * (target - ag[lo]);

@@ -3312,6 +3312,30 @@ int cc_upload_tables(void* ctx, const cc_tables* t) {
   if ((r = upload(c, t->q_s, nq, &f))) return r; T.q_s = f;
   if ((r = upload(c, t->ray_x, nq, &f))) return r; T.ray_x = f;
   if ((r = upload(c, t->ray_a, nel * nq, &f))) return r; T.ray_a = f;
+  {
+    // Rayleigh guide tables: bucket(v) = min(floor(v scale), kRayGuide - 1) in float arithmetic, as
+    // the device computes it; entry k is the largest j <= nq - 2 whose A[j] falls in an earlier
+    // bucket (0 if none).  A target in bucket k exceeds every such A[j] (bucket is monotone in v),
+    // so the largest j with A[j] <= target is at or after the entry.
+    std::vector<float> gsc(std::max<size_t>(nel, 1));
+    std::vector<int> gd(std::max<size_t>(nel * kRayGuide, 1));
+    for (size_t e = 0; e < nel; ++e) {
+      const float* A = t->ray_a + e * nq;
+      const float last = A[nq - 1];
+      const float sc = last > 0.f ? (float)kRayGuide / last : 0.f;
+      gsc[e] = sc;
+      size_t j = 0;   // first index whose bucket is >= k
+      for (int k = 0; k < kRayGuide; ++k) {
+        while (j <= nq - 2 && std::min((int)(A[j] * sc), kRayGuide - 1) < k) ++j;
+        gd[e * kRayGuide + k] = j > 0 ? (int)(j - 1) : 0;
+      }
+    }
+    int* gdev;
+    if ((r = upload(c, gd.data(), gd.size(), &gdev))) return r;
+    T.ray_guide = gdev;
+    if ((r = upload(c, gsc.data(), gsc.size(), &f))) return r;
+    T.ray_gscale = f;
+  }
   if ((r = upload(c, t->fl, nel * kFlStride, &f))) return r; T.fl = f;
   if ((r = upload(c, t->h10, (size_t)4 * t->n_h10, &f))) return r; T.h10 = f;
   c->T = T;
