@@ -2087,8 +2087,8 @@ __device__ __forceinline__ bf16x4 ld_bf16x4(const uint16_t* p) {   // 4 consecut
 __global__ __launch_bounds__(kBlock) void k_mlp_grad(const float* w, const uint16_t* wb, int n, const float* feat,
                                                      const float* yv, const float* rec, uint32_t n_rec,
                                                      uint64_t pa, uint64_t pb, uint32_t k0, MlpFrame frame,
-                                                     float yscale, int power, float gscale, float* grad,
-                                                     double* loss) {
+                                                     float yscale, int power, float gscale, float* part,
+                                                     double* lpart) {
   __shared__ __align__(16) uint16_t sW2[kMlpH * kMlpH];   // W2 [i][j]
   __shared__ float sbw[2 * kMlpH];                        // b2, w3
   __shared__ __align__(16) GradWaveLds swl[kBlock / kWave];
@@ -2260,8 +2260,26 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(const float* w, const uint1
       wave_lds_order();
     }
   }
-  // the waves' gradients into the block's, one wave after the other with plain LDS adds (LDS
-  // atomics from all four waves at once were measured 24 us of a 62 us step), then into grad
+  // The lane sums of dw3, db2, db3 and the loss (butterflies over the whole wave: every lane is
+  // active here), then the waves' gradients into the block's, one wave after the other with plain
+  // LDS adds in wave order (LDS atomics from all four waves at once were measured 24 us of a 62 us
+  // step).  The block's gradient goes out as a partial of its own with plain stores: the partials
+  // are summed in a fixed order by the kernel that consumes them, so the gradient does not depend
+  // on scheduling, and no two blocks add into one address (atomics from every block into the same
+  // 5249 floats serialised at the memory).
+#pragma unroll
+  for (int tm = 0; tm < 4; ++tm) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        dw3[tm][e] += __shfl_xor(dw3[tm][e], off);
+        db2[tm][e] += __shfl_xor(db2[tm][e], off);
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) db3 += __shfl_xor(db3, off);
+  lsum = wave_sum_f64(lsum);
   for (int wv = 0; wv < (int)(blockDim.x / kWave); ++wv) {
     if (wave == wv) {
 #pragma unroll
@@ -2273,62 +2291,109 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(const float* w, const uint1
           for (int tj = 0; tj < 4; ++tj) sGrad[kOffW2 + (16 * ti + 4 * g + e) * kMlpH + 16 * tj + c] += acc2[ti][tj][e];
         }
       }
+      if (c == 0) {   // lanes 0, 16, 32, 48: hidden units 4g .. 4g + 3 of each 16
+#pragma unroll
+        for (int tm = 0; tm < 4; ++tm) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            sGrad[kOffW3 + 16 * tm + 4 * g + e] += dw3[tm][e];
+            sGrad[kOffB2 + 16 * tm + 4 * g + e] += db2[tm][e];
+          }
+        }
+      }
+      if (lane == 0) {
+        sGrad[kOffB3] += db3;
+        sL[wave] = lsum;
+      }
     }
     __syncthreads();
   }
-#pragma unroll
-  for (int tm = 0; tm < 4; ++tm) {
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      float a = dw3[tm][e], b = db2[tm][e];
-#pragma unroll
-      for (int off = 1; off < 16; off <<= 1) {
-        a += __shfl_xor(a, off);
-        b += __shfl_xor(b, off);
-      }
-      if (c == 0) {
-        atomicAdd(&sGrad[kOffW3 + 16 * tm + 4 * g + e], a);
-        atomicAdd(&sGrad[kOffB2 + 16 * tm + 4 * g + e], b);
-      }
-    }
-  }
-  {
-    float s = db3;
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (lane == 0) atomicAdd(&sGrad[kOffB3], s);
-    double l = lsum;
-    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
-    if (lane == 0) sL[wave] = l;
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < kMlpParams; k += blockDim.x)
-    if (sGrad[k] != 0.f) atomicAdd(grad + k, sGrad[k]);
+  float* const out = part + (size_t)blockIdx.x * kMlpParams;
+  for (int k = threadIdx.x; k < kMlpParams; k += blockDim.x) out[k] = sGrad[k];
   if (threadIdx.x == 0) {
     double l = 0.0;
     for (int k = 0; k < (int)(blockDim.x / kWave); ++k) l += sL[k];
-    atomic_add_f64(loss, l);
+    lpart[blockIdx.x] = l;
   }
+}
+
+// Sums of the nb block partials of k_mlp_grad (part[b][p]) for the kRedParams parameters
+// p = kRedParams blockIdx.x + q: thread (grp, q) sums partials grp, grp + 32, ... (independent
+// loads, issued together), wave 0 adds the 32 group sums of each parameter four at a time and
+// then by a butterfly over its lanes, so every sum is formed in one fixed order.  256 threads;
+// the result is valid in threads 0..kRedParams-1.  Block 0 also sums the loss partials into *loss.
+constexpr int kRedParams = 8;
+constexpr int kRedGroups = kBlock / kRedParams;   // 32
+__device__ __forceinline__ float sum_partials(const float* part, int nb, const double* lpart, double* loss) {
+  __shared__ float red[kRedGroups][kRedParams];
+  __shared__ double lred[kBlock / kWave];
+  const int grp = threadIdx.x / kRedParams, q = threadIdx.x % kRedParams;
+  const int p = blockIdx.x * kRedParams + q;
+  float s = 0.f;
+  if (p < kMlpParams) {
+    float a[4] = {0.f, 0.f, 0.f, 0.f};
+    int b = grp;
+    for (; b + 3 * kRedGroups < nb; b += 4 * kRedGroups) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) a[k] += part[(size_t)(b + k * kRedGroups) * kMlpParams + p];
+    }
+    for (; b < nb; b += kRedGroups) a[0] += part[(size_t)b * kMlpParams + p];
+    s = (a[0] + a[1]) + (a[2] + a[3]);
+  }
+  red[grp][q] = s;
+  if (blockIdx.x == 0 && loss) {
+    double l = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) l += lpart[b];
+    l = wave_sum_f64(l);
+    if ((threadIdx.x & (kWave - 1)) == 0) lred[threadIdx.x / kWave] = l;
+  }
+  __syncthreads();
+  float t = 0.f;
+  if (threadIdx.x < kWave) {   // lane (h, q): groups 4h .. 4h + 3 of parameter q, then over h
+    const int h = threadIdx.x / kRedParams;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t += red[4 * h + k][q];
+#pragma unroll
+    for (int off = kRedParams; off < kWave; off <<= 1) t += __shfl_xor(t, off);
+  }
+  if (blockIdx.x == 0 && loss && threadIdx.x == 0) {
+    double l = 0.0;
+    for (int k = 0; k < kBlock / kWave; ++k) l += lred[k];
+    *loss = l;
+  }
+  return t;
+}
+
+// The gradient itself (cc_mlp_grad): the partials summed into grad.
+__global__ __launch_bounds__(kBlock) void k_grad_sum(const float* part, int nb, const double* lpart, float* grad,
+                                                     double* loss) {
+  const float g = sum_partials(part, nb, lpart, loss);
+  const int p = blockIdx.x * kRedParams + (int)threadIdx.x;
+  if (threadIdx.x < kRedParams && p < kMlpParams) grad[p] = g;
 }
 
 
 
-// Adam on the fp32 master weights; refreshes the bf16 copies and clears grad.
-// Rows 8..15 of W1 (padding columns of the input) stay at zero.  (Taken by the last block of
-// k_mlp_grad instead, the step ran 61 us against 39 + 5 us: one block's dependent loads.)
-__global__ void k_adam(float* w, float* m, float* v, float* grad, uint16_t* wb, float lr, float b1, float b2,
-                       float c1, float c2, float eps) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < kMlpParams; k += gridDim.x * blockDim.x) {
-    const bool pad = k < kOffW2 && (k / kMlpH) >= 8;
-    float g = pad ? 0.f : grad[k];
-    float mk = b1 * m[k] + (1.f - b1) * g;
-    float vk = b2 * v[k] + (1.f - b2) * g * g;
-    m[k] = mk;
-    v[k] = vk;
-    float wk = pad ? 0.f : w[k] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
-    w[k] = wk;
-    grad[k] = 0.f;
-    if (k < kOffB2) wb[k] = f2bf(wk);
-  }
+// One Adam step on the fp32 master weights from the block partials of k_mlp_grad (summed as
+// k_grad_sum sums them, so training is reproducible run to run), refreshing the bf16 copies; the
+// step's loss into *loss.  Rows 8..15 of W1 (padding columns of the input) stay at zero.  (Taken
+// by the last block of the gradient kernel instead, the step ran 61 us against 39 + 5 us: one
+// block's dependent loads.)
+__global__ __launch_bounds__(kBlock) void k_adam(const float* part, int nb, const double* lpart, double* loss, float* w,
+                                                 float* m, float* v, uint16_t* wb, float lr, float b1, float b2,
+                                                 float c1, float c2, float eps) {
+  const float gs = sum_partials(part, nb, lpart, loss);
+  const int k = blockIdx.x * kRedParams + (int)threadIdx.x;
+  if (threadIdx.x >= kRedParams || k >= kMlpParams) return;
+  const bool pad = k < kOffW2 && (k / kMlpH) >= 8;
+  const float g = pad ? 0.f : gs;
+  const float mk = b1 * m[k] + (1.f - b1) * g;
+  const float vk = b2 * v[k] + (1.f - b2) * g * g;
+  m[k] = mk;
+  v[k] = vk;
+  const float wk = pad ? 0.f : w[k] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+  w[k] = wk;
+  if (k < kOffB2) wb[k] = f2bf(wk);
 }
 
 __global__ void k_fill_f32(float* p, size_t n, float v) {
@@ -2710,7 +2775,7 @@ struct Ctx {
   float rkeep = 1.f;
   // two maps: 0 = states about to fly, 1 = states about to interact
   bool have_mlp[2] = {false, false};
-  float *mw[2] = {}, *mm[2] = {}, *mv[2] = {}, *mgrad[2] = {};
+  float *mw[2] = {}, *mm[2] = {}, *mv[2] = {};
   uint16_t* mwb[2] = {};
   uint32_t adam_t[2] = {0, 0};
   MlpFrame frame{};
@@ -3253,8 +3318,8 @@ int cc_ctx_destroy(void* ctx) {
   free_pool(c);
   for (void* p : c->tbufs) (void)hipFree(p);
   void* ps[] = {c->st_tally, c->st_edep, c->tr_rec, c->tr_cnt, c->tr_last, c->prims, c->tk, c->th, c->qk, c->qh, c->hits, c->G.t, c->cnt, c->stats, c->edep, c->err,
-                c->spec, c->rec, c->rcnt, c->mw[0], c->mm[0], c->mv[0], c->mgrad[0], c->mwb[0],
-                c->mw[1], c->mm[1], c->mv[1], c->mgrad[1], c->mwb[1], c->bcdf, c->bwt, c->stage};
+                c->spec, c->rec, c->rcnt, c->mw[0], c->mm[0], c->mv[0], c->mwb[0],
+                c->mw[1], c->mm[1], c->mv[1], c->mwb[1], c->bcdf, c->bwt, c->stage};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   free_batch(c);
@@ -4303,13 +4368,13 @@ int cc_mlp_set_params(void* ctx, int32_t net, const float* w) {
     if (!std::isfinite(w[k])) return fail(CC_ERR_ARG, "parameter %d is not finite", k);
   HIP_TRY(hipSetDevice(c->device));
   if (!c->mw[net] && ((r = dalloc(&c->mw[net], kMlpParams)) || (r = dalloc(&c->mm[net], kMlpParams)) ||
-                      (r = dalloc(&c->mv[net], kMlpParams)) || (r = dalloc(&c->mgrad[net], kMlpParams)) ||
+                      (r = dalloc(&c->mv[net], kMlpParams)) ||
                       (r = dalloc(&c->mwb[net], kOffB2))))
     return r;
   std::vector<float> h(w, w + kMlpParams);
   for (int k = kMlpIn / 2 * kMlpH; k < kOffW2; ++k) h[k] = 0.f;   // rows 8..15 of W1
   HIP_TRY(hipMemcpy(c->mw[net], h.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
-  float* zs[] = {c->mm[net], c->mv[net], c->mgrad[net]};
+  float* zs[] = {c->mm[net], c->mv[net]};
   for (float* z : zs) HIP_TRY(hipMemset(z, 0, kMlpParams * sizeof(float)));
   hipLaunchKernelGGL(k_mlp_refresh, dim3(32), dim3(kBlock), 0, c->stream, c->mw[net], c->mwb[net]);
   HIP_TRY(hipGetLastError());
@@ -4376,17 +4441,24 @@ int cc_mlp_grad(void* ctx, int32_t net, int64_t n, const float* feat, const floa
   if (n < 0 || n > (1ll << 28)) return fail(CC_ERR_ARG, "n out of range");
   HIP_TRY(hipSetDevice(c->device));
   DevBufs Bf;
-  float *df = nullptr, *dy = nullptr, *dg;
-  double* dl;
-  if ((r = Bf.out(kMlpParams, &dg)) || (r = Bf.out(1, &dl))) return r;
+  float *df = nullptr, *dy = nullptr, *dg, *dp;
+  double *dl, *dlp;
+  // one block partial of the gradient and the loss per gradient block (at least one, all zero
+  // for n = 0), summed in a fixed order by k_grad_sum
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((n + kGradRows - 1) / kGradRows, 2048));
+  if ((r = Bf.out(kMlpParams, &dg)) || (r = Bf.out(1, &dl)) || (r = Bf.out((size_t)nb * kMlpParams, &dp)) ||
+      (r = Bf.out((size_t)nb, &dlp)))
+    return r;
   if (n > 0 && ((r = Bf.in(feat, (size_t)n * kMlpIn, &df)) || (r = Bf.in(y, (size_t)n, &dy)))) return r;
-  hipLaunchKernelGGL(k_fill_f32, dim3(32), dim3(kBlock), 0, c->stream, dg, (size_t)kMlpParams, 0.f);
-  hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, c->stream, dl, (size_t)1, 0.0);
   if (n > 0) {
-    const int nb = (int)std::min<int64_t>((n + kGradRows - 1) / kGradRows, 2048);
     hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(kBlock), 0, c->stream, c->mw[net], c->mwb[net], (int)n, df, dy,
-                       (const float*)nullptr, 0u, (uint64_t)0, (uint64_t)0, 0u, c->frame, yscale, 1, 1.0f, dg, dl);
+                       (const float*)nullptr, 0u, (uint64_t)0, (uint64_t)0, 0u, c->frame, yscale, 1, 1.0f, dp, dlp);
+  } else {
+    hipLaunchKernelGGL(k_fill_f32, dim3(32), dim3(kBlock), 0, c->stream, dp, (size_t)kMlpParams, 0.f);
+    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, c->stream, dlp, (size_t)1, 0.0);
   }
+  hipLaunchKernelGGL(k_grad_sum, dim3((kMlpParams + kRedParams - 1) / kRedParams), dim3(kBlock), 0, c->stream,
+                     (const float*)dp, nb, (const double*)dlp, dg, dl);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   if ((r = to_host(grad, dg, kMlpParams))) return r;
@@ -4412,9 +4484,14 @@ int cc_mlp_train(void* ctx, int32_t net, const cc_train_cfg* cfg, cc_train_out* 
   if ((r = records_held(c, net, &base, &nr))) return r;
   if (nr == 0) return fail(CC_ERR_STATE, "no contribution records of kind %d held", net);
   const float* rec = c->rec + (size_t)base * CC_RECORD_FLOATS;
+  // at most kGradBlocks blocks: a larger minibatch gives each block more tiles, not more blocks
+  const int gb = (int)std::min<uint32_t>((cfg->batch + kGradRows - 1) / kGradRows, kGradBlocks);
   DevBufs Bf;
-  double *acc, *dl;
-  if ((r = Bf.out(2, &acc)) || (r = Bf.out(cfg->steps, &dl))) return r;
+  double *acc, *dl, *dlp;
+  float* dp;
+  if ((r = Bf.out(2, &acc)) || (r = Bf.out(cfg->steps, &dl)) || (r = Bf.out((size_t)gb * kMlpParams, &dp)) ||
+      (r = Bf.out((size_t)gb, &dlp)))
+    return r;
   hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, c->stream, acc, (size_t)2, 0.0);
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(cfg->steps)), dim3(kBlock), 0, c->stream, dl, (size_t)cfg->steps, 0.0);
   hipLaunchKernelGGL(k_record_sum, dim3(grid_blocks(nr)), dim3(kBlock), 0, c->stream, rec, nr, acc);
@@ -4447,9 +4524,6 @@ int cc_mlp_train(void* ctx, int32_t net, const cc_train_cfg* cfg, cc_train_out* 
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   HIP_TRY(hipEventRecord(e0, c->stream));
-  // at most kGradBlocks blocks: a larger minibatch gives each block more tiles, not more blocks,
-  // so the global gradient atomics per step stay at kGradBlocks x kMlpParams
-  const int gb = (int)std::min<uint32_t>((cfg->batch + kGradRows - 1) / kGradRows, kGradBlocks);
   const float gscale = 1.0f / (float)cfg->batch;
   double b1t = std::pow((double)cfg->beta1, (double)c->adam_t[net]);
   double b2t = std::pow((double)cfg->beta2, (double)c->adam_t[net]);
@@ -4457,16 +4531,16 @@ int cc_mlp_train(void* ctx, int32_t net, const cc_train_cfg* cfg, cc_train_out* 
     const uint32_t k0 = (uint32_t)(((uint64_t)t * cfg->batch) % nr);
     hipLaunchKernelGGL(k_mlp_grad, dim3(gb), dim3(kBlock), 0, c->stream, (const float*)c->mw[net],
                        (const uint16_t*)c->mwb[net], (int)cfg->batch, (const float*)nullptr, (const float*)nullptr,
-                       rec, nr, pa, pb, k0, c->frame, ys, power, gscale, c->mgrad[net], dl + t);
+                       rec, nr, pa, pb, k0, c->frame, ys, power, gscale, dp, dlp);
     ++c->adam_t[net];
     b1t *= cfg->beta1;
     b2t *= cfg->beta2;
     float lr = cfg->lr;
     if (cfg->lr_end > 0.f && cfg->steps > 1)   // cosine decay from lr to lr_end
       lr = (float)(cfg->lr_end + 0.5 * (cfg->lr - cfg->lr_end) * (1.0 + std::cos(M_PI * t / (cfg->steps - 1))));
-    hipLaunchKernelGGL(k_adam, dim3((kMlpParams + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->mw[net],
-                       c->mm[net], c->mv[net], c->mgrad[net], c->mwb[net], lr, cfg->beta1, cfg->beta2,
-                       (float)(1.0 - b1t), (float)(1.0 - b2t), cfg->eps);
+    hipLaunchKernelGGL(k_adam, dim3((kMlpParams + kRedParams - 1) / kRedParams), dim3(kBlock), 0, c->stream,
+                       (const float*)dp, gb, (const double*)dlp, dl + t, c->mw[net], c->mm[net], c->mv[net],
+                       c->mwb[net], lr, cfg->beta1, cfg->beta2, (float)(1.0 - b1t), (float)(1.0 - b2t), cfg->eps);
   }
   HIP_TRY(hipEventRecord(e1, c->stream));
   HIP_TRY(hipGetLastError());

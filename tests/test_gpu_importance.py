@@ -304,6 +304,25 @@ def test_adam_step_matches_oracle(engine):
     assert np.all(w1[M.OFF_W1 + 8 * M.N_H:M.OFF_W2] == 0)
 
 
+def test_training_is_reproducible(engine):
+    """Each gradient block writes its partial sum and the Adam kernel adds the partials in one
+    fixed order, so two trainings from the same parameters on the same records give bit-identical
+    weights and losses."""
+    from chatcarlo_amd.importance import LearnedImportance
+    raw, c, r = _side_scene()
+    li = LearnedImportance(engine, raw, c, r)
+    n_rec, _ = li.generate_records(50_000, 1_000_000, keep=0.5, seed=52, batch=50_000)
+    assert n_rec > 4096
+    runs = []
+    for _ in range(2):
+        li.set_params(_random_params(13))
+        res = li.train(0, steps=20, batch=4096, lr=3e-3, reinit=False)
+        runs.append((li.get_params(0), res.loss_first, res.loss_last))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1:] == runs[1][1:]
+    assert not np.array_equal(runs[0][0], _random_params(13))
+
+
 def test_learned_importance_is_unbiased_and_beats_analog(engine):
     """Trained map on the angio room: every learned run agrees with its analog run within 3.5 sigma,
     and the median figure of merit 1/(R^2 t) over nine seed pairs, at the same relative error, is at
