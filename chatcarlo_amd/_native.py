@@ -79,7 +79,8 @@ class CcRunOut(C.Structure):
                 ("kernel_launches", C.c_uint64 * 8),
                 ("region_sum", C.c_double), ("region_sum2", C.c_double),
                 ("region_batch_sum2", C.c_double), ("n_split", C.c_uint64),
-                ("n_roulette", C.c_uint64), ("n_records", C.c_uint64)]
+                ("n_roulette", C.c_uint64), ("n_records", C.c_uint64),
+                ("kernel_busy_ms", C.c_double * 8)]
 
 
 class CcTrainCfg(C.Structure):

@@ -247,8 +247,8 @@ __device__ __forceinline__ bool isect_cylinder(const Prim& P, const float o[3], 
   const int ax = P.axis;
   const int i0 = (ax == 0) ? 1 : 0;
   const int i1 = (ax == 2) ? 1 : 2;
-  ff ox = two_sum(sel3(i0, o[0], o[1], o[2]), -P.p[i0]);
-  ff oy = two_sum(sel3(i1, o[0], o[1], o[2]), -P.p[i1]);
+  ff ox = two_sum(sel3(i0, o[0], o[1], o[2]), -sel3(i0, P.p[0], P.p[1], P.p[2]));
+  ff oy = two_sum(sel3(i1, o[0], o[1], o[2]), -sel3(i1, P.p[0], P.p[1], P.p[2]));
   float dx = sel3(i0, d[0], d[1], d[2]), dy = sel3(i1, d[0], d[1], d[2]);
   ff aa = ff_add(two_prod(dx, dx), two_prod(dy, dy));
   float a = ff_f(aa);
@@ -270,8 +270,9 @@ __device__ __forceinline__ bool isect_cylinder(const Prim& P, const float o[3], 
   // caps at c_ax +- half_h, plane offsets formed exactly in float-float
   float dz = sel3(ax, d[0], d[1], d[2]);
   const float oz = sel3(ax, o[0], o[1], o[2]);
-  ff lo_rel = ff_add(two_sum(P.p[ax], -P.p[4]), {-oz, 0.f});
-  ff hi_rel = ff_add(two_sum(P.p[ax], P.p[4]), {-oz, 0.f});
+  const float cz = sel3(ax, P.p[0], P.p[1], P.p[2]);
+  ff lo_rel = ff_add(two_sum(cz, -P.p[4]), {-oz, 0.f});
+  ff hi_rel = ff_add(two_sum(cz, P.p[4]), {-oz, 0.f});
   float ten_z, tex_z;
   bool miss_z = false;
   if (fabsf(dz) < 1e-12f) {
@@ -318,26 +319,56 @@ __device__ __forceinline__ bool contains(const Prim& P, const float q[3]) {
   const int ax = P.axis;
   const int i0 = (ax == 0) ? 1 : 0;
   const int i1 = (ax == 2) ? 1 : 2;
-  ff s = ff_add(ff_sqr(two_sum(sel3(i0, q[0], q[1], q[2]), -P.p[i0])),
-                ff_sqr(two_sum(sel3(i1, q[0], q[1], q[2]), -P.p[i1])));
+  ff s = ff_add(ff_sqr(two_sum(sel3(i0, q[0], q[1], q[2]), -sel3(i0, P.p[0], P.p[1], P.p[2]))),
+                ff_sqr(two_sum(sel3(i1, q[0], q[1], q[2]), -sel3(i1, P.p[0], P.p[1], P.p[2]))));
   s = ff_add(s, ff_neg(two_prod(P.p[3], P.p[3])));
   if (!(s.h < 0.f || (s.h == 0.f && s.l <= 0.f))) return false;
-  ff z = two_sum(sel3(ax, q[0], q[1], q[2]), -P.p[ax]);
+  ff z = two_sum(sel3(ax, q[0], q[1], q[2]), -sel3(ax, P.p[0], P.p[1], P.p[2]));
   ff za = (z.h < 0.f || (z.h == 0.f && z.l < 0.f)) ? ff_neg(z) : z;
   ff e = ff_add(za, {-P.p[4], 0.f});
   return e.h < 0.f || (e.h == 0.f && e.l <= 0.f);
 }
 
+// A scene's primitives read through the constant address space.  In every loop over the
+// primitives the index is uniform across the wave, so these loads are scalar (s_load, through
+// the scalar cache) and the fields live in SGPRs: the kind of each primitive is a scalar branch,
+// and no VGPRs or LDS round trips go to them.  Valid for memory that no kernel writes while
+// it is read (a scene is uploaded before its launches).
+struct ConstPrims {
+  const __attribute__((address_space(4))) int* w;
+};
+__device__ __forceinline__ ConstPrims const_prims(const Prim* p) {
+  return ConstPrims{(const __attribute__((address_space(4))) int*)p};
+}
+__device__ __forceinline__ Prim load_prim(ConstPrims s, int i) {
+  static_assert(sizeof(Prim) == 12 * sizeof(int), "Prim is 12 words");
+  const __attribute__((address_space(4))) int* q = s.w + 12 * i;
+  Prim P;
+  P.kind = q[0];
+  P.axis = q[1];
+  P.material = q[2];
+  P.pad = q[3];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) P.p[k] = __int_as_float(q[4 + k]);
+  return P;
+}
+// per-lane primitives (batched scenes: each lane's scene has its own array)
+__device__ __forceinline__ Prim load_prim(const Prim* s, int i) { return s[i]; }
+
 // Last primitive containing the point wins (chatcarlo/geometry.py:205-212).
-__device__ __forceinline__ int locate(const Prim* prims, int n_prim, int background, const float q[3]) {
-  for (int i = n_prim - 1; i >= 0; --i)
-    if (contains(prims[i], q)) return prims[i].material;
+template <class PS>
+__device__ __forceinline__ int locate(PS prims, int n_prim, int background, const float q[3]) {
+  for (int i = n_prim - 1; i >= 0; --i) {
+    const Prim P = load_prim(prims, i);
+    if (contains(P, q)) return P.material;
+  }
   return background;
 }
 
 // Min root beyond tmin over all primitives vs the world exit (chatcarlo/geometry.py:214-231,
 // where tmin is the reference's 1e-6 cm threshold).
-__device__ __forceinline__ float next_boundary(const Prim* prims, int n_prim, const World& W,
+template <class PS>
+__device__ __forceinline__ float next_boundary(PS prims, int n_prim, const World& W,
                                                const float o[3], const float d[3], float tmin,
                                                bool& escape) {
   // one reciprocal per axis and call, not two divisions per axis and box; every box root is
@@ -346,7 +377,7 @@ __device__ __forceinline__ float next_boundary(const Prim* prims, int n_prim, co
   float t_obj = INFINITY;
   for (int i = 0; i < n_prim; ++i) {
     float t1, t2;
-    if (isect_prim(prims[i], o, d, inv_d, t1, t2)) {
+    if (isect_prim(load_prim(prims, i), o, d, inv_d, t1, t2)) {
       if (t1 > tmin && t1 < t_obj) t_obj = t1;
       if (t2 > tmin && t2 < t_obj) t_obj = t2;
     }
@@ -627,7 +658,7 @@ __device__ __forceinline__ void atomic_add_f64(double* p, double v) {
 // voxel owns goes straight to memory; the table is added into the grid once, when the
 // block ends.  (Probing further slots was measured slower: when the table misses, the
 // voxels are not shared anyway.)
-constexpr int kCacheSlots = 2304;   // a multiple of 4; 45 KB, so that three blocks fit a CU's LDS
+constexpr int kCacheSlots = 2560;   // a multiple of 4; 50 KB, so that three blocks fit a CU's LDS (2688 fit two)
 constexpr uint32_t kCacheEmpty = 0xffffffffu;
 template <int kSlots>
 struct TallyCacheT {

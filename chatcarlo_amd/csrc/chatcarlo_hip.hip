@@ -16,8 +16,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/chatcarlo_hip.h"
@@ -352,14 +354,6 @@ __device__ __forceinline__ void wave_add_by_key(bool active, int key, double val
 }
 
 __device__ __forceinline__ void raise_err(int* err, int bits) { atomicOr(err, bits); }
-
-__device__ __forceinline__ void stage_prims(const Prim* g, Prim* s, int n) {
-  const int* src = reinterpret_cast<const int*>(g);
-  int* dst = reinterpret_cast<int*>(s);
-  const int words = n * (int)(sizeof(Prim) / sizeof(int));
-  for (int k = threadIdx.x; k < words; k += blockDim.x) dst[k] = src[k];
-  __syncthreads();
-}
 
 // ---------------------------------------------------------------------------
 // source (chatcarlo/source.py:67-210, spectrum.py:82)
@@ -719,7 +713,8 @@ __device__ __forceinline__ bool record_take(const Records& R, uint32_t hid, uint
 // walked boundary by boundary up to the sphere's far side.  Its expectation is
 // the realised chord's, so contribution records built from it keep their mean
 // and lose the 0-or-chord noise of single flights.
-__device__ float region_expect(const Prim* sp, const World& W, const Tables& T, const Region& R,
+template <class PS>
+__device__ float region_expect(PS sp, const World& W, const Tables& T, const Region& R,
                                const float o0[3], const float d[3], float E, const EPos& ep, float eps0) {
   const float ox = o0[0] - R.c[0], oy = o0[1] - R.c[1], oz = o0[2] - R.c[2];
   const float b = ox * d[0] + oy * d[1] + oz * d[2];
@@ -771,12 +766,10 @@ __device__ float region_expect(const Prim* sp, const World& W, const Tables& T, 
 template <bool kRec, bool kMulti, bool kTrace = false, bool kStream = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_flight(Launch L, const int* act, const uint32_t* cnt_in, int* qp,
                                                    int* qc, int* qr, int* coll) {
-  __shared__ Prim sp_lds[kMulti ? 1 : CC_MAX_PRIMS];
   __shared__ TallyCache tc_lds;
   __shared__ BlockQueues bq;
   TallyCache* tcache = &tc_lds;
   if (L.grid_on) cache_init(tcache);
-  if (!kMulti) stage_prims(L.prims, sp_lds, L.W.n_prim);
   const uint32_t n = *cnt_in;
   const Tables& T = L.T;
   PState S = L.S;
@@ -796,9 +789,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     uint64_t key = 0;
     uint32_t hid = 0, tag = 0, ctr = 0;
     EPos ep{0, 0.f, 0.f};
-    // batched scenes read their scene's primitives from global memory: starting from that array's
-    // base keeps every value of sp a global pointer, so its loads are global, not flat, loads
-    const Prim* sp = kMulti ? L.MS.prims : sp_lds;
+    // one scene: its primitives through the constant address space (scalar loads, uniform);
+    // batched scenes: each lane's scene's array in global memory, starting from that array's base
+    // so that every value of sp is a global pointer and its loads are global, not flat, loads
+    std::conditional_t<kMulti, const Prim*, ConstPrims> sp;
+    if constexpr (kMulti) sp = L.MS.prims; else sp = const_prims(L.prims);
     World Wd = L.W;
     GridDev G = L.G;
     if (valid) {
@@ -811,7 +806,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         if (kStream || kMulti) scene = (int)r3.w;   // W_SC: the batch slot (streamed run) or scene (batched scenes)
         if (kTrace) flight = (int)r3.x;             // W_NSC
       }
-      if (kMulti) {
+      if constexpr (kMulti) {
         sp = L.MS.prims + L.MS.prim_off[scene];
         Wd = L.MS.worlds[scene];
         G = L.MS.grids[scene];
@@ -1252,10 +1247,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     Launch L, Src src, int epoch, const int* act, const uint32_t* cnt_act, uint32_t* claim) {
   constexpr int kPersistSlots = PersistShape<kThreads, kStream>::kSlots;
   __shared__ TallyCacheT<kPersistSlots> tc_lds;
-  __shared__ Prim sp_lds[kStream ? CC_MAX_PRIMS : 1];
   TallyCacheT<kPersistSlots>* const tcache = &tc_lds;
   cache_init(tcache);
-  if (kStream) stage_prims(L.prims, sp_lds, L.W.n_prim);
   const MultiScene& M = L.MS;
   const Tables& T = L.T;
   const int lane = __lane_id();
@@ -1265,7 +1258,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   uint64_t key = kStream ? L.seed : 0;
   float E = 1.f, tau = 0.f, w = 1.f;
   float p[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 1.f};
-  const Prim* sp = kStream ? sp_lds : M.prims;
+  // a streamed run's one scene through the constant address space (scalar loads); batched
+  // scenes per lane from global memory
+  std::conditional_t<kStream, ConstPrims, const Prim*> sp;
+  if constexpr (kStream) sp = const_prims(L.prims); else sp = M.prims;
   World Wd = L.W;
   // the current flight
   bool fresh = false;
@@ -1408,7 +1404,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             }
             if (lo != scene) {
               scene = lo;
-              sp = M.prims + M.prim_off[lo];
+              if constexpr (!kStream) sp = M.prims + M.prim_off[lo];
               Wd = M.worlds[lo];
               key = M.seeds[lo];
             }
@@ -2516,8 +2512,7 @@ __global__ void k_add_i32(int* dst, const int* src, size_t n) {
 // ---------------------------------------------------------------------------
 __global__ void k_test_boundary(const Prim* prims, World W, int n, const float* o, const float* d, float eps,
                                 float* t, int* esc) {
-  __shared__ Prim sp[CC_MAX_PRIMS];
-  stage_prims(prims, sp, W.n_prim);
+  const ConstPrims sp = const_prims(prims);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float oo[3] = {o[3 * i], o[3 * i + 1], o[3 * i + 2]};
@@ -2528,8 +2523,7 @@ __global__ void k_test_boundary(const Prim* prims, World W, int n, const float* 
 }
 
 __global__ void k_locate(const Prim* prims, World W, int n, const float* pts, int* mat) {
-  __shared__ Prim sp[CC_MAX_PRIMS];
-  stage_prims(prims, sp, W.n_prim);
+  const ConstPrims sp = const_prims(prims);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float q[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
@@ -2722,6 +2716,11 @@ int grid_blocks(size_t n) {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;      // the second lane of streamed runs (created on first use)
+  std::vector<hipEvent_t> ev2;        // timing events of the second lane
+  uint32_t* cnt2 = nullptr;           // the second lane's counters
+  hipEvent_t lane_ev[2] = {nullptr, nullptr};   // lane start and end joins
+  hipEvent_t t_base = nullptr;        // time origin of a timed streamed run (busy-time intervals)
   unsigned char* scratch = nullptr;   // this device's copy of `scratch` (cc_transport_noalloc)
   // tables
   bool have_tables = false;
@@ -3000,21 +2999,31 @@ Launch make_launch(Ctx* c, uint64_t seed, uint64_t hist0, int grid_on, int fluor
 // segments: mark(kind) ends the open segment (if any) and opens one of `kind`, mark(-1) just ends
 // it, so back-to-back kernels share one event per boundary (every event adds a few microseconds
 // between kernels).  begin(kind) / end() bracket a single segment.
+struct TimedSeg {
+  int kind;
+  float t0, t1;   // ms after the run's time origin
+};
 struct Timer {
   Ctx* c;
   bool on;
   std::vector<int> kinds;   // kinds[i]: kind of the segment that event i opens (-1: none)
+  hipStream_t stream = nullptr;           // the stream timed (default: the context's)
+  std::vector<hipEvent_t>* evs = nullptr; // its events (default: the context's)
+  hipEvent_t base = nullptr;              // when set, every segment's interval after it into segs
+  std::vector<TimedSeg>* segs = nullptr;
+  std::vector<hipEvent_t>& pool() { return evs ? *evs : c->ev; }
   hipEvent_t next() {
-    if (kinds.size() >= c->ev.size()) {
+    std::vector<hipEvent_t>& ev = pool();
+    if (kinds.size() >= ev.size()) {
       hipEvent_t e;
       (void)hipEventCreate(&e);
-      c->ev.push_back(e);
+      ev.push_back(e);
     }
-    return c->ev[kinds.size()];
+    return ev[kinds.size()];
   }
   void mark(int kind) {
     if (!on) return;
-    (void)hipEventRecord(next(), c->stream);
+    (void)hipEventRecord(next(), stream ? stream : c->stream);
     kinds.push_back(kind);
   }
   int begin(int kind) {
@@ -3028,8 +3037,14 @@ struct Timer {
     for (size_t i = 0; i + 1 < kinds.size(); ++i) {
       if (kinds[i] < 0) continue;
       float t = 0.f;
-      (void)hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]);
+      (void)hipEventElapsedTime(&t, pool()[i], pool()[i + 1]);
       ms[kinds[i]] += t;
+      if (base && segs) {
+        float a = 0.f, b = 0.f;
+        (void)hipEventElapsedTime(&a, base, pool()[i]);
+        (void)hipEventElapsedTime(&b, base, pool()[i + 1]);
+        segs->push_back(TimedSeg{kinds[i], a, b});
+      }
       if (launches) launches[kinds[i]] += 1;
     }
     kinds.clear();
@@ -3038,6 +3053,27 @@ struct Timer {
 
 enum { K_SOURCE = 0, K_FLIGHT = 1, K_PHOTO = 2, K_COMPTON = 3, K_RAYLEIGH = 4, K_END = 5, K_OTHER = 6,
        K_INTERACT = 7 };
+
+// Per kernel kind, the time during which at least one of its timed segments ran: the length of
+// the union of their intervals (launches of one kind on two streams overlap).
+void add_busy(std::vector<TimedSeg>& v, double* busy) {
+  std::sort(v.begin(), v.end(), [](const TimedSeg& a, const TimedSeg& b) {
+    return a.kind != b.kind ? a.kind < b.kind : a.t0 < b.t0;
+  });
+  for (size_t i = 0; i < v.size();) {
+    const int k = v[i].kind;
+    float lo = v[i].t0, hi = v[i].t1;
+    for (++i; i < v.size() && v[i].kind == k; ++i) {
+      if (v[i].t0 > hi) {
+        busy[k] += hi - lo;
+        lo = v[i].t0;
+      }
+      hi = std::max(hi, v[i].t1);
+    }
+    busy[k] += hi - lo;
+  }
+  v.clear();
+}
 
 // Per-round extras of the learned-importance runs.
 struct RoundOpts {
@@ -3058,6 +3094,10 @@ struct RoundOpts {
 // grids (all kernels stride over their lists) are sized for four times that count.
 // Rounds that find an empty list exit at once.
 constexpr int kRoundChunk = 6;
+// Streamed runs of at least two full pools run as this many lanes: independent halves of the
+// pool, each with its own counters, free-slot stack and history cursor over its own part of the
+// histories, and its own stream, so that one lane's kernels fill the tails of the other's.
+constexpr int kStreamLanes = 2;
 int run_rounds(Ctx* c, const Launch& L, uint32_t n, Timer& tm, uint32_t* rounds_out, const RoundOpts& o) {
   int cur = C_ACT_A, nxt = C_ACT_B;
   int* act_cur = c->act0;
@@ -3144,10 +3184,12 @@ int ensure_queue(Ctx* c, uint32_t pool) {
     if ((r = dalloc(&c->b_free, pool))) return r;
     c->b_free_cap = pool;
   }
-  if (!c->b_free_cnt && ((r = dalloc(&c->b_free_cnt, 1)) || (r = dalloc(&c->b_cursor, 1)) ||
-                         (r = dalloc(&c->b_status, 2))))
+  // one free count, cursor and pair of status words per lane of a streamed run
+  if (!c->b_free_cnt && ((r = dalloc(&c->b_free_cnt, kStreamLanes)) || (r = dalloc(&c->b_cursor, kStreamLanes)) ||
+                         (r = dalloc(&c->b_status, 2 * kStreamLanes))))
     return r;
-  if (!c->h_status) HIP_TRY(hipHostMalloc((void**)&c->h_status, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+  if (!c->h_status)
+    HIP_TRY(hipHostMalloc((void**)&c->h_status, 2 * kStreamLanes * sizeof(unsigned long long), hipHostMallocDefault));
   return CC_OK;
 }
 
@@ -3315,6 +3357,7 @@ int cc_ctx_destroy(void* ctx) {
   if (!c) return CC_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   free_pool(c);
   for (void* p : c->tbufs) (void)hipFree(p);
   void* ps[] = {c->st_tally, c->st_edep, c->tr_rec, c->tr_cnt, c->tr_last, c->prims, c->tk, c->th, c->qk, c->qh, c->hits, c->G.t, c->cnt, c->stats, c->edep, c->err,
@@ -3332,6 +3375,12 @@ int cc_ctx_destroy(void* ctx) {
     if (e) (void)hipEventDestroy(e);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
   for (auto e : c->ev) (void)hipEventDestroy(e);
+  for (auto e : c->ev2) (void)hipEventDestroy(e);
+  if (c->t_base) (void)hipEventDestroy(c->t_base);
+  for (auto e : c->lane_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->cnt2) (void)hipFree(c->cnt2);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CC_OK;
@@ -3672,8 +3721,24 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
   constexpr size_t kStreamDep = (size_t)kStreamSlots * CC_MAX_MAT;   // deposit sums per copy: [slot][material]
   if (!c->st_edep && (r = dalloc(&c->st_edep, kAccCopies * kStreamDep))) return r;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->err, (size_t)1, 0);
-  Timer tm{c, (flags & CC_RUN_TIMING) != 0};
-  const int nbk = grid_blocks(pool);
+  // two lanes when the run fills two pools (persistent runs keep one launch per group)
+  const int lanes = (!persistent && n_hist >= 2ull * kStreamPool) ? kStreamLanes : 1;
+  const uint32_t lpool = pool / (uint32_t)lanes;   // photons in flight per lane
+  if (lanes > 1) {
+    if (!c->stream2) HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    if (!c->cnt2 && (r = dalloc(&c->cnt2, C_NCNT))) return r;
+    for (auto& e : c->lane_ev)
+      if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  const bool timing = (flags & CC_RUN_TIMING) != 0;
+  std::vector<TimedSeg> segs;
+  if (timing) {
+    if (!c->t_base) HIP_TRY(hipEventCreate(&c->t_base));
+    HIP_TRY(hipEventRecord(c->t_base, c->stream));
+  }
+  Timer tm{c, timing, {}, nullptr, nullptr, timing ? c->t_base : nullptr, &segs};
+  Timer tm2{c, timing, {}, c->stream2, &c->ev2, timing ? c->t_base : nullptr, &segs};
+  const int nbk = grid_blocks(lpool);
   int err_bits = 0;
   int drain_blocks = 0, stream_blocks = 0;   // every block of the persistent launches resident at once
   {
@@ -3685,24 +3750,31 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
     stream_blocks = std::max(1, per_cu2) * std::max(1, ncu);
   }
   const unsigned long long drain_below = (unsigned long long)drain_blocks * 512ull * kDrainPerLane;
+  // One lane: a contiguous part of the group's histories on its own slice of the pool.
+  struct Lane {
+    hipStream_t st;
+    Timer* tm;
+    Launch L;
+    uint32_t* cnt;
+    int *act_cur, *act_nxt, *qp, *qc, *qr;
+    unsigned long long *status, *h_status;
+    uint64_t h_end;
+    int cur, nxt, chunk;
+    uint32_t rounds;
+    bool done;
+  };
   for (uint64_t g0 = 0; g0 < nbt; g0 += (uint64_t)slots) {
     const int ns = (int)std::min<uint64_t>((uint64_t)slots, nbt - g0);
     const uint64_t h_begin = g0 * batch, h_end = std::min<uint64_t>(n_hist, (g0 + ns) * (uint64_t)batch);
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, c->stream, c->cnt, c->stats, c->edep, (int)C_ACT_A, 0u);
+    if (lanes > 1) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, (int*)c->cnt2, (size_t)C_NCNT, 0);
     hipLaunchKernelGGL(k_fill_f64, dim3(grid_blocks(kAccCopies * kStreamDep)), dim3(kBlock), 0, c->stream, c->st_edep,
                        kAccCopies * kStreamDep, 0.0);
-    hipLaunchKernelGGL(k_iota, dim3(grid_blocks(pool)), dim3(kBlock), 0, c->stream, c->b_free, pool);
-    hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, c->stream, c->b_free_cnt, pool, c->b_cursor,
-                       (unsigned long long)h_begin);
     Launch L = make_launch(c, seed, first, grid_on ? 1 : 0, (flags & CC_RUN_FLUORESCENCE) ? 1 : 0);
     L.multi = 1;   // deposits and RNG keys per batch slot, freed slots to the stack
     L.edep = c->st_edep;
     L.dep_stride = kStreamDep;
     L.MS.seeds = nullptr;   // every batch slot of a streamed run draws from the run's seed
-    L.MS.free_stack = c->b_free;
-    L.MS.free_cnt = c->b_free_cnt;
-    L.MS.cursor = c->b_cursor;
-    L.MS.total = (unsigned long long)h_end;
     L.St.batch = batch;
     L.St.g0 = g0;
     L.St.slot_stride = nvp;
@@ -3711,58 +3783,110 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
       L.G.cstride = 2 * (size_t)slots * nvp;
       L.G.base = 0;
     }
+    Lane ln[kStreamLanes];
+    for (int k = 0; k < lanes; ++k) {
+      Lane& q = ln[k];
+      const size_t off = (size_t)k * lpool;   // the lane's slice of the pool, lists and stack
+      const uint64_t l0 = h_begin + (h_end - h_begin) * (uint64_t)k / (uint64_t)lanes;
+      q.h_end = h_begin + (h_end - h_begin) * (uint64_t)(k + 1) / (uint64_t)lanes;
+      q.st = k == 0 ? c->stream : c->stream2;
+      q.tm = k == 0 ? &tm : &tm2;
+      q.cnt = k == 0 ? c->cnt : c->cnt2;
+      q.act_cur = c->act0 + off;
+      q.act_nxt = c->act1 + off;
+      q.qp = c->qp + off;
+      q.qc = c->qc + off;
+      q.qr = c->qr + off;
+      q.status = c->b_status + 2 * k;
+      q.h_status = c->h_status + 2 * k;
+      q.cur = C_ACT_A;
+      q.nxt = C_ACT_B;
+      q.chunk = kRoundChunk;
+      q.rounds = 0;
+      q.done = false;
+      q.L = L;
+      q.L.S = records_state(c->photons + off * kPhotonWords, c->S.score + off);
+      q.L.cnt = q.cnt;
+      q.L.MS.free_stack = c->b_free + off;
+      q.L.MS.free_cnt = c->b_free_cnt + k;
+      q.L.MS.cursor = c->b_cursor + k;
+      q.L.MS.total = (unsigned long long)q.h_end;
+      hipLaunchKernelGGL(k_iota, dim3(grid_blocks(lpool)), dim3(kBlock), 0, c->stream, q.L.MS.free_stack, lpool);
+      hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, c->stream, q.L.MS.free_cnt, lpool, q.L.MS.cursor,
+                         (unsigned long long)l0);
+    }
+    if (lanes > 1) {   // the second lane starts once the group is set up
+      HIP_TRY(hipEventRecord(c->lane_ev[0], c->stream));
+      HIP_TRY(hipStreamWaitEvent(c->stream2, c->lane_ev[0], 0));
+    }
     HIP_TRY(hipGetLastError());
-    int cur = C_ACT_A, nxt = C_ACT_B;
-    int* act_cur = c->act0;
-    int* act_nxt = c->act1;
-    uint32_t rounds = 0;
-    int chunk = kRoundChunk;
     if (persistent) {   // the group's histories in one persistent launch
       tm.begin(K_OTHER);
-      hipLaunchKernelGGL((k_persistent<256, true>), dim3(stream_blocks), dim3(256), 0, c->stream, L, s, kStreamEpoch,
-                         (const int*)nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr);
+      hipLaunchKernelGGL((k_persistent<256, true>), dim3(stream_blocks), dim3(256), 0, c->stream, ln[0].L, s,
+                         kStreamEpoch, (const int*)nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr);
       tm.end();
       HIP_TRY(hipGetLastError());
+      ln[0].done = true;
     }
-    for (; !persistent;) {
-      for (int k = 0; k < chunk; ++k) {
-        tm.mark(K_SOURCE);
-        hipLaunchKernelGGL(k_refill_stream, dim3(nbk), dim3(kBlock), 0, c->stream, L, s, act_cur, c->cnt + cur, pool);
-        hipLaunchKernelGGL(k_commit_and_reset, dim3(1), dim3(64), 0, c->stream, L, pool, nxt);
-        tm.mark(K_FLIGHT);
-        hipLaunchKernelGGL((k_flight<false, false, false, true>), dim3(nbk), dim3(kBlock), 0, c->stream, L,
-                           (const int*)act_cur, (const uint32_t*)(c->cnt + cur), c->qp, c->qc, c->qr, (int*)nullptr);
-        tm.mark(K_INTERACT);
-        hipLaunchKernelGGL(k_interact, dim3(nbk), dim3(kBlock), 0, c->stream, L, (const int*)c->qp,
-                           (const int*)c->qc, (const int*)c->qr, act_nxt, c->cnt + nxt);
-        std::swap(cur, nxt);
-        std::swap(act_cur, act_nxt);
-      }
-      tm.mark(-1);
-      rounds += chunk;
-      hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(64), 0, c->stream, L, (const uint32_t*)(c->cnt + cur),
-                         c->b_status);
-      HIP_TRY(hipMemcpyAsync(c->h_status, c->b_status, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                             c->stream));
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(hipStreamSynchronize(c->stream));
-      tm.collect(out->kernel_ms, out->kernel_launches);
-      if (c->h_status[0] >= h_end && c->h_status[1] == 0) break;
-      if (c->h_status[0] >= h_end) {
-        if (c->h_status[1] <= drain_below) {   // the last photons in flight, to their ends in one launch
-          hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, (int*)(c->cnt + C_DRAIN), (size_t)1, 0);
-          tm.begin(K_OTHER);
-          hipLaunchKernelGGL((k_persistent<512, true>), dim3(drain_blocks), dim3(512), 0, c->stream, L, s, kDrainEpoch,
-                             (const int*)act_cur, (const uint32_t*)(c->cnt + cur), c->cnt + C_DRAIN);
-          tm.end();
-          HIP_TRY(hipGetLastError());
-          break;
+    // Rounds are issued in chunks per lane without a host wait; each lane's status is read
+    // once per chunk, while the other lane's chunk runs.
+    for (int live = persistent ? 0 : lanes; live > 0;) {
+      for (int k = 0; k < lanes; ++k) {
+        Lane& q = ln[k];
+        if (q.done) continue;
+        for (int j = 0; j < q.chunk; ++j) {
+          q.tm->mark(K_SOURCE);
+          hipLaunchKernelGGL(k_refill_stream, dim3(nbk), dim3(kBlock), 0, q.st, q.L, s, q.act_cur, q.cnt + q.cur,
+                             lpool);
+          hipLaunchKernelGGL(k_commit_and_reset, dim3(1), dim3(64), 0, q.st, q.L, lpool, q.nxt);
+          q.tm->mark(K_FLIGHT);
+          hipLaunchKernelGGL((k_flight<false, false, false, true>), dim3(nbk), dim3(kBlock), 0, q.st, q.L,
+                             (const int*)q.act_cur, (const uint32_t*)(q.cnt + q.cur), q.qp, q.qc, q.qr,
+                             (int*)nullptr);
+          q.tm->mark(K_INTERACT);
+          hipLaunchKernelGGL(k_interact, dim3(nbk), dim3(kBlock), 0, q.st, q.L, (const int*)q.qp, (const int*)q.qc,
+                             (const int*)q.qr, q.act_nxt, q.cnt + q.nxt);
+          std::swap(q.cur, q.nxt);
+          std::swap(q.act_cur, q.act_nxt);
         }
-        chunk = 1;
+        q.tm->mark(-1);
+        q.rounds += q.chunk;
+        hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(64), 0, q.st, q.L, (const uint32_t*)(q.cnt + q.cur),
+                           q.status);
+        HIP_TRY(hipMemcpyAsync(q.h_status, q.status, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, q.st));
+        HIP_TRY(hipGetLastError());
       }
-      if (rounds >= (uint32_t)kMaxRounds) return fail(CC_ERR_DEVICE, "streamed run did not drain");
+      for (int k = 0; k < lanes; ++k) {
+        Lane& q = ln[k];
+        if (q.done) continue;
+        HIP_TRY(hipStreamSynchronize(q.st));
+        q.tm->collect(out->kernel_ms, out->kernel_launches);
+        if (q.h_status[0] >= q.h_end && q.h_status[1] == 0) {
+          q.done = true;
+        } else if (q.h_status[0] >= q.h_end) {
+          if (q.h_status[1] <= drain_below) {   // the last photons in flight, to their ends in one launch
+            hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, q.st, (int*)(q.cnt + C_DRAIN), (size_t)1, 0);
+            q.tm->begin(K_OTHER);
+            hipLaunchKernelGGL((k_persistent<512, true>), dim3(drain_blocks), dim3(512), 0, q.st, q.L, s,
+                               kDrainEpoch, (const int*)q.act_cur, (const uint32_t*)(q.cnt + q.cur),
+                               q.cnt + C_DRAIN);
+            q.tm->end();
+            HIP_TRY(hipGetLastError());
+            q.done = true;
+          } else {
+            q.chunk = 1;
+          }
+        }
+        if (q.done) --live;
+        if (q.rounds >= (uint32_t)kMaxRounds) return fail(CC_ERR_DEVICE, "streamed run did not drain");
+      }
     }
-    if (rounds > out->rounds_max) out->rounds_max = rounds;
+    if (lanes > 1) {   // the group ends once both lanes have
+      HIP_TRY(hipEventRecord(c->lane_ev[1], c->stream2));
+      HIP_TRY(hipStreamWaitEvent(c->stream, c->lane_ev[1], 0));
+    }
+    for (int k = 0; k < lanes; ++k)
+      if (ln[k].rounds > out->rounds_max) out->rounds_max = ln[k].rounds;
     if (grid_on) {
       tm.begin(K_END);
       if (ncopy > 1)
@@ -3784,8 +3908,9 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
     HIP_TRY(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(ed.data(), c->st_edep, ed.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(&e, c->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));   // after the join: the second lane's work is done too
     tm.collect(out->kernel_ms, out->kernel_launches);
+    if (lanes > 1) tm2.collect(out->kernel_ms, out->kernel_launches);
     out->n_absorbed += st[S_ABS];
     out->n_escaped += st[S_ESC];
     out->scatter_sum += st[S_SCAT];
@@ -3806,6 +3931,7 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
   }
   out->n_histories = n_hist;
   out->error_bits = (uint32_t)err_bits;
+  add_busy(segs, out->kernel_busy_ms);
   if (err_bits) return fail(CC_ERR_DEVICE, "device error bits 0x%x", err_bits);
   if (grid_on) c->st_clean = true;   // every group's fold cleared its slots (nothing else writes them)
   return CC_OK;
@@ -3928,6 +4054,7 @@ int cc_run(void* ctx, const cc_source* src, uint64_t n_hist, uint64_t seed, uint
   }
   out->n_histories = n_hist;
   out->error_bits = (uint32_t)err_bits;
+  for (int k = 0; k < 8; ++k) out->kernel_busy_ms[k] = out->kernel_ms[k];   // one stream: no overlap
   if (err_bits) return fail(CC_ERR_DEVICE, "device error bits 0x%x", err_bits);
   return CC_OK;
 }

@@ -137,6 +137,10 @@ typedef struct {
   uint64_t n_split;             /* copies made by splitting */
   uint64_t n_roulette;          /* photons ended by Russian roulette */
   uint64_t n_records;           /* contribution records held after the run */
+  double kernel_busy_ms[8];     /* per kernel kind of kernel_ms: the time during which at least one
+                                   of its launches ran (the union of their intervals).  Streamed runs
+                                   of two lanes overlap launches of one kind, so this is below
+                                   kernel_ms; batch-by-batch runs: equal to kernel_ms */
 } cc_run_out;
 
 /* Learned importance map: f(x) = MLP 7 -> 64 -> 64 -> 1 (BF16 MFMA, FP32 accumulate).

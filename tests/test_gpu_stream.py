@@ -3,7 +3,8 @@ event-based rounds in one pool whose last photons one persistent launch carries 
 and one persistent history-based launch per group of batches) against the batch-by-batch loop
 (CC_RUN_PER_BATCH): the same histories on the same Philox streams, so tallies, deposits, batch
 moments and counters agree up to FP64 summation order (counters and hit counts exactly).  Small
-batches make several groups of batches in flight."""
+batches make several groups of batches in flight; runs of at least two pools (2 x 2^21 histories)
+run as two lanes on two streams, each over its own half of a group's histories."""
 import numpy as np
 import pytest
 
@@ -26,7 +27,7 @@ def _run(engine, raw, grid, n, batch, seed, form, first=0):
 
 
 @pytest.mark.parametrize("form", ["persistent", "rounds"])
-@pytest.mark.parametrize("case", ["slab_many_groups", "chest_room", "ct_helical_offset"])
+@pytest.mark.parametrize("case", ["slab_many_groups", "chest_room", "ct_helical_offset", "chest_room_two_lanes"])
 def test_streamed_equals_batch_by_batch(engine, case, form):
     from chatcarlo_amd import workloads as W
     from chatcarlo_amd.geometry import Geometry
@@ -34,12 +35,15 @@ def test_streamed_equals_batch_by_batch(engine, case, form):
     if case == "slab_many_groups":
         raw, grid = W.water_slab_60kev()
         n, batch, first = 150_000, 1000, 0          # 150 batches: groups of 64, 64, 22
-    elif case == "chest_room":
+    elif case.startswith("chest_room"):
         raw, _ = W.chest_room()
         geo = Geometry(raw["geometry"])
         vg = VoxelGrid.from_bbox(geo.bbox_min, geo.bbox_max, 5.0)
         grid = (vg.origin_cm, vg.shape, vg.voxel_size_cm)
-        n, batch, first = 1_000_003, 100_000, 0     # a short last batch
+        if case == "chest_room":
+            n, batch, first = 1_000_003, 100_000, 0     # a short last batch
+        else:   # two lanes; 72 batches in groups of 64 and 8, lanes split inside a batch
+            n, batch, first = 5_000_011, 70_000, 123_456
     else:
         raw, grid = W.ct_helical()
         n, batch, first = 300_000, 30_000, 7_777_777
@@ -55,7 +59,8 @@ def test_streamed_equals_batch_by_batch(engine, case, form):
     assert ga["n_batches"] == gb["n_batches"] and ga["n_histories"] == gb["n_histories"] == n
 
 
-def test_streamed_run_without_grid(engine):
+@pytest.mark.parametrize("n", [500_000, 4_500_001])
+def test_streamed_run_without_grid(engine, n):
     from chatcarlo_amd import _native as N
     from chatcarlo_amd import workloads as W
     from chatcarlo_amd.geometry import Geometry
@@ -63,9 +68,9 @@ def test_streamed_run_without_grid(engine):
     raw, _ = W.chest_room()
     engine.set_geometry(Geometry(raw["geometry"]))
     rec = SourceRecord(raw["source"])
-    b = engine.run(rec, 500_000, 9, 0, 50_000, N.RUN_FLUORESCENCE | N.RUN_PER_BATCH)
+    b = engine.run(rec, n, 9, 0, 50_000, N.RUN_FLUORESCENCE | N.RUN_PER_BATCH)
     for extra in (0, N.RUN_PERSISTENT):
-        a = engine.run(rec, 500_000, 9, 0, 50_000, N.RUN_FLUORESCENCE | extra)
+        a = engine.run(rec, n, 9, 0, 50_000, N.RUN_FLUORESCENCE | extra)
         assert (a.n_absorbed, a.n_escaped, a.scatter_sum, a.n_events) == (b.n_absorbed, b.n_escaped, b.scatter_sum,
                                                                           b.n_events)
         np.testing.assert_allclose(np.array(a.edep_keV[:]), np.array(b.edep_keV[:]), rtol=1e-9, atol=1e-12)
