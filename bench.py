@@ -345,7 +345,7 @@ def main():
     flights = sum(o.n_flights for o in outs)
     visits = sum(o.n_voxel_visits for o in outs)
     events = sum(o.n_events for o in outs)
-    # The run's two lanes overlap their launches: per kind, kernel_ms sums the launch durations and
+    # The run's lanes overlap their launches: per kind, kernel_ms sums the launch durations and
     # kernel_busy_ms is the time during which at least one launch of the kind ran.  The flight
     # kernel's rate is its algorithmic bytes over its busy time (over its summed durations, i.e.
     # per launch, when launches do not overlap).
@@ -384,7 +384,7 @@ def main():
                      "launch_overlap": f_ms / 1000.0 / f_busy_s,
                      "achieved_per_launch": f_bytes / max(f_launch, 1) / f_avg_s / 1e9,
                      "achieved_from": "algorithmic bytes of every k_flight launch over the time at least one "
-                                      "ran (launches of the run's two lanes overlap; achieved_per_launch divides "
+                                      "ran (launches of the run's lanes overlap; achieved_per_launch divides "
                                       "by the summed launch durations instead)"},
         "atomic_roofline": {"bound": "fp64 atomic requests (performed beyond the L2s)", "kernel": "k_flight",
                             "achieved": atom_achieved, "peak": atom_peak, "unit": "requests/s",

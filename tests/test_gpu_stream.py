@@ -4,7 +4,7 @@ and one persistent history-based launch per group of batches) against the batch-
 (CC_RUN_PER_BATCH): the same histories on the same Philox streams, so tallies, deposits, batch
 moments and counters agree up to FP64 summation order (counters and hit counts exactly).  Small
 batches make several groups of batches in flight; runs of at least two pools (2 x 2^21 histories)
-run as two lanes on two streams, each over its own half of a group's histories."""
+run as several lanes on their own streams, each over its own part of a group's histories."""
 import numpy as np
 import pytest
 
@@ -27,7 +27,7 @@ def _run(engine, raw, grid, n, batch, seed, form, first=0):
 
 
 @pytest.mark.parametrize("form", ["persistent", "rounds"])
-@pytest.mark.parametrize("case", ["slab_many_groups", "chest_room", "ct_helical_offset", "chest_room_two_lanes"])
+@pytest.mark.parametrize("case", ["slab_many_groups", "chest_room", "ct_helical_offset", "chest_room_lanes"])
 def test_streamed_equals_batch_by_batch(engine, case, form):
     from chatcarlo_amd import workloads as W
     from chatcarlo_amd.geometry import Geometry
@@ -42,7 +42,7 @@ def test_streamed_equals_batch_by_batch(engine, case, form):
         grid = (vg.origin_cm, vg.shape, vg.voxel_size_cm)
         if case == "chest_room":
             n, batch, first = 1_000_003, 100_000, 0     # a short last batch
-        else:   # two lanes; 72 batches in groups of 64 and 8, lanes split inside a batch
+        else:   # lanes; 72 batches in groups of 64 and 8, lanes split inside batches
             n, batch, first = 5_000_011, 70_000, 123_456
     else:
         raw, grid = W.ct_helical()
