@@ -3743,6 +3743,14 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
     if (!c->t_base) HIP_TRY(hipEventCreate(&c->t_base));
     HIP_TRY(hipEventRecord(c->t_base, c->stream));
   }
+  // every return, error paths included, leaves no lane's work running
+  struct LaneSync {
+    Ctx* c;
+    int lanes;
+    ~LaneSync() {
+      for (int k = 1; k < lanes; ++k) (void)hipStreamSynchronize(c->lane_st[k]);
+    }
+  } lane_sync{c, lanes};
   Timer tms[kMaxLanes];
   for (int k = 0; k < kMaxLanes; ++k)
     tms[k] = Timer{c, timing, {}, k ? c->lane_st[k] : nullptr, k ? &c->lane_tev[k] : nullptr,

@@ -74,3 +74,31 @@ def test_streamed_run_without_grid(engine, n):
         assert (a.n_absorbed, a.n_escaped, a.scatter_sum, a.n_events) == (b.n_absorbed, b.n_escaped, b.scatter_sum,
                                                                           b.n_events)
         np.testing.assert_allclose(np.array(a.edep_keV[:]), np.array(b.edep_keV[:]), rtol=1e-9, atol=1e-12)
+
+
+def test_kernel_timing_of_lanes(engine):
+    """With CC_RUN_TIMING the run reports per kernel kind the summed launch durations and the time
+    during which at least one launch of the kind ran: in lanes the launches overlap (busy time below
+    the summed durations, and below the run's wall time); one after the other (the batch-by-batch
+    loop) the two are the same."""
+    import time
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd import workloads as W
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.source import SourceRecord
+    raw, _ = W.chest_room()
+    engine.set_geometry(Geometry(raw["geometry"]))
+    rec = SourceRecord(raw["source"])
+    flags = N.RUN_FLUORESCENCE | N.RUN_TIMING
+    engine.run(rec, 4_500_000, 3, 0, 500_000, flags)
+    t0 = time.perf_counter()
+    a = engine.run(rec, 4_500_000, 3, 0, 500_000, flags)
+    wall_ms = (time.perf_counter() - t0) * 1e3
+    ms, busy, launches = list(a.kernel_ms), list(a.kernel_busy_ms), list(a.kernel_launches)
+    assert launches[1] > 0 and busy[1] > 0
+    for k in range(8):
+        assert busy[k] <= ms[k] * 1.001 + 0.01, (k, busy[k], ms[k])
+    assert busy[1] < 0.95 * ms[1]         # the lanes' flight launches overlap
+    assert busy[1] < wall_ms
+    b = engine.run(rec, 500_000, 3, 0, 500_000, flags | N.RUN_PER_BATCH)
+    assert list(b.kernel_busy_ms) == list(b.kernel_ms) and b.kernel_launches[1] > 0
