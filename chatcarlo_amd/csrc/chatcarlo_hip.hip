@@ -3847,34 +3847,36 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
       HIP_TRY(hipGetLastError());
       ln[0].done = true;
     }
-    // Rounds are issued in chunks per lane without a host wait; each lane's status is read
-    // once per chunk, while the other lane's chunk runs.
-    for (int live = persistent ? 0 : lanes; live > 0;) {
-      for (int k = 0; k < lanes; ++k) {
-        Lane& q = ln[k];
-        if (q.done) continue;
-        for (int j = 0; j < q.chunk; ++j) {
-          q.tm->mark(K_SOURCE);
-          hipLaunchKernelGGL(k_refill_stream, dim3(nbk), dim3(kBlock), 0, q.st, q.L, s, q.act_cur, q.cnt + q.cur,
-                             lpool);
-          hipLaunchKernelGGL(k_commit_and_reset, dim3(1), dim3(64), 0, q.st, q.L, lpool, q.nxt);
-          q.tm->mark(K_FLIGHT);
-          hipLaunchKernelGGL((k_flight<false, false, false, true>), dim3(nbk), dim3(kBlock), 0, q.st, q.L,
-                             (const int*)q.act_cur, (const uint32_t*)(q.cnt + q.cur), q.qp, q.qc, q.qr,
-                             (int*)nullptr);
-          q.tm->mark(K_INTERACT);
-          hipLaunchKernelGGL(k_interact, dim3(nbk), dim3(kBlock), 0, q.st, q.L, (const int*)q.qp, (const int*)q.qc,
-                             (const int*)q.qr, q.act_nxt, q.cnt + q.nxt);
-          std::swap(q.cur, q.nxt);
-          std::swap(q.act_cur, q.act_nxt);
-        }
-        q.tm->mark(-1);
-        q.rounds += q.chunk;
-        hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(64), 0, q.st, q.L, (const uint32_t*)(q.cnt + q.cur),
-                           q.status);
-        HIP_TRY(hipMemcpyAsync(q.h_status, q.status, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, q.st));
-        HIP_TRY(hipGetLastError());
+    // Rounds are issued in chunks per lane without a host wait.  The host takes the lanes in
+    // turn: it waits for a lane's chunk, reads its status and issues its next chunk (or its
+    // drain) at once, while the other lanes' chunks run, so no lane waits for another.
+    auto issue_chunk = [&](Lane& q) -> int {
+      for (int j = 0; j < q.chunk; ++j) {
+        q.tm->mark(K_SOURCE);
+        hipLaunchKernelGGL(k_refill_stream, dim3(nbk), dim3(kBlock), 0, q.st, q.L, s, q.act_cur, q.cnt + q.cur,
+                           lpool);
+        hipLaunchKernelGGL(k_commit_and_reset, dim3(1), dim3(64), 0, q.st, q.L, lpool, q.nxt);
+        q.tm->mark(K_FLIGHT);
+        hipLaunchKernelGGL((k_flight<false, false, false, true>), dim3(nbk), dim3(kBlock), 0, q.st, q.L,
+                           (const int*)q.act_cur, (const uint32_t*)(q.cnt + q.cur), q.qp, q.qc, q.qr,
+                           (int*)nullptr);
+        q.tm->mark(K_INTERACT);
+        hipLaunchKernelGGL(k_interact, dim3(nbk), dim3(kBlock), 0, q.st, q.L, (const int*)q.qp, (const int*)q.qc,
+                           (const int*)q.qr, q.act_nxt, q.cnt + q.nxt);
+        std::swap(q.cur, q.nxt);
+        std::swap(q.act_cur, q.act_nxt);
       }
+      q.tm->mark(-1);
+      q.rounds += q.chunk;
+      hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(64), 0, q.st, q.L, (const uint32_t*)(q.cnt + q.cur),
+                         q.status);
+      HIP_TRY(hipMemcpyAsync(q.h_status, q.status, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, q.st));
+      HIP_TRY(hipGetLastError());
+      return CC_OK;
+    };
+    for (int k = 0; k < lanes && !persistent; ++k)
+      if ((r = issue_chunk(ln[k]))) return r;
+    for (int live = persistent ? 0 : lanes; live > 0;) {
       for (int k = 0; k < lanes; ++k) {
         Lane& q = ln[k];
         if (q.done) continue;
@@ -3882,22 +3884,22 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
         q.tm->collect(out->kernel_ms, out->kernel_launches);
         if (q.h_status[0] >= q.h_end && q.h_status[1] == 0) {
           q.done = true;
-        } else if (q.h_status[0] >= q.h_end) {
-          if (q.h_status[1] <= drain_below) {   // the last photons in flight, to their ends in one launch
-            hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, q.st, (int*)(q.cnt + C_DRAIN), (size_t)1, 0);
-            q.tm->begin(K_OTHER);
-            hipLaunchKernelGGL((k_persistent<512, true>), dim3(drain_blocks), dim3(512), 0, q.st, q.L, s,
-                               kDrainEpoch, (const int*)q.act_cur, (const uint32_t*)(q.cnt + q.cur),
-                               q.cnt + C_DRAIN);
-            q.tm->end();
-            HIP_TRY(hipGetLastError());
-            q.done = true;
-          } else {
-            q.chunk = 1;
-          }
+        } else if (q.h_status[0] >= q.h_end && q.h_status[1] <= drain_below) {
+          // the last photons in flight, to their ends in one launch
+          hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, q.st, (int*)(q.cnt + C_DRAIN), (size_t)1, 0);
+          q.tm->begin(K_OTHER);
+          hipLaunchKernelGGL((k_persistent<512, true>), dim3(drain_blocks), dim3(512), 0, q.st, q.L, s,
+                             kDrainEpoch, (const int*)q.act_cur, (const uint32_t*)(q.cnt + q.cur),
+                             q.cnt + C_DRAIN);
+          q.tm->end();
+          HIP_TRY(hipGetLastError());
+          q.done = true;
+        } else {
+          if (q.rounds >= (uint32_t)kMaxRounds) return fail(CC_ERR_DEVICE, "streamed run did not drain");
+          if (q.h_status[0] >= q.h_end) q.chunk = 1;   // every history started: one round at a time
+          if ((r = issue_chunk(q))) return r;
         }
         if (q.done) --live;
-        if (q.rounds >= (uint32_t)kMaxRounds) return fail(CC_ERR_DEVICE, "streamed run did not drain");
       }
     }
     for (int k = 1; k < lanes; ++k) {   // the group ends once every lane has
