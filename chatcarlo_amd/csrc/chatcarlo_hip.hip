@@ -3678,12 +3678,17 @@ int cc_load_grid(void* ctx, const double* kerma, const double* h10, const double
   return CC_OK;
 }
 
-constexpr uint32_t kStreamPool = 1u << 21;      // photons in flight in a streamed run
+// Photons in flight in a streamed run, over all its lanes.  In three lanes, pools of 2^21 /
+// 1.5 x 2^20 / 1.25 x 2^20 / 2^20 / 0.75 x 2^20 photons ran the chest room at 471 / 482 / 485 /
+// 492 / 478 M histories/s (one lane: 2^21 was best).
+constexpr uint32_t kStreamPool = 1u << 20;
 constexpr int kStreamSlots = 64;                 // batches in flight at most (one group)
 constexpr size_t kStreamTallyBytes = 8ull << 30; // cap on the group's tally slots (all copies)
 // The drain: once every history of a group has started, the rounds go on until at most
 // kDrainPerLane photons per resident lane of the drain kernel are left in flight (the status is
 // read after every round from then on), and one persistent launch carries those to their ends.
+// (With three lanes of a third of 2^20 photons each, a lane's drain starts as soon as its last
+// history has started; thresholds of 1 and 2 photons per lane were no faster.)
 constexpr int kDrainPerLane = 4;
 constexpr int kDrainEpoch = 32;    // wave iterations between tally-cache flushes of the drain
 // CC_RUN_PERSISTENT: each group of batches in one persistent launch of 256-thread blocks, flushing

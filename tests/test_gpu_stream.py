@@ -3,7 +3,7 @@ event-based rounds in one pool whose last photons one persistent launch carries 
 and one persistent history-based launch per group of batches) against the batch-by-batch loop
 (CC_RUN_PER_BATCH): the same histories on the same Philox streams, so tallies, deposits, batch
 moments and counters agree up to FP64 summation order (counters and hit counts exactly).  Small
-batches make several groups of batches in flight; runs of at least two pools (2 x 2^21 histories)
+batches make several groups of batches in flight; runs of at least two pools (2 x 2^20 histories)
 run as several lanes on their own streams, each over its own part of a group's histories."""
 import numpy as np
 import pytest
