@@ -1215,7 +1215,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
 //  - batched scenes (kStream = false): histories of the batched index space, drawn as k_refill
 //    draws them, on their scenes' primitives (global memory) and grids;
 //  - a streamed run (kStream = true): histories of the run's group of batches, drawn as
-//    k_refill_stream draws them, on the scene's primitives staged in LDS, each tallied into the
+//    k_refill_stream draws them, on the scene's primitives (scalar loads), each tallied into the
 //    slot of its batch.  Photons already in flight come first: the entries of an active list
 //    (cnt_act of them; a wave claims entries for its idle lanes through `claim`), so that the
 //    launch can also carry the last photons of event-based rounds to their ends.
@@ -1237,8 +1237,7 @@ constexpr int kMaxFlights = 1 << 12;      // flights of one history (CC_DEV_ROUN
 template <int kThreads, bool kStream>
 struct PersistShape {
   static_assert(kThreads == 256 || kThreads == 512, "blocks of 256 or 512 threads");
-  // the drain stages the scene's primitives in LDS as well
-  static constexpr int kSlots = 4000 * (kThreads / 256) - (kStream ? 160 * (kThreads / 256) : 0);
+  static constexpr int kSlots = 4000 * (kThreads / 256);
 };
 enum { PH_IDLE = 0, PH_FLY = 1, PH_HIT = 2 };
 

@@ -381,8 +381,10 @@ def run_transport(scene, n_histories: int = 100_000, seed: int | None = None,
         rel = moments.relative_errors()
         n_batches = moments.n_batches
     else:
+        # the reference's count when statistics are off: the grid's batch count, which end_batch
+        # leaves at 0 then (chatcarlo/transport.py:455-459, tally.py:59-64)
         sem, rel = {}, {}
-        n_batches = int(out.n_batches)
+        n_batches = grid.n_batches if grid is not None else 0
     stats = dict(n_events=int(out.n_events), n_flights=int(out.n_flights),
                  n_voxel_visits=int(out.n_voxel_visits), rounds_max=int(out.rounds_max),
                  kernel_ms=list(out.kernel_ms), kernel_launches=list(out.kernel_launches))
