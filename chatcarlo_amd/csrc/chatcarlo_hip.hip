@@ -2084,30 +2084,49 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(const float* w, const uint1
                                                      uint64_t pa, uint64_t pb, uint32_t k0, MlpFrame frame,
                                                      float yscale, int power, float gscale, float* part,
                                                      double* lpart) {
-  __shared__ __align__(16) uint16_t sW2[kMlpH * kMlpH];   // W2 [i][j]
-  __shared__ float sbw[2 * kMlpH];                        // b2, w3
+  __shared__ __align__(16) uint16_t sW2[kMlpH * kMlpH];    // W2 [i][j]
+  __shared__ __align__(16) uint16_t sW2T[kMlpH * kMlpH];   // W2^T [j][i]
+  __shared__ __align__(16) uint16_t sW1T[kMlpH * kMlpIn];  // W1^T [j][i]
+  __shared__ float sbw[2 * kMlpH];                         // b2, w3
   __shared__ __align__(16) GradWaveLds swl[kBlock / kWave];
   __shared__ float sGrad[kMlpParams];
   __shared__ double sL[kBlock / kWave];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = lane & 15, g = lane >> 4;
   for (int k = threadIdx.x; k < kMlpParams; k += blockDim.x) sGrad[k] = 0.f;
-  for (int k = threadIdx.x; k < kMlpH * kMlpH; k += blockDim.x) sW2[k] = wb[kOffW2 + k];
+  // the bf16 weights in 16-byte loads, into LDS as W2 and transposed as W2^T and W1^T, so that
+  // the forward operands below are one 8-byte LDS read each (64-bit loads of four consecutive
+  // bf16) instead of one 2-byte global load per element
+  static_assert((kMlpIn * kMlpH) % 8 == 0 && (kMlpH * kMlpH) % 8 == 0 && kOffW2 % 8 == 0, "weight blocks of 8");
+  for (int q = threadIdx.x; q < (kMlpIn * kMlpH + kMlpH * kMlpH) / 8; q += blockDim.x) {
+    const uint4 v = reinterpret_cast<const uint4*>(wb)[q];
+    const uint16_t h[8] = {(uint16_t)(v.x & 0xFFFFu), (uint16_t)(v.x >> 16), (uint16_t)(v.y & 0xFFFFu),
+                           (uint16_t)(v.y >> 16), (uint16_t)(v.z & 0xFFFFu), (uint16_t)(v.z >> 16),
+                           (uint16_t)(v.w & 0xFFFFu), (uint16_t)(v.w >> 16)};
+    const int k0 = 8 * q;
+    if (k0 < kOffW2) {   // W1 [i][j], i < kMlpIn
+      const int i = k0 / kMlpH, j0 = k0 % kMlpH;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) sW1T[(j0 + t) * kMlpIn + i] = h[t];
+    } else {             // W2 [i][j]
+      const int k = k0 - kOffW2, i = k / kMlpH, j0 = k % kMlpH;
+      reinterpret_cast<uint4*>(sW2)[k / 8] = v;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) sW2T[(j0 + t) * kMlpH + i] = h[t];
+    }
+  }
   for (int k = threadIdx.x; k < 2 * kMlpH; k += blockDim.x) sbw[k] = w[kOffB2 + k];
+  const float b3 = w[kOffB3];
+  __syncthreads();
   // forward weights in registers (mlp_wave_load's layout): a1[tm][e] = W1[4g + e][16 tm + c],
   // a2[tm][kb][e] = W2[16 kb + 4g + e][16 tm + c]
   bf16x4 a1[4], a2[4][4];
 #pragma unroll
   for (int tm = 0; tm < 4; ++tm) {
+    a1[tm] = ld_bf16x4(&sW1T[(16 * tm + c) * kMlpIn + 4 * g]);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      a1[tm][e] = (short)wb[(4 * g + e) * kMlpH + 16 * tm + c];
-#pragma unroll
-      for (int kb = 0; kb < 4; ++kb) a2[tm][kb][e] = (short)wb[kOffW2 + (16 * kb + 4 * g + e) * kMlpH + 16 * tm + c];
-    }
+    for (int kb = 0; kb < 4; ++kb) a2[tm][kb] = ld_bf16x4(&sW2T[(16 * tm + c) * kMlpH + 16 * kb + 4 * g]);
   }
-  const float b3 = w[kOffB3];
-  __syncthreads();
   GradWaveLds& L = swl[wave];
   // gradient accumulators: acc2[ti][tj][e] = dW2[16 ti + 4g + e][16 tj + c], acc1[ti][e] =
   // dW1[4g + e][16 ti + c]; dw3, db2 of hidden unit 16 tm + 4g + e over this lane's column rows
