@@ -1775,17 +1775,21 @@ constexpr int kSplitBlocks = 4096;  // grid of the collision kernel (its staging
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_scatter_split(Launch L, Importance I, const int* in,
                                                           const uint32_t* cnt_in, int* act_next, uint32_t* cnt_next) {
   __shared__ float sbw[2 * kMlpH];
+  // the block's pool reservation and allocations: one set of counter atomics per block, not per
+  // wave (every wave of every block on the same three counters serialised the first rounds)
+  __shared__ uint32_t s_worst[kBlock / kWave], s_staged[kBlock / kWave], s_next[kBlock / kWave];
+  __shared__ uint32_t s_room, s_pool0, s_next0;
   MlpWave M;
   mlp_wave_load(I.wb, I.w, M, sbw);
   const uint32_t n_in = *cnt_in;
   PState S = L.S;
   const Tables& T = L.T;
-  const int lane = __lane_id();
-  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = __lane_id(), wave = threadIdx.x / kWave;
   const float kTwoPiF = 6.283185307179586f;
   unsigned long long n_abs = 0, n_fluor = 0, n_scat = 0, n_roul = 0, n_split = 0;
-  for (uint32_t base = (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; base < n_in;
-       base += waves * kWave) {
+  // block-uniform trip count: every wave of a block reaches the block's barriers
+  for (uint32_t bbase = blockIdx.x * blockDim.x; bbase < n_in; bbase += gridDim.x * blockDim.x) {
+    const uint32_t base = bbase + (uint32_t)wave * kWave;
     const uint32_t j = base + lane;
     const bool valid = j < n_in;
     int i = 0, m = 0, nsc = 0;
@@ -1821,14 +1825,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     mlp_features(I.frame, p, u, E, x);
     const float fr = mlp_wave_eval(M, x);
     const uint32_t worst = 2u * (uint32_t)I.split_max * (uint32_t)__popcll(__ballot(valid));
-    int room = 0;
-    if (lane == 0) {
-      const uint32_t r_all = atomicAdd(L.cnt + C_RESV, worst) + worst;
+    if (lane == 0) s_worst[wave] = worst;
+    __syncthreads();
+    if (threadIdx.x == 0) {   // the block's worst case reserved at once, or no splitting in the block
+      uint32_t wb = 0;
+      for (int k = 0; k < (int)(blockDim.x / kWave); ++k) wb += s_worst[k];
+      const uint32_t r_all = atomicAdd(L.cnt + C_RESV, wb) + wb;
       const uint32_t used = atomicAdd(L.cnt + C_POOL, 0u);
-      room = (uint64_t)used + r_all <= I.pool_cap;
-      if (!room) atomicSub(L.cnt + C_RESV, worst);
+      const bool ok = (uint64_t)used + r_all <= I.pool_cap;
+      if (!ok) atomicSub(L.cnt + C_RESV, wb);
+      s_room = ok ? wb : 0u;   // the reservation to give back at the end (0: none held)
     }
-    room = __shfl(room, 0);
+    __syncthreads();
+    const int room = s_room != 0u ? 1 : 0;
     int ncp = 1;
     if (valid && room) {
       const float want = I.split_gain * w * expf(fminf(fmaxf(fr, -80.f), 80.f)) / I.cw;
@@ -1861,7 +1870,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
       }
       // azimuth: one uniform draw, or the two strata of the arc towards the region
       int nout = out_ok ? 1 : 0;
-      float phi[2] = {0.f, 0.f}, share[2] = {1.f, 0.f};
+      // the two outgoing azimuths and their shares as scalars (picked by selects below: an array
+      // indexed by the loop's run-time counter would live in scratch memory)
+      float phi_a = 0.f, phi_b = 0.f, share_a = 1.f, share_b = 0.f;
       if (out_ok) {
         const float uphi = rng.next();
         float half = -1.f;   // half-width of the arc, when it is neither empty nor full
@@ -1871,13 +1882,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         }
         if (half > 0.f) {
           const float frac = half / 3.14159265f;
-          phi[0] = phi0 + (2.f * uphi - 1.f) * half;
-          phi[1] = phi0 + half + rng.next() * (kTwoPiF - 2.f * half);
-          share[0] = frac;
-          share[1] = 1.f - frac;
+          phi_a = phi0 + (2.f * uphi - 1.f) * half;
+          phi_b = phi0 + half + rng.next() * (kTwoPiF - 2.f * half);
+          share_a = frac;
+          share_b = 1.f - frac;
           nout = 2;
         } else {
-          phi[0] = kTwoPiF * uphi;
+          phi_a = kTwoPiF * uphi;
         }
       }
       const uint32_t ctr_after = rng.ctr;
@@ -1885,12 +1896,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
       for (int o = 0; o < n_o; ++o) {
         const bool has = o < nout;
         float dout[3] = {0.f, 0.f, 1.f};
-        if (has) dir_at(d, e1, e2, ct, st, phi[o], dout);
+        if (has) dir_at(d, e1, e2, ct, st, o == 0 ? phi_a : phi_b, dout);
         if (has) mlp_features(I.frame, p, dout, eo, x);
         else for (int k = 0; k < kMlpIn; ++k) x[k] = 0.f;
         const float fo = mlp_wave_eval(M, x);
         bool alive = has;
-        float wo = wc * share[o], tau = 0.f;
+        float wo = wc * (o == 0 ? share_a : share_b), tau = 0.f;
         const uint32_t tgo = o == 0 ? tg : mix32(tg, ctr_after, 0x51u);
         PhiloxRng ro(L.seed, L.hist0 + (uint64_t)hid, tgo, o == 0 ? ctr_after : 0u);
         if (has) {
@@ -1931,17 +1942,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
       }
     }
     wave_add_by_key(valid, m, dep_acc, dep_copy(L));
-    // move the staged photons to pool slots; append parents and new photons to the next list
+    // move the staged photons to pool slots; append parents and new photons to the next list:
+    // the block's slots and list entries in one atomic each, dealt to its waves in wave order
     uint32_t npar = 0;
     const uint32_t poff = wave_excl_scan(parent_alive ? 1u : 0u, npar);
-    uint32_t s0 = 0, a0 = 0;
     if (lane == 0) {
-      if (staged > 0) s0 = atomicAdd(L.cnt + C_POOL, staged);
-      if (room) atomicSub(L.cnt + C_RESV, worst);
-      if (staged + npar > 0) a0 = atomicAdd(cnt_next, staged + npar);
+      s_staged[wave] = staged;
+      s_next[wave] = staged + npar;
     }
-    s0 = __shfl(s0, 0);
-    a0 = __shfl(a0, 0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t ts = 0, tn = 0;
+      for (int k = 0; k < (int)(blockDim.x / kWave); ++k) {
+        ts += s_staged[k];
+        tn += s_next[k];
+      }
+      s_pool0 = ts > 0 ? atomicAdd(L.cnt + C_POOL, ts) : 0u;
+      if (s_room) atomicSub(L.cnt + C_RESV, s_room);
+      s_next0 = tn > 0 ? atomicAdd(cnt_next, tn) : 0u;
+    }
+    __syncthreads();
+    uint32_t s0 = s_pool0, a0 = s_next0;
+    for (int k = 0; k < wave; ++k) {
+      s0 += s_staged[k];
+      a0 += s_next[k];
+    }
     if (parent_alive) act_next[a0 + poff] = i;
     for (uint32_t k = lane; k < staged; k += kWave) {
       const uint32_t ns = I.stage_slots;
