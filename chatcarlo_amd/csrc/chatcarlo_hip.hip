@@ -17,7 +17,9 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <condition_variable>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -2756,6 +2758,67 @@ int grid_blocks(size_t n) {
   return (int)b;
 }
 
+// Host threads that copy one buffer into another in slices, for the grid reads: the caller's
+// arrays are usually fresh (their pages are first touched by the copy), and page faults and
+// copies of several threads overlap.  Created on first use, joined when the context goes.
+struct CopyPool {
+  std::vector<std::thread> th;
+  std::mutex mu;
+  std::condition_variable go, done;
+  unsigned char* dst = nullptr;
+  const unsigned char* src = nullptr;
+  size_t len = 0;
+  uint64_t gen = 0;
+  int pending = 0;
+  bool stop = false;
+  void start(int n) {
+    for (int k = 0; k < n; ++k)
+      th.emplace_back([this, k, n] {
+        uint64_t seen = 0;
+        for (;;) {
+          unsigned char* d;
+          const unsigned char* sp;
+          size_t l;
+          {
+            std::unique_lock<std::mutex> lk(mu);
+            go.wait(lk, [&] { return stop || gen != seen; });
+            if (stop) return;
+            seen = gen;
+            d = dst;
+            sp = src;
+            l = len;
+          }
+          const size_t a = l * (size_t)k / (size_t)n, b = l * (size_t)(k + 1) / (size_t)n;
+          if (b > a) std::memcpy(d + a, sp + a, b - a);
+          std::lock_guard<std::mutex> lk(mu);
+          if (--pending == 0) done.notify_one();
+        }
+      });
+  }
+  void copy(void* d, const void* sp, size_t l) {
+    if (th.empty() || l < (1u << 20)) {
+      std::memcpy(d, sp, l);
+      return;
+    }
+    std::unique_lock<std::mutex> lk(mu);
+    dst = (unsigned char*)d;
+    src = (const unsigned char*)sp;
+    len = l;
+    pending = (int)th.size();
+    ++gen;
+    go.notify_all();
+    done.wait(lk, [&] { return pending == 0; });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    go.notify_all();
+    for (auto& t : th) t.join();
+  }
+};
+
 constexpr int kMaxLanes = 4;   // lanes a streamed run can be split into
 struct Ctx {
   int device = 0;
@@ -2855,17 +2918,31 @@ struct Ctx {
   // pinned staging of grid reads: two chunks, the DMA of one overlapping the copy of the other
   unsigned char* h_stage = nullptr;
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  CopyPool copier;
 };
 
 constexpr size_t kStageChunk = 8u << 20;
 
 // Device -> caller (pageable) host memory through the pinned staging chunks.
-int stage_to_host(Ctx* c, void* dst, const void* src, size_t bytes) {
-  if (bytes == 0) return CC_OK;
+// Pinned staging and copy threads of the grid reads, set up when a grid is bound so that the
+// reads themselves do not pay for them.
+constexpr int kCopyThreads = 8;
+int ensure_stage(Ctx* c) {
   if (!c->h_stage) {
     HIP_TRY(hipHostMalloc((void**)&c->h_stage, 2 * kStageChunk, hipHostMallocDefault));
     for (auto& e : c->stage_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
+  if (c->copier.th.empty()) {
+    const int hw = (int)std::thread::hardware_concurrency();
+    c->copier.start(std::max(1, std::min(kCopyThreads, hw / 2)));
+  }
+  return CC_OK;
+}
+
+int stage_to_host(Ctx* c, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return CC_OK;
+  int rs = ensure_stage(c);
+  if (rs) return rs;
   const size_t nchunk = (bytes + kStageChunk - 1) / kStageChunk;
   auto issue = [&](size_t i) -> int {
     const size_t off = i * kStageChunk, len = std::min(kStageChunk, bytes - off);
@@ -2881,7 +2958,7 @@ int stage_to_host(Ctx* c, void* dst, const void* src, size_t bytes) {
     if (i + 1 < nchunk && (r = issue(i + 1))) return r;
     HIP_TRY(hipEventSynchronize(c->stage_ev[i & 1]));
     const size_t off = i * kStageChunk, len = std::min(kStageChunk, bytes - off);
-    std::memcpy((unsigned char*)dst + off, c->h_stage + (i & 1) * kStageChunk, len);
+    c->copier.copy((unsigned char*)dst + off, c->h_stage + (i & 1) * kStageChunk, len);
   }
   return CC_OK;
 }
@@ -3560,6 +3637,8 @@ int cc_bind_grid(void* ctx, const double* origin, const int32_t* shape, double v
   c->G.inv_h = (float)(1.0 / voxel_cm);
   c->track = (flags & 1) != 0;
   c->have_grid = true;
+  int rs = ensure_stage(c);   // the grid's reads will need the staging
+  if (rs) return rs;
   return cc_reset_grid(ctx);
 }
 
