@@ -212,6 +212,23 @@ class Engine:
                      hits=hits.reshape(shape))
         return r
 
+    def load_grid(self, saved: dict):
+        """Adds a grid read by read_grid (totals, and batch moments and hit counts when present)
+        into the bound grid, with its batch and history counts: the resume step of a run split
+        over history ranges, and the add-into-totals step of a merge (cc_load_grid)."""
+        def arr(key, dtype, ctype):
+            a = saved.get(key)
+            if a is None:
+                return C.POINTER(ctype)()
+            a = np.ascontiguousarray(a, dtype=dtype).reshape(-1)
+            keep.append(a)
+            return N.ptr(a, ctype)
+        keep = []
+        N.check(self.lib.cc_load_grid(
+            self.ctx, arr("kerma", np.float64, C.c_double), arr("h10", np.float64, C.c_double),
+            arr("kerma_sum2", np.float64, C.c_double), arr("h10_sum2", np.float64, C.c_double),
+            arr("hits", np.int32, C.c_int32), int(saved.get("n_batches", 0)), int(saved.get("n_histories", 0))))
+
     # -- transport ---------------------------------------------------------------
     def run(self, src: SourceRecord, n, seed, first=0, batch=200_000, flags=0) -> N.CcRunOut:
         out = N.CcRunOut()
