@@ -458,3 +458,25 @@ def test_dose_maximum_inside_an_all_water_grid_is_not_flagged(engine):
     p = max_voxel_position_cm(grid, dose_map_Gy(grid, g))
     material = str(g.material_at(p[None, :])[0])
     assert material == "water" and background_medium_warning(material, g.background) is None
+
+
+def test_out_of_table_energy_fails_the_run_loudly(engine):
+    """The reference raises ValueError for energies outside its tables (chatcarlo/materials.py:234-237);
+    on the GPU a photon above 150 keV sets the energy-range bit of the launch's error word, the photon
+    is stopped, and the call fails with the device error (no silent result)."""
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.transport import transport_photons
+    g = Geometry([{"name": "slab", "shape": "box", "material": "water", "center": [0, 0, 0],
+                   "size_cm": [10.0, 10.0, 10.0]}])
+    pos = np.tile([-20.0, 0.0, 0.0], (64, 1))
+    dirv = np.tile([1.0, 0.0, 0.0], (64, 1))
+    e = np.full(64, 60.0)
+    e[5] = 200.0
+    with pytest.raises((N.NativeError, ValueError), match="0x1|energy"):
+        transport_photons(pos, dirv, e, g, np.random.default_rng(3))
+    # the context stays usable: the same call without the bad photon runs
+    e[5] = 60.0
+    r = transport_photons(np.tile([-20.0, 0.0, 0.0], (64, 1)), np.tile([1.0, 0.0, 0.0], (64, 1)), e, g,
+                          np.random.default_rng(3))
+    assert int(r.absorbed.sum() + r.escaped.sum()) == 64
