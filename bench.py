@@ -112,6 +112,118 @@ def cpu_baseline(raw, all_cores):
     return out
 
 
+def _config_grid(raw, grid, voxel_cm=5.0):
+    """The workload's own grid, or the reference's default (5 cm over the world box)."""
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.tally import VoxelGrid
+    if grid is not None:
+        return grid
+    geo = Geometry(raw["geometry"])
+    vg = VoxelGrid.from_bbox(geo.bbox_min, geo.bbox_max, voxel_cm)
+    return vg.origin_cm, vg.shape, vg.voxel_size_cm
+
+
+def _oracle_counts(raw, grid, n, seed, exact):
+    """The oracle's transport of n histories on the workload's grid: (segments, voxel pieces,
+    seconds).  exact: exact-overlap tally (voxel pieces = voxel crossings); else the reference's
+    stratified sub-step tally (its CPU speed)."""
+    from chatcarlo_amd.datapack import build_pack
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.spectrum import spectrum_table
+    from oracle import core
+    from oracle import transport as OT
+    pack = build_pack()
+    geo = Geometry(raw["geometry"])
+    prims, world = geo.packed(pack.mat_id)
+    og = core.OGeom.from_packed(prims, world)
+    g = OT.Grid(grid[0], grid[1], grid[2])
+    e, cdf = spectrum_table(raw["source"])
+    fluor = raw.get("physics", {}).get("fluorescence", True)
+    t0 = time.perf_counter()
+    r = OT.run(raw["source"], og, pack, e, cdf, n, seed, batch_size=REF_BATCH, grid=g, fluorescence=fluor,
+               exact_tally=exact)
+    return r["n_segments"], r["n_voxel_pieces"], time.perf_counter() - t0
+
+
+def _config_cpu_worker(args):
+    raw, grid, n, seed = args
+    _, _, dt = _oracle_counts(raw, grid, n, seed, False)
+    return n, dt
+
+
+def _gpu_counts(eng, raw, grid, n, timed):
+    """The GPU run of n histories on the workload's grid (batches of 10^6, resident grid):
+    (events per history, voxel crossings per history, histories/s or None)."""
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.source import SourceRecord
+    eng.set_geometry(Geometry(raw["geometry"]))
+    flags = N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY | (
+        N.RUN_FLUORESCENCE if raw.get("physics", {}).get("fluorescence", True) else 0)
+    rec = SourceRecord(raw["source"])
+    if timed:
+        eng.bind_grid(grid[0], grid[1], grid[2], True)
+        eng.run(rec, n // 10, 1, 0, 1_000_000, flags)
+    eng.bind_grid(grid[0], grid[1], grid[2], True)
+    t0 = time.perf_counter()
+    o = eng.run(rec, n, 2, 0, 1_000_000, flags)
+    dt = time.perf_counter() - t0
+    return o.n_events / n, o.n_voxel_visits / n, (n / dt if timed else None)
+
+
+def configs_table(eng, n_oracle=200_000, n_gpu=10_000_000, n_scene_oracle=10_000, n_scene_gpu=1_000_000):
+    """Per configuration: transport events (flight segments) and voxel crossings per history, from
+    the oracle (n_oracle histories, exact-overlap tally) and from the GPU (n_gpu histories); the
+    algorithmic bytes per history B_hist = 32 N_vox + 128 N_evt + 64 (from the GPU counts);
+    histories/s of the oracle's reference-style transport (stratified tally, one batch of the
+    reference's default 200,000 histories) in one process and in one process per host core (up to
+    16, one such batch each, rate = all histories / slowest process), and of the GPU (resident
+    grid, batches of 10^6).  The batched configuration is given as three of its 64 scenes (the most
+    primitives, the most lead, the highest kVp, each on its own 2 cm grid) with rates, and as the
+    history-weighted mean counts over all 64 scenes (n_scene_oracle / n_scene_gpu histories each)."""
+    import multiprocessing as mp
+    from chatcarlo_amd import workloads as W
+    cases = [("1_water_slab_60keV_pencil", *W.water_slab_60kev()), ("2_chest_room", *W.chest_room()),
+             ("3_ct_helical", *W.ct_helical())]
+    sc = W.batched_scenes(64)
+    picks = {"most_primitives": max(range(64), key=lambda k: len(sc[k][0]["geometry"])),
+             "most_lead": max(range(64), key=lambda k: sum(g["material"] == "lead" for g in sc[k][0]["geometry"])),
+             "highest_kvp": max(range(64), key=lambda k: max(b["energy_keV"] for b in sc[k][0]["source"]["spectrum"]))}
+    for name, k in picks.items():
+        cases.append((f"4_batched_scene_{k}_{name}", sc[k][0], sc[k][1]))
+    cases.append(("5_angio_room_analog", *W.angio_room()))
+    nproc = max(1, min(os.cpu_count() or 1, 16))
+    out = {}
+    with mp.get_context("spawn").Pool(nproc) as pool:
+        for name, raw, grid in cases:
+            grid = _config_grid(raw, grid)
+            seg, vox, _ = _oracle_counts(raw, grid, n_oracle, 5, True)
+            _, _, dt = _oracle_counts(raw, grid, REF_BATCH, 6, False)
+            res = pool.map(_config_cpu_worker, [(raw, grid, REF_BATCH, 100 + i) for i in range(nproc)])
+            n_evt, n_vox, rate = _gpu_counts(eng, raw, grid, n_gpu, True)
+            out[name] = {"grid": [int(x) for x in grid[1]], "voxel_cm": float(grid[2]),
+                         "n_evt_oracle": seg / n_oracle, "n_vox_oracle": vox / n_oracle,
+                         "n_evt_gpu": n_evt, "n_vox_gpu": n_vox,
+                         "bytes_per_history": VOXEL_B * n_vox + EVENT_B * n_evt + SOURCE_B,
+                         "cpu_histories_per_s_1proc": REF_BATCH / dt,
+                         "cpu_histories_per_s_all_cores": sum(r[0] for r in res) / max(r[1] for r in res),
+                         "gpu_histories_per_s": rate}
+            print(f"configs-table: {name} done", file=sys.stderr, flush=True)
+    acc = np.zeros(5)
+    for k, (raw, grid, n_hist, _) in enumerate(sc):
+        seg, vox, _ = _oracle_counts(raw, grid, n_scene_oracle, 5, True)
+        n_evt, n_vox, _ = _gpu_counts(eng, raw, grid, n_scene_gpu, False)
+        acc += n_hist * np.array([1.0, seg / n_scene_oracle, vox / n_scene_oracle, n_evt, n_vox])
+        if k % 16 == 15:
+            print(f"configs-table: batched scenes {k + 1} of 64", file=sys.stderr, flush=True)
+    m = acc[1:] / acc[0]
+    out["4_batched_all_64_scenes_history_weighted"] = {
+        "histories": int(acc[0]), "n_evt_oracle": m[0], "n_vox_oracle": m[1], "n_evt_gpu": m[2], "n_vox_gpu": m[3],
+        "bytes_per_history": VOXEL_B * m[3] + EVENT_B * m[2] + SOURCE_B}
+    return {"oracle_histories": n_oracle, "cpu_timed_histories": REF_BATCH, "cpu_processes_all_cores": nproc,
+            "gpu_histories": n_gpu, "configs": out}
+
+
 def ct_helical_run(eng, n_histories=100_000_000):
     """The CT helical workload at its own size: 1e8 histories of the 120 kVp helical scan on the
     CTDI phantom, kerma and H*(10) on the 0.5 cm grid over the phantom, batch moments, grid read
@@ -262,6 +374,8 @@ def main():
     ap.add_argument("--no-ct", action="store_true", help="skip the 1e8-history CT helical run")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the 64-scene batched sweep (prints its JSON object)")
+    ap.add_argument("--configs-table", action="store_true",
+                    help="only the per-configuration events, voxel crossings, bytes and CPU / GPU rates")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (PMC byte-counter calibration)")
     ap.add_argument("--calibrate-atomics", action="store_true",
@@ -288,6 +402,10 @@ def main():
               "(histories shard across GPUs only in the design, not in this build)", file=sys.stderr)
         sys.exit(2)
 
+    if args.configs_table:
+        from chatcarlo_amd.transport import default_engine
+        print(json.dumps(configs_table(default_engine())), flush=True)
+        return
     if args.batched_only:
         from chatcarlo_amd.transport import default_engine
         print(json.dumps(batched_sweep(default_engine())), flush=True)
