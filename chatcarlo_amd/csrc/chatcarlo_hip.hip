@@ -20,6 +20,7 @@
 #include <condition_variable>
 #include <mutex>
 #include <thread>
+#include <system_error>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -2769,31 +2770,40 @@ struct CopyPool {
   const unsigned char* src = nullptr;
   size_t len = 0;
   uint64_t gen = 0;
-  int pending = 0;
+  int pending = 0, parts = 0;
   bool stop = false;
+  // Starts up to n workers; if the system refuses a thread, the pool keeps those it has (none:
+  // every copy runs on the calling thread).
   void start(int n) {
-    for (int k = 0; k < n; ++k)
-      th.emplace_back([this, k, n] {
-        uint64_t seen = 0;
-        for (;;) {
-          unsigned char* d;
-          const unsigned char* sp;
-          size_t l;
-          {
-            std::unique_lock<std::mutex> lk(mu);
-            go.wait(lk, [&] { return stop || gen != seen; });
-            if (stop) return;
-            seen = gen;
-            d = dst;
-            sp = src;
-            l = len;
+    for (int k = 0; k < n; ++k) {
+      try {
+        th.emplace_back([this, k] {
+          uint64_t seen = 0;
+          for (;;) {
+            unsigned char* d;
+            const unsigned char* sp;
+            size_t l;
+            int np;
+            {
+              std::unique_lock<std::mutex> lk(mu);
+              go.wait(lk, [&] { return stop || gen != seen; });
+              if (stop) return;
+              seen = gen;
+              d = dst;
+              sp = src;
+              l = len;
+              np = parts;
+            }
+            const size_t a = l * (size_t)k / (size_t)np, b = l * (size_t)(k + 1) / (size_t)np;
+            if (b > a) std::memcpy(d + a, sp + a, b - a);
+            std::lock_guard<std::mutex> lk(mu);
+            if (--pending == 0) done.notify_one();
           }
-          const size_t a = l * (size_t)k / (size_t)n, b = l * (size_t)(k + 1) / (size_t)n;
-          if (b > a) std::memcpy(d + a, sp + a, b - a);
-          std::lock_guard<std::mutex> lk(mu);
-          if (--pending == 0) done.notify_one();
-        }
-      });
+        });
+      } catch (const std::system_error&) {
+        break;
+      }
+    }
   }
   void copy(void* d, const void* sp, size_t l) {
     if (th.empty() || l < (1u << 20)) {
@@ -2804,7 +2814,7 @@ struct CopyPool {
     dst = (unsigned char*)d;
     src = (const unsigned char*)sp;
     len = l;
-    pending = (int)th.size();
+    pending = parts = (int)th.size();
     ++gen;
     go.notify_all();
     done.wait(lk, [&] { return pending == 0; });
