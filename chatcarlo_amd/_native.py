@@ -66,7 +66,9 @@ class CcSource(C.Structure):
                 ("v", C.c_double * 3), ("sid", C.c_double), ("width", C.c_double),
                 ("height", C.c_double), ("cos_half", C.c_double), ("iso", C.c_double * 3),
                 ("axis", C.c_double * 3), ("scan_length", C.c_double), ("spec_e", dp),
-                ("spec_cdf", dp)]
+                ("spec_cdf", dp), ("heel_bins", C.c_int32), ("heel_nz_lo", C.c_int32),
+                ("heel_nz_hi", C.c_int32), ("heel_pad", C.c_int32), ("heel_hu", C.c_double),
+                ("heel_hv", C.c_double), ("heel_span", C.c_double), ("heel_rel_flu", dp), ("heel_cdf", dp)]
 
 
 class CcRunOut(C.Structure):

@@ -62,6 +62,7 @@ constexpr int kMaxCrossings = 1 << 14;
 // (flight 21.6 / 19.5 / 18.9 / 18.8 / 18.9 ms), and every copy is folded and cleared per group.
 constexpr int kGridCopies = 4;
 constexpr int kTrialCap = 100000;
+constexpr int kHeelTrialCap = 4096;   // field points drawn for one photon with the heel effect
 
 enum { F_ABSORBED = 1, F_ESCAPED = 2, F_DEAD = 4 };
 // device counter slots (uint32)
@@ -106,6 +107,12 @@ struct Src {
   float scan;
   const float* spec_e;
   const float* spec_cdf;
+  // heel effect (heel_bins = 0: none): axis coefficients in the beam basis, half-span on the
+  // SID plane, bins of non-zero fluence, relative fluence per bin, one CDF row per bin
+  int heel_bins, heel_lo, heel_hi;
+  float heel_hu, heel_hv, heel_span;
+  const float* heel_flu;
+  const float* heel_cdf;
 };
 
 struct Region {            // spherical scoring region for H*(10) (operator position)
@@ -402,6 +409,46 @@ __device__ __forceinline__ void sample_source(const Src& s, Rng& rng, float p[3]
       pos[0] += s.axis[0] * sh; pos[1] += s.axis[1] * sh; pos[2] += s.axis[2] * sh;
     }
   }
+  // Heel effect: field points (u2, u3) are drawn until one is kept with the relative fluence
+  // at its heel coordinate (chatcarlo/source.py:136-149); hs is that coordinate.  A photon
+  // none of whose kHeelTrialCap points is kept gets energy 0, which the flight rejects
+  // (CC_DEV_ENERGY_RANGE): the run fails instead of keeping a biased point.
+  float hs = 0.f;
+  bool heel_fail = false;
+  if (s.heel_bins > 0) {
+    const float bw = 2.f * s.heel_span / (float)s.heel_bins;
+    for (int t = 0;; ++t) {
+      if (s.field == CC_FIELD_CONE) {
+        const float omc = s.omch * (1.f - u2);
+        const float c = 1.f - omc;
+        const float sn = sqrtf(fmaxf(omc * (2.f - omc), 0.f));
+        float sp, cp;
+        sincosf(kTwoPi * u3, &sp, &cp);
+        hs = s.sid * (sn / c) * (cp * s.heel_hu + sp * s.heel_hv);
+      } else {
+        hs = (-0.5f * s.width + s.width * u2) * s.heel_hu + (-0.5f * s.height + s.height * u3) * s.heel_hv;
+      }
+      // np.interp over the bin centres, end values beyond them
+      const float x = (hs + s.heel_span) / bw - 0.5f;
+      float keep;
+      if (x <= 0.f) {
+        keep = s.heel_flu[0];
+      } else if (x >= (float)(s.heel_bins - 1)) {
+        keep = s.heel_flu[s.heel_bins - 1];
+      } else {
+        const int k = (int)x;
+        const float f = x - (float)k;
+        keep = s.heel_flu[k] + f * (s.heel_flu[k + 1] - s.heel_flu[k]);
+      }
+      if (rng.next() < keep) break;
+      if (t + 1 >= kHeelTrialCap) {
+        heel_fail = true;
+        break;
+      }
+      u2 = rng.next();
+      u3 = rng.next();
+    }
+  }
   if (s.field == CC_FIELD_CONE) {
     // c = cos_half + (1 - cos_half) u; 1 - c = (1 - cos_half)(1 - u) keeps sin accurate near the axis
     float omc = s.omch * (1.f - u2);
@@ -424,14 +471,21 @@ __device__ __forceinline__ void sample_source(const Src& s, Rng& rng, float p[3]
     float inv = 1.f / sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
     for (int k = 0; k < 3; ++k) { d[k] = t[k] * inv; p[k] = pos[k]; }
   }
-  // rng.choice: first bin with cdf > U (searchsorted side="right")
+  // rng.choice: first bin with cdf > U (searchsorted side="right"); with the heel effect, over
+  // the CDF of the photon's heel bin, clamped to the bins of non-zero fluence (source.py:195-207)
+  const float* cdf = s.spec_cdf;
+  if (s.heel_bins > 0) {
+    const int b = min(max((int)((hs + s.heel_span) / (2.f * s.heel_span / (float)s.heel_bins)), s.heel_lo),
+                      s.heel_hi);
+    cdf = s.heel_cdf + (size_t)b * s.n_spec;
+  }
   float ue = rng.next();
   int lo = 0, hi = s.n_spec - 1;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
-    if (s.spec_cdf[mid] > ue) hi = mid; else lo = mid + 1;
+    if (cdf[mid] > ue) hi = mid; else lo = mid + 1;
   }
-  E = s.spec_e[lo];
+  E = heel_fail ? 0.f : s.spec_e[lo];
 }
 
 // Philox uniforms with the draws of the field offsets and the energy (calls
@@ -2882,7 +2936,7 @@ struct Ctx {
   uint32_t* h_cnt = nullptr;   // pinned
   // source scratch
   float* spec = nullptr;
-  int spec_cap = 0;
+  size_t spec_cap = 0;
   std::vector<hipEvent_t> ev;
   // learned importance
   Region reg{};
@@ -3396,16 +3450,65 @@ int check_src(const cc_source* s) {
       return fail(CC_ERR_ARG, "spectrum energy %g keV outside [1, 150] keV", s->spec_e[i]);
   if (s->field < 0 || s->field > 2) return fail(CC_ERR_ARG, "field kind %d", s->field);
   if (s->rotation == 2 && s->n_angles < 2) return fail(CC_ERR_ARG, "n_angles must be >= 2");
+  if (s->heel_bins < 0) return fail(CC_ERR_ARG, "heel_bins %d", s->heel_bins);
+  if (s->heel_bins > 0) {
+    if (s->field == CC_FIELD_PARALLEL) return fail(CC_ERR_ARG, "the heel effect needs a rect or cone field");
+    if (!s->heel_rel_flu || !s->heel_cdf) return fail(CC_ERR_ARG, "heel tables missing");
+    if (!(s->heel_span > 0.0)) return fail(CC_ERR_ARG, "heel_span %g must be > 0", s->heel_span);
+    if (s->heel_nz_lo < 0 || s->heel_nz_hi >= s->heel_bins || s->heel_nz_lo > s->heel_nz_hi)
+      return fail(CC_ERR_ARG, "heel non-zero bins [%d, %d] out of [0, %d)", s->heel_nz_lo, s->heel_nz_hi,
+                  s->heel_bins);
+    double top = 0.0;
+    for (int b = 0; b < s->heel_bins; ++b) {
+      const double f = s->heel_rel_flu[b];
+      if (!(f >= 0.0 && f <= 1.0)) return fail(CC_ERR_ARG, "heel relative fluence %g of bin %d", f, b);
+      top = std::max(top, f);
+      const double* row = s->heel_cdf + (size_t)b * s->n_spec;
+      for (int i = 1; i < s->n_spec; ++i)
+        if (!(row[i] >= row[i - 1])) return fail(CC_ERR_ARG, "heel CDF of bin %d decreases at %d", b, i);
+    }
+    if (!(top > 0.0)) return fail(CC_ERR_ARG, "heel relative fluence zero in every bin");
+  }
   return CC_OK;
 }
 
-// Source fields; the spectrum (energies then CDF, last CDF value 1) goes to h at [off, off + 2 n).
-void fill_src(const cc_source* s, Src* d, std::vector<float>& h, size_t off) {
-  for (int i = 0; i < s->n_spec; ++i) {
+// Floats of a source's tables on the device: spectrum energies and CDF, then with the heel effect
+// the relative fluences and one CDF row per heel bin.
+size_t src_floats(const cc_source* s) {
+  return 2 * (size_t)s->n_spec + (s->heel_bins > 0 ? (size_t)s->heel_bins * (1 + (size_t)s->n_spec) : 0);
+}
+
+// Source fields; the tables (spectrum energies, then its CDF with last value 1, then with the
+// heel effect the relative fluences and the per-bin CDF rows, each ending at 1) go to h at
+// [off, off + src_floats(s)), and the pointers of d to dev + off onwards, where the caller
+// copies h.
+void fill_src(const cc_source* s, Src* d, std::vector<float>& h, size_t off, const float* dev) {
+  const size_t n = (size_t)s->n_spec;
+  for (size_t i = 0; i < n; ++i) {
     h[off + i] = (float)s->spec_e[i];
-    h[off + s->n_spec + i] = (float)s->spec_cdf[i];
+    h[off + n + i] = (float)s->spec_cdf[i];
   }
-  h[off + 2 * s->n_spec - 1] = 1.0f;
+  h[off + 2 * n - 1] = 1.0f;
+  d->spec_e = dev + off;
+  d->spec_cdf = dev + off + n;
+  d->heel_bins = s->heel_bins;
+  d->heel_lo = s->heel_nz_lo;
+  d->heel_hi = s->heel_nz_hi;
+  d->heel_hu = (float)s->heel_hu;
+  d->heel_hv = (float)s->heel_hv;
+  d->heel_span = (float)s->heel_span;
+  d->heel_flu = nullptr;
+  d->heel_cdf = nullptr;
+  if (s->heel_bins > 0) {
+    const size_t nb = (size_t)s->heel_bins, f0 = off + 2 * n, c0 = f0 + nb;
+    for (size_t b = 0; b < nb; ++b) {
+      h[f0 + b] = (float)s->heel_rel_flu[b];
+      for (size_t i = 0; i < n; ++i) h[c0 + b * n + i] = (float)s->heel_cdf[b * n + i];
+      h[c0 + b * n + n - 1] = 1.0f;
+    }
+    d->heel_flu = dev + f0;
+    d->heel_cdf = dev + c0;
+  }
   d->field = s->field; d->n_spec = s->n_spec; d->n_angles = s->n_angles; d->rotation = s->rotation;
   for (int k = 0; k < 3; ++k) {
     d->pos[k] = (float)s->pos[k]; d->dir[k] = (float)s->dir[k];
@@ -3420,17 +3523,17 @@ void fill_src(const cc_source* s, Src* d, std::vector<float>& h, size_t off) {
 int make_src_impl(Ctx* c, const cc_source* s, Src* d) {
   int r = check_src(s);
   if (r) return r;
-  if (2 * s->n_spec > c->spec_cap) {
+  const size_t nf = src_floats(s);
+  if (nf > c->spec_cap) {
     if (c->spec) (void)hipFree(c->spec);
     c->spec = nullptr;
-    if ((r = dalloc(&c->spec, 2 * (size_t)s->n_spec))) return r;
-    c->spec_cap = 2 * s->n_spec;
+    c->spec_cap = 0;
+    if ((r = dalloc(&c->spec, nf))) return r;
+    c->spec_cap = nf;
   }
-  std::vector<float> h(2 * (size_t)s->n_spec);
-  fill_src(s, d, h, 0);
+  std::vector<float> h(nf);
+  fill_src(s, d, h, 0, c->spec);
   HIP_TRY(hipMemcpy(c->spec, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
-  d->spec_e = c->spec;
-  d->spec_cdf = c->spec + s->n_spec;
   return CC_OK;
 }
 
@@ -4111,6 +4214,7 @@ int cc_run(void* ctx, const cc_source* src, uint64_t n_hist, uint64_t seed, uint
   if (record && (imp || bias)) return fail(CC_ERR_ARG, "contribution records come from analog runs only");
   if ((imp || bias) && !c->have_imp) return fail(CC_ERR_STATE, "importance not set (cc_set_importance)");
   if (bias && !c->have_bias) return fail(CC_ERR_STATE, "source cells not built (cc_importance_cfg.source_bias)");
+  if (bias && src->heel_bins > 0) return fail(CC_ERR_ARG, "source biasing does not combine with the heel effect");
   HIP_TRY(hipSetDevice(c->device));
   std::memset(out, 0, sizeof(*out));
   Src s;
@@ -4857,6 +4961,9 @@ int cc_set_importance(void* ctx, const cc_importance_cfg* cfg, const cc_source* 
   if (!c->have_mlp[0]) return fail(CC_ERR_STATE, "no parameters set for map 0 (flight starts)");
   const bool need_cells = cfg->source_bias || !(cfg->cw > 0.f);
   if (need_cells && !src) return fail(CC_ERR_ARG, "the source is needed for the source cells");
+  // the cells stratify the sampler's fixed uniform slots, which the heel effect's rejection shifts
+  if (need_cells && src->heel_bins > 0)
+    return fail(CC_ERR_ARG, "source cells (source_bias, or cw from the cells) do not combine with the heel effect");
   if (cfg->source_bias && !(cfg->bias_mix >= 0.f && cfg->bias_mix <= 1.f))
     return fail(CC_ERR_ARG, "bias_mix must be in [0, 1]");
   HIP_TRY(hipSetDevice(c->device));
@@ -4950,7 +5057,7 @@ int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int
   free_batch(c);
   const int total_prims = prim_off[n_scenes];
   size_t nspec = 0;
-  for (int s = 0; s < n_scenes; ++s) nspec += 2 * (size_t)sources[s].n_spec;
+  for (int s = 0; s < n_scenes; ++s) nspec += src_floats(&sources[s]);
   int r;
   if ((r = dalloc(&c->b_prims, std::max(total_prims, 1))) || (r = dalloc(&c->b_prim_off, n_scenes + 1)) ||
       (r = dalloc(&c->b_worlds, n_scenes)) || (r = dalloc(&c->b_grids, n_scenes)) ||
@@ -4971,10 +5078,8 @@ int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int
   std::vector<Src> src(n_scenes);
   size_t so = 0;
   for (int s = 0; s < n_scenes; ++s) {
-    fill_src(&sources[s], &src[s], hs, so);
-    src[s].spec_e = c->b_spec + so;
-    src[s].spec_cdf = c->b_spec + so + sources[s].n_spec;
-    so += 2 * (size_t)sources[s].n_spec;
+    fill_src(&sources[s], &src[s], hs, so, c->b_spec);
+    so += src_floats(&sources[s]);
   }
   HIP_TRY(hipMemcpy(c->b_spec, hs.data(), sizeof(float) * nspec, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->b_srcs, src.data(), sizeof(Src) * n_scenes, hipMemcpyHostToDevice));

@@ -5,9 +5,11 @@ here the host only resolves the scene's source section into a cc_source record
 (focal spot, beam basis, field, CT rotation / helical shift, spectrum CDF) and
 the source kernel samples every history from its own Philox stream.  Uniform
 slots per history: angle, helical shift, field coordinate a, field coordinate
-b, energy, tau.  The heel effect and mAs calibration need SpekPy (absent) and
-raise, as the reference does without SpekPy (chatcarlo/source.py:222-223,
-spectrum.py:100-102).
+b, energy, tau; with the heel effect, each rejected field point draws one more
+uniform and a new (a, b).  The heel effect's spectra come from
+spectrum.heel_spectra_for_source (SpekPy, absent here: it raises, as the
+reference does, chatcarlo/spectrum.py:100-102); the mAs calibration needs
+SpekPy as well and raises (chatcarlo/source.py:222-223).
 """
 from __future__ import annotations
 
@@ -16,6 +18,7 @@ import math
 import numpy as np
 
 from . import _native as N
+from . import spectrum as _spectrum
 from .spectrum import spectrum_table
 
 _AXES = {"x": np.array([1.0, 0.0, 0.0]), "y": np.array([0.0, 1.0, 0.0]), "z": np.array([0.0, 0.0, 1.0])}
@@ -37,20 +40,77 @@ def cone_half_angle_rad(fld: dict) -> float:
     return math.atan((fld["diameter_cm"] / 2.0) / fld["sid_cm"])
 
 
+def heel_axis_coeffs(src: dict, d, u, v):
+    """The anode direction projected on the plane across the beam, as coefficients of the beam
+    basis (u, v), so that the heel axis turns with a rotating tube (chatcarlo/source.py:43-56)."""
+    a = np.asarray(src["anode_direction"], dtype=float)
+    a = a - (a @ d) * d
+    norm = np.linalg.norm(a)
+    if norm < 1e-9:
+        raise ValueError("source.anode_direction is parallel to the beam: the anode-cathode axis needs a "
+                         "component across the central axis")
+    a = a / norm
+    return float(a @ u), float(a @ v)
+
+
+def heel_span_cm(fld: dict, hu: float, hv: float) -> float:
+    """Half-width of the field along the heel axis on the SID plane (chatcarlo/source.py:59-64)."""
+    if fld.get("shape", "rect") == "cone":
+        return fld["diameter_cm"] / 2.0
+    w, h = fld["size_cm"]
+    return abs(hu) * w / 2.0 + abs(hv) * h / 2.0
+
+
+def heel_tables(centres, spectra, rel_flu):
+    """The heel spectra as sampler tables: one energy grid (every energy some bin emits at),
+    each bin's CDF on it, and the first and last bin of non-zero fluence.  A bin that emits
+    nothing (beyond the anode cut-off) takes the CDF of the nearest bin that does; the sampler
+    never draws from it.  Returns (energies, cdf rows, relative fluences, lo, hi)."""
+    rel = np.asarray(rel_flu, dtype=float)
+    nz = np.flatnonzero(rel > 0)
+    if len(nz) == 0:
+        raise ValueError("heel effect: zero fluence in every bin")
+    grid = np.unique(np.concatenate([np.asarray(e, dtype=float)[np.asarray(w) > 0] for e, w in spectra]))
+    if grid[0] < 1.0 or grid[-1] > 150.0:
+        raise ValueError(f"heel spectra emit at {grid[0]:g}-{grid[-1]:g} keV, outside [1, 150] keV")
+    rows = np.zeros((len(spectra), len(grid)))
+    for k, (e, w) in enumerate(spectra):
+        e, w = np.asarray(e, dtype=float), np.asarray(w, dtype=float)
+        on = w > 0
+        np.add.at(rows[k], np.searchsorted(grid, e[on]), w[on])
+    for k in range(len(spectra)):
+        if rel[k] <= 0 or rows[k].sum() <= 0:
+            rows[k] = rows[nz[np.argmin(np.abs(nz - k))]]
+    cdf = np.cumsum(rows, axis=1)
+    cdf /= cdf[:, -1:]
+    return grid, cdf, rel, int(nz[0]), int(nz[-1])
+
+
 class SourceRecord:
     """A cc_source plus the arrays it points to (kept alive with it)."""
 
     def __init__(self, src: dict):
-        if src.get("heel_effect"):
-            raise RuntimeError("the heel effect needs spekpy, which is not installed")
-        self.spec_e, self.spec_cdf = spectrum_table(src)
-        self.spec_e = np.ascontiguousarray(self.spec_e, dtype=np.float64)
-        self.spec_cdf = np.ascontiguousarray(self.spec_cdf, dtype=np.float64)
         d = np.asarray(src["direction"], dtype=float)
         d = d / np.linalg.norm(d)
         u, v = beam_basis(d)
         fld = src["field"]
         shape = fld.get("shape", "rect")
+        heel = None
+        if src.get("heel_effect"):
+            if shape == "parallel":
+                raise ValueError("a parallel field cannot take source.heel_effect")
+            hu, hv = heel_axis_coeffs(src, d, u, v)
+            span = heel_span_cm(fld, hu, hv)
+            centres, spectra, rel, _ = _spectrum.heel_spectra_for_source(src, float(fld["sid_cm"]), span)
+            grid, cdf, rel, lo, hi = heel_tables(centres, spectra, rel)
+            heel = (hu, hv, span, rel, cdf, lo, hi)
+            # the plain spectrum fields: the bin nearest the central axis (unused by the sampler)
+            mid = min(max(len(rel) // 2, lo), hi)
+            self.spec_e, self.spec_cdf = grid, cdf[mid]
+        else:
+            self.spec_e, self.spec_cdf = spectrum_table(src)
+        self.spec_e = np.ascontiguousarray(self.spec_e, dtype=np.float64)
+        self.spec_cdf = np.ascontiguousarray(self.spec_cdf, dtype=np.float64)
         s = N.CcSource()
         s.field = _FIELD[shape]
         s.n_spec = len(self.spec_e)
@@ -75,6 +135,14 @@ class SourceRecord:
             s.scan_length = float(rot.get("scan_length_cm") or 0.0)
         s.spec_e = N.ptr(self.spec_e, N.C.c_double)
         s.spec_cdf = N.ptr(self.spec_cdf, N.C.c_double)
+        if heel is not None:
+            hu, hv, span, rel, cdf, lo, hi = heel
+            self.heel_rel_flu = np.ascontiguousarray(rel, dtype=np.float64)
+            self.heel_cdf = np.ascontiguousarray(cdf, dtype=np.float64)
+            s.heel_bins, s.heel_nz_lo, s.heel_nz_hi = len(rel), lo, hi
+            s.heel_hu, s.heel_hv, s.heel_span = hu, hv, span
+            s.heel_rel_flu = N.ptr(self.heel_rel_flu, N.C.c_double)
+            s.heel_cdf = N.ptr(self.heel_cdf, N.C.c_double)
         self.struct = s
 
 

@@ -1,10 +1,10 @@
-"""Source energy spectra: explicit list or the Kramers + Al fallback.
+"""Source energy spectra: explicit list or the Kramers + Al fallback, and the heel-axis spectra.
 
-Follows chatcarlo/spectrum.py:55-82.  SpekPy spectra (chatcarlo/spectrum.py:36-52,
-88-129) need the absent spekpy package and are not built; a scene that gives
-`kvp` without `spectrum` gets the reference's own SpekPy-absent fallback,
-Kramers' law E (kVp - E) on 60 bins from 5 keV to kVp attenuated by
-exp(-mu_Al t).
+Follows chatcarlo/spectrum.py:55-82.  A scene that gives `kvp` without `spectrum` gets the
+reference's own SpekPy-absent fallback, Kramers' law E (kVp - E) on 60 bins from 5 keV to kVp
+attenuated by exp(-mu_Al t).  The heel effect's off-axis spectra (chatcarlo/spectrum.py:88-152)
+come from SpekPy when it is installed and raise otherwise, as the reference does; the GPU
+sampler takes them as data (per-bin CDFs and relative fluences, chatcarlo_amd/source.py).
 """
 from __future__ import annotations
 
@@ -59,3 +59,48 @@ def spectrum_table(src: dict):
     cdf = np.cumsum(w)
     cdf /= cdf[-1]
     return e, cdf
+
+
+HEEL_N_BINS = 15   # bins along the heel axis (chatcarlo/spectrum.py:85)
+_heel_cache: dict = {}
+
+
+def heel_spectra_for_source(src: dict, sid: float, span_cm: float, n_bins: int = HEEL_N_BINS):
+    """Spectra along the heel axis (chatcarlo/spectrum.py:88-152): bin centres s on the SID
+    plane over [-span_cm, span_cm] (s > 0 on the anode side), each bin's (energies, normalised
+    weights), the relative fluence (largest 1) and the fluence per mAs.  SpekPy's off-axis
+    spectrum at lateral offset -s gives each bin (its x < 0 is the anode side).  Bins beyond the
+    anode cut-off emit nothing; that is warned about, and a field with no emission raises.
+    Without spekpy this raises RuntimeError, as the reference does."""
+    kvp = float(src["kvp"])
+    filt = float(src.get("filtration_mm_al", 2.5))
+    angle = float(src.get("anode_angle_deg", 12.0))
+    key = (kvp, filt, angle, float(sid), float(span_cm), int(n_bins))
+    if key in _heel_cache:
+        return _heel_cache[key]
+    try:
+        import spekpy
+    except ImportError:
+        raise RuntimeError("the heel effect needs spekpy, which is not installed") from None
+    centres = (np.arange(n_bins) + 0.5) * (2.0 * span_cm / n_bins) - span_cm
+    spectra, flu = [], np.zeros(n_bins)
+    for k, s in enumerate(centres):
+        spk = spekpy.Spek(kvp=kvp, th=angle, z=float(sid), x=-float(s))
+        spk.filter("Al", filt)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")   # zero emission beyond the cut-off is reported below
+            e, phi = spk.get_spectrum(edges=False)
+            w = np.clip(np.asarray(phi, dtype=float), 0.0, None)
+            if w.sum() > 0:
+                flu[k] = float(spk.get_flu())
+                w = w / w.sum()
+        spectra.append((np.asarray(e, dtype=float), w))
+    if not flu.max() > 0:
+        raise ValueError("heel effect: no emission in any bin along the heel axis; check the anode angle "
+                         "and the field")
+    if (flu <= 0).any():
+        warnings.warn("the anode-side edge of the field lies beyond the anode cut-off (take-off angle 0): "
+                      "no photons are emitted there; narrow the field or steepen the anode angle", stacklevel=2)
+    out = (centres, tuple(spectra), flu / flu.max(), flu)
+    _heel_cache[key] = out
+    return out

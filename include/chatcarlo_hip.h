@@ -91,7 +91,7 @@ typedef struct {
 
 enum { CC_FIELD_RECT = 0, CC_FIELD_CONE = 1, CC_FIELD_PARALLEL = 2 };
 
-/* Source (chatcarlo/source.py:67-210 without heel, spectrum.py:73-82). */
+/* Source (chatcarlo/source.py:67-210, spectrum.py:73-82). */
 typedef struct {
   int32_t field, n_spec, n_angles, rotation;   /* rotation: 0 none, 1 continuous, 2 discrete */
   double pos[3], dir[3], u[3], v[3];
@@ -100,6 +100,18 @@ typedef struct {
   double scan_length;
   const double* spec_e;      /* n_spec energies keV */
   const double* spec_cdf;    /* n_spec cumulative probabilities, last = 1 */
+  /* Heel effect (chatcarlo/source.py:43-65,128-149,195-207); heel_bins = 0: none.  The heel
+   * axis has coefficients (heel_hu, heel_hv) in the beam basis (u, v), so it turns with a rotating
+   * tube.  A photon's heel coordinate on the SID plane is s = a heel_hu + b heel_hv (rect field,
+   * a and b its offsets) or sid tan(theta) (cos(phi) heel_hu + sin(phi) heel_hv) (cone field).
+   * Field points are drawn until one is kept, with probability heel_rel_flu interpolated at s
+   * over the bin centres of [-heel_span, heel_span] (heel_bins equal bins).  The photon then
+   * draws its energy from spec_e with the CDF of its bin, the bin clamped to [heel_nz_lo,
+   * heel_nz_hi], the bins of non-zero fluence. */
+  int32_t heel_bins, heel_nz_lo, heel_nz_hi, heel_pad;
+  double heel_hu, heel_hv, heel_span;
+  const double* heel_rel_flu;  /* heel_bins relative fluences in [0, 1] */
+  const double* heel_cdf;      /* heel_bins rows of n_spec cumulative probabilities on spec_e, each ending at 1 */
 } cc_source;
 
 enum {

@@ -1,9 +1,12 @@
-"""Oracle source sampling (chatcarlo/source.py:20-35, 67-210 without heel).
+"""Oracle source sampling (chatcarlo/source.py:20-35, 67-210).
 
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 Pinned by the closed-form checks of the reference's tests/test_cone_beam.py,
 test_ct_rotation.py and test_parallel_beam.py (cone opening, rotation radius,
 helical shift range, parallel-beam footprint), restated in tests/test_source.py.
+The heel effect (sample_source_heel) is pinned in tests/test_source.py by the
+closed forms a rect field gives: the kept heel coordinates follow the
+interpolated relative fluence, and each heel bin's energies its own spectrum.
 """
 from __future__ import annotations
 
@@ -140,3 +143,62 @@ def source_from_uniforms(src, u, spec_e, spec_cdf):
     k = np.searchsorted(spec_cdf, u[:, 4], side="right")
     energies = spec_e[np.minimum(k, len(spec_e) - 1)]
     return origins, dirs, energies
+
+
+def sample_source_heel(src, n, rng, hu, hv, span, centres, spectra, rel_flu):
+    """Source photons with the heel effect, as chatcarlo/source.py:128-149,195-207 draws them:
+    field points (rect: offsets a, b; cone: cos theta, phi) are drawn in rounds and each kept
+    with probability interp(s, centres, rel_flu) at its heel coordinate s, until n are kept;
+    each photon's energy comes from the spectrum of its bin of [-span, span] (clamped to the
+    bins of non-zero fluence).  Returns origins, directions, energies and the heel coordinates."""
+    pos = np.asarray(src["position"], dtype=float)
+    d = np.asarray(src["direction"], dtype=float)
+    d = d / np.linalg.norm(d)
+    fld = src["field"]
+    sid = float(fld["sid_cm"])
+    u, v = beam_basis(d)
+    rot = src.get("rotation")
+    if rot is not None:
+        iso = np.asarray(rot["isocenter"], dtype=float)
+        axis = AXES[rot.get("axis", "z")]
+        angles = rng.uniform(0.0, 2.0 * np.pi, size=n)
+        origins = iso[None, :] + rotate_batch(pos - iso, axis, angles)
+        d_a, u_a, v_a = (rotate_batch(x, axis, angles) for x in (d, u, v))
+    else:
+        origins = np.tile(pos, (n, 1))
+        d_a, u_a, v_a = (np.tile(x, (n, 1)) for x in (d, u, v))
+    cone = fld.get("shape", "rect") == "cone"
+    if cone:
+        cos_half = math.cos(math.atan((fld["diameter_cm"] / 2.0) / sid))
+    else:
+        w, h = fld["size_cm"]
+    kept = [np.empty(0)] * 3
+    while len(kept[0]) < n:
+        m = max(int((n - len(kept[0])) * 1.8), 1024)
+        if cone:
+            p1 = rng.uniform(cos_half, 1.0, m)
+            p2 = rng.uniform(0.0, 2.0 * np.pi, m)
+            s = sid * np.sqrt(1.0 - p1 * p1) / p1 * (np.cos(p2) * hu + np.sin(p2) * hv)
+        else:
+            p1 = rng.uniform(-w / 2, w / 2, m)
+            p2 = rng.uniform(-h / 2, h / 2, m)
+            s = p1 * hu + p2 * hv
+        keep = rng.random(m) < np.interp(s, centres, rel_flu)
+        kept = [np.concatenate([a, b[keep]]) for a, b in zip(kept, (s, p1, p2))]
+    s, p1, p2 = (a[:n] for a in kept)
+    if cone:
+        sn = np.sqrt(1.0 - p1 * p1)
+        dirs = p1[:, None] * d_a + (sn * np.cos(p2))[:, None] * u_a + (sn * np.sin(p2))[:, None] * v_a
+    else:
+        t = d_a * sid + p1[:, None] * u_a + p2[:, None] * v_a
+        dirs = t / np.linalg.norm(t, axis=1, keepdims=True)
+    nb = len(centres)
+    bins = np.clip(((s + span) / (2.0 * span / nb)).astype(int), 0, nb - 1)
+    nz = np.flatnonzero(np.asarray(rel_flu) > 0)
+    bins = np.clip(bins, nz.min(), nz.max())
+    energies = np.empty(n)
+    for b in np.unique(bins):
+        sel = bins == b
+        e_b, w_b = spectra[b]
+        energies[sel] = np.asarray(e_b)[rng.choice(len(e_b), size=int(sel.sum()), p=np.asarray(w_b))]
+    return origins, dirs, energies, s

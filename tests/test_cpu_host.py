@@ -36,6 +36,8 @@ def test_ctypes_layouts_match_the_header(tmp_path):
     from chatcarlo_amd import _native as N
     checks = [("sizeof(cc_prim)", C.sizeof(N.CcPrim)), ("sizeof(cc_world)", C.sizeof(N.CcWorld)),
               ("sizeof(cc_tables)", C.sizeof(N.CcTables)), ("sizeof(cc_source)", C.sizeof(N.CcSource)),
+              ("offsetof(cc_source, heel_span)", N.CcSource.heel_span.offset),
+              ("offsetof(cc_source, heel_cdf)", N.CcSource.heel_cdf.offset),
               ("sizeof(cc_run_out)", C.sizeof(N.CcRunOut)),
               ("offsetof(cc_run_out, edep_keV)", N.CcRunOut.edep_keV.offset),
               ("offsetof(cc_run_out, region_sum)", N.CcRunOut.region_sum.offset),

@@ -1,0 +1,146 @@
+"""The heel effect on the GPU (chatcarlo/source.py:128-149,195-207), with heel spectra from
+the test provider in heel_model.py (SpekPy, which makes them in the reference, is absent):
+the kept heel coordinates follow the interpolated relative fluence and each heel bin's
+energies its spectrum (exact integrals, chi-square), the GPU samples agree with the oracle's
+heel sampler, and the reference's tests/test_heel_effect.py checks hold (fewer and harder
+photons on the anode side, symmetric without the effect, with cone and rotation, the anode
+cut-off).  Batched runs with a heel source equal single-scene runs."""
+import numpy as np
+import pytest
+from scipy import stats
+
+from heel_model import expected_coordinate_probabilities, make_provider
+
+pytestmark = pytest.mark.gpu
+
+SRC = {"kvp": 120.0, "filtration_mm_al": 2.5, "anode_angle_deg": 12.0, "position": [0.0, 0.0, 170.0],
+       "direction": [0.0, 0.0, -1.0], "heel_effect": True, "anode_direction": [1.0, 0.0, 0.0],
+       "field": {"shape": "rect", "size_cm": [40.0, 40.0], "sid_cm": 100.0}}
+
+
+@pytest.fixture
+def heel(monkeypatch):
+    from chatcarlo_amd import spectrum
+    prov = make_provider()
+    monkeypatch.setattr(spectrum, "heel_spectra_for_source", prov)
+    return prov
+
+
+def _sample(src, n, seed=53):
+    from chatcarlo_amd.source import sample_source_photons
+    return sample_source_photons(src, n, np.random.default_rng(seed))
+
+
+def _axis(src):
+    from chatcarlo_amd.source import beam_basis, heel_axis_coeffs, heel_span_cm
+    d = np.asarray(src["direction"], dtype=float)
+    d = d / np.linalg.norm(d)
+    u, v = beam_basis(d)
+    hu, hv = heel_axis_coeffs(src, d, u, v)
+    return d, u, v, hu, hv, heel_span_cm(src["field"], hu, hv)
+
+
+def _heel_coordinate(src, dirs):
+    """A photon's heel coordinate from its direction (static beam): its point on the SID plane."""
+    d, u, v, hu, hv, _ = _axis(src)
+    t = dirs / (dirs @ d)[:, None] * src["field"]["sid_cm"]
+    return (t @ u) * hu + (t @ v) * hv
+
+
+def test_heel_coordinates_and_energies_match_the_closed_forms_and_the_oracle(heel):
+    from oracle.source import sample_source_heel
+    n = 400_000
+    _, dirs, e = _sample(SRC, n)
+    s = _heel_coordinate(SRC, dirs)
+    d, u, v, hu, hv, span = _axis(SRC)
+    centres, spectra, rel, _ = heel(SRC, 100.0, span)
+    edges = np.linspace(-span, span, 41)
+    obs = np.histogram(s, edges)[0]
+    exp = expected_coordinate_probabilities(edges, centres, rel, -span, span) * n
+    chi2 = ((obs - exp) ** 2 / exp).sum()
+    print(f"heel coordinates: chi2 {chi2:.1f} over {len(obs)} bins", flush=True)
+    assert stats.chi2.sf(chi2, len(obs) - 1) > 1e-3, chi2
+    nb = len(centres)
+    bins = np.clip(((s + span) / (2 * span / nb)).astype(int), 0, nb - 1)
+    grp = np.arange(0.0, 130.0, 10.0)
+    for b in range(nb):
+        eb, wb = spectra[b]
+        got = np.histogram(e[bins == b], grp)[0]
+        exp_b = np.histogram(eb, grp, weights=wb)[0] * got.sum()
+        keep = exp_b > 5
+        chi2 = ((got[keep] - exp_b[keep]) ** 2 / exp_b[keep]).sum()
+        assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-4, (b, chi2)
+    # against the oracle's heel sampler (an independent stream): two-sample chi-square of the
+    # heel coordinates and of the energies
+    _, _, e_o, s_o = sample_source_heel(SRC, n, np.random.default_rng(9), hu, hv, span, centres, spectra, rel)
+    for a, b_, ed in ((s, s_o, edges), (e, e_o, grp)):
+        table = np.array([np.histogram(a, ed)[0], np.histogram(b_, ed)[0]])
+        table = table[:, table.sum(axis=0) > 10]
+        assert stats.chi2_contingency(table)[1] > 1e-4
+
+
+def test_fewer_photons_on_anode_side(heel):
+    """tests/test_heel_effect.py:35-41."""
+    _, dirs, _ = _sample(SRC, 60_000)
+    assert (dirs[:, 0] > 0).sum() < 0.92 * (dirs[:, 0] < 0).sum()
+
+
+def test_harder_spectrum_on_anode_side(heel):
+    """tests/test_heel_effect.py:44-50."""
+    _, dirs, e = _sample(SRC, 60_000)
+    s = dirs[:, 0] / (-dirs[:, 2]) * 100.0
+    assert e[s > 10.0].mean() > e[s < -10.0].mean() + 1.0
+
+
+def test_heel_off_is_symmetric():
+    """tests/test_heel_effect.py:53-57."""
+    _, dirs, _ = _sample({**SRC, "heel_effect": False}, 60_000)
+    assert abs(int((dirs[:, 0] > 0).sum()) - int((dirs[:, 0] < 0).sum())) < 5 * np.sqrt(len(dirs))
+
+
+def test_heel_composes_with_cone_and_rotation(heel):
+    """tests/test_heel_effect.py:60-74: the anode axis along the rotation axis stays the world x
+    axis, so the anode side keeps fewer photons."""
+    src = {**SRC, "field": {"shape": "cone", "diameter_cm": 20.0, "sid_cm": 60.0},
+           "rotation": {"isocenter": [0.0, 0.0, 110.0], "axis": "x"}}
+    _, dirs, e = _sample(src, 20_000)
+    assert np.all(np.isfinite(dirs)) and np.all(np.isfinite(e))
+    np.testing.assert_allclose(np.linalg.norm(dirs, axis=1), 1.0, atol=1e-5)
+    assert (dirs[:, 0] > 0).sum() < (dirs[:, 0] < 0).sum()
+
+
+def test_anode_cutoff_warns_and_still_samples(monkeypatch):
+    """tests/test_heel_effect.py:77-88: bins beyond the cut-off emit nothing; the sampler still
+    draws, with no photons there."""
+    from chatcarlo_amd import spectrum
+    monkeypatch.setattr(spectrum, "heel_spectra_for_source", make_provider(cutoff_cm=10.0))
+    src = {**SRC, "field": {"shape": "cone", "diameter_cm": 30.0, "sid_cm": 60.0}}
+    with pytest.warns(UserWarning, match="cut-off"):
+        _, dirs, e = _sample(src, 5_000)
+    assert np.all(np.isfinite(e)) and np.all(e >= 1.0)
+    s = dirs[:, 0] / (-dirs[:, 2] + 1e-12) * 60.0
+    assert np.sum(s > 13.0) < 0.01 * len(s)
+
+
+def test_batched_heel_source_equals_a_single_scene_run(engine, heel):
+    """A heel source in a batched run draws every history as a single-scene run does."""
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd.batch import BatchedRun, SceneSpec, Tuning
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.source import SourceRecord
+    raw = {"source": {**SRC, "position": [0.0, 0.0, 100.0]},
+           "geometry": [{"name": "slab", "shape": "box", "material": "water", "center": [0, 0, 0],
+                         "size_cm": [30, 30, 10]}], "physics": {"fluorescence": True}}
+    grid = (np.array([-20.0, -20.0, -10.0]), (20, 20, 10), 2.0)
+    spec = SceneSpec(raw, grid, 30_000, 77)
+    engine.set_geometry(Geometry(raw["geometry"]))
+    engine.bind_grid(grid[0], grid[1], grid[2], False)
+    o = engine.run(SourceRecord(raw["source"]), spec.n_histories, spec.seed, 0, 4096, N.RUN_FLUORESCENCE | N.RUN_GRID)
+    k_ref = engine.read_grid(grid[1], moments=False)["kerma"]
+    b = BatchedRun([spec], engine)
+    st = b.run(Tuning(512, tally_copies=2, persistent=True, epoch=2))
+    assert (st.n_absorbed, st.n_escaped, st.scatter_sum, st.n_events) == (o.n_absorbed, o.n_escaped, o.scatter_sum,
+                                                                          o.n_events)
+    np.testing.assert_allclose(b.read(0)[0], k_ref, rtol=1e-9, atol=1e-12 * k_ref.max())
+    # the dose is heavier on the cathode side (x < 0) of the slab
+    assert k_ref[:10].sum() > k_ref[10:].sum()
