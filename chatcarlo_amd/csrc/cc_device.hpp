@@ -131,6 +131,11 @@ __device__ __forceinline__ float u01(uint32_t x) {   // (0, 1], float32-exact
 }
 
 // Per-history stream: counter (history lo, history hi, event counter, lane tag), key = seed.
+struct PhiloxRegRng;
+// Tag of a history's heel stream: the source sampler's heel-effect rejection draws from the
+// history's Philox stream under tag ^ kHeelTag, counters from 0.
+constexpr uint32_t kHeelTag = 0x48454C00u;
+
 struct PhiloxRng {
   uint2 key;
   uint32_t h_lo, h_hi, tag, ctr;
@@ -148,6 +153,7 @@ struct PhiloxRng {
     }
     return buf[pos++];
   }
+  __device__ __forceinline__ PhiloxRegRng heel_stream() const;
 };
 
 // The same stream as PhiloxRng, with the block's four uniforms in four registers picked by
@@ -174,7 +180,14 @@ struct PhiloxRegRng {
     ++pos;
     return v;
   }
+  __device__ __forceinline__ PhiloxRegRng heel_stream() const {
+    return PhiloxRegRng(((uint64_t)key.y << 32) | key.x, ((uint64_t)h_hi << 32) | h_lo, tag ^ kHeelTag, 0u);
+  }
 };
+
+__device__ __forceinline__ PhiloxRegRng PhiloxRng::heel_stream() const {
+  return PhiloxRegRng(((uint64_t)key.y << 32) | key.x, ((uint64_t)h_hi << 32) | h_lo, tag ^ kHeelTag, 0u);
+}
 
 // Caller-supplied uniforms (sub-kernel parity tests).
 struct BufRng {

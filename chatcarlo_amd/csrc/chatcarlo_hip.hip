@@ -378,8 +378,9 @@ __device__ __forceinline__ void rodrigues(const float v[3], const float ax[3], f
   out[2] = v[2] * ca + cz * sa + ax[2] * dt * (1.f - ca);
 }
 
-// Slots: [angle][shift][a][b] then [energy][tau] from the next block.
-template <class Rng>
+// Slots: [angle][shift][a][b] then [energy][tau] from the next block.  kHeel = false leaves the
+// heel effect out (for sources without it), so that a kernel holds no registers for it.
+template <bool kHeel = true, class Rng>
 __device__ __forceinline__ void sample_source(const Src& s, Rng& rng, float p[3], float d[3], float& E) {
   float u0 = rng.next(), u1 = rng.next(), u2 = rng.next(), u3 = rng.next();
   float pos[3] = {s.pos[0], s.pos[1], s.pos[2]};
@@ -409,14 +410,18 @@ __device__ __forceinline__ void sample_source(const Src& s, Rng& rng, float p[3]
       pos[0] += s.axis[0] * sh; pos[1] += s.axis[1] * sh; pos[2] += s.axis[2] * sh;
     }
   }
-  // Heel effect: field points (u2, u3) are drawn until one is kept with the relative fluence
-  // at its heel coordinate (chatcarlo/source.py:136-149); hs is that coordinate.  A photon
+  // Heel effect: field points are drawn until one is kept with the relative fluence at its heel
+  // coordinate hs (chatcarlo/source.py:136-149): the first is (u2, u3), later ones and every
+  // keep decision come from the heel stream.  A photon
   // none of whose kHeelTrialCap points is kept gets energy 0, which the flight rejects
   // (CC_DEV_ENERGY_RANGE): the run fails instead of keeping a biased point.
   float hs = 0.f;
   bool heel_fail = false;
-  if (s.heel_bins > 0) {
+  if (kHeel && s.heel_bins > 0) {
     const float bw = 2.f * s.heel_span / (float)s.heel_bins;
+    // the keep decisions and the redrawn points come from the history's own heel stream, so the
+    // sampler's other draws keep their fixed slots
+    PhiloxRegRng hr = rng.heel_stream();
     for (int t = 0;; ++t) {
       if (s.field == CC_FIELD_CONE) {
         const float omc = s.omch * (1.f - u2);
@@ -440,13 +445,13 @@ __device__ __forceinline__ void sample_source(const Src& s, Rng& rng, float p[3]
         const float f = x - (float)k;
         keep = s.heel_flu[k] + f * (s.heel_flu[k + 1] - s.heel_flu[k]);
       }
-      if (rng.next() < keep) break;
+      if (hr.next() < keep) break;
       if (t + 1 >= kHeelTrialCap) {
         heel_fail = true;
         break;
       }
-      u2 = rng.next();
-      u3 = rng.next();
+      u2 = hr.next();
+      u3 = hr.next();
     }
   }
   if (s.field == CC_FIELD_CONE) {
@@ -474,7 +479,7 @@ __device__ __forceinline__ void sample_source(const Src& s, Rng& rng, float p[3]
   // rng.choice: first bin with cdf > U (searchsorted side="right"); with the heel effect, over
   // the CDF of the photon's heel bin, clamped to the bins of non-zero fluence (source.py:195-207)
   const float* cdf = s.spec_cdf;
-  if (s.heel_bins > 0) {
+  if (kHeel && s.heel_bins > 0) {
     const int b = min(max((int)((hs + s.heel_span) / (2.f * s.heel_span / (float)s.heel_bins)), s.heel_lo),
                       s.heel_hi);
     cdf = s.heel_cdf + (size_t)b * s.n_spec;
@@ -500,6 +505,7 @@ struct CellRng {
     if (k >= 2 && k <= 4) u = lo[k - 2] + u * (1.0f / (float)kSrcSide);
     return u;
   }
+  __device__ __forceinline__ PhiloxRegRng heel_stream() const { return r.heel_stream(); }
 };
 
 __device__ __forceinline__ CellRng cell_rng(PhiloxRng& r, int cell) {
@@ -627,6 +633,7 @@ __global__ void k_commit_and_reset(Launch L, uint32_t limit, int next_slot) {
 // Starts the next histories of a streamed run in free pool slots: history h (counted from
 // the run's first) draws from the stream keyed by (seed, first + h), exactly as in the
 // source kernel of a batch-by-batch run, and goes to the tally slot of its batch.
+template <bool kHeel>
 __global__ __launch_bounds__(kBlock) void k_refill_stream(Launch L, Src s, int* act, uint32_t* cnt_act,
                                                           uint32_t limit) {
   const MultiScene& M = L.MS;
@@ -644,7 +651,7 @@ __global__ __launch_bounds__(kBlock) void k_refill_stream(Launch L, Src s, int* 
       q = M.free_stack[fc - 1 - t];
       PhiloxRng rng(L.seed, L.hist0 + (uint64_t)h, 0u, 0u);
       float p[3], d[3], E;
-      sample_source(s, rng, p, d, E);
+      sample_source<kHeel>(s, rng, p, d, E);
       const float tau = -logf(rng.next());
       // the whole record in four 16-byte stores: position, direction, energy, tau, weight 1,
       // counter, history, tag 0, no scatters, material unknown, no flags, batch slot
@@ -4093,8 +4100,12 @@ int run_stream(Ctx* c, const Src& s, uint64_t n_hist, uint64_t seed, uint64_t fi
     auto issue_chunk = [&](Lane& q) -> int {
       for (int j = 0; j < q.chunk; ++j) {
         q.tm->mark(K_SOURCE);
-        hipLaunchKernelGGL(k_refill_stream, dim3(nbk), dim3(kBlock), 0, q.st, q.L, s, q.act_cur, q.cnt + q.cur,
-                           lpool);
+        if (s.heel_bins > 0)
+          hipLaunchKernelGGL(k_refill_stream<true>, dim3(nbk), dim3(kBlock), 0, q.st, q.L, s, q.act_cur,
+                             q.cnt + q.cur, lpool);
+        else
+          hipLaunchKernelGGL(k_refill_stream<false>, dim3(nbk), dim3(kBlock), 0, q.st, q.L, s, q.act_cur,
+                             q.cnt + q.cur, lpool);
         hipLaunchKernelGGL(k_commit_and_reset, dim3(1), dim3(64), 0, q.st, q.L, lpool, q.nxt);
         q.tm->mark(K_FLIGHT);
         hipLaunchKernelGGL((k_flight<false, false, false, true>), dim3(nbk), dim3(kBlock), 0, q.st, q.L,
@@ -4961,7 +4972,8 @@ int cc_set_importance(void* ctx, const cc_importance_cfg* cfg, const cc_source* 
   if (!c->have_mlp[0]) return fail(CC_ERR_STATE, "no parameters set for map 0 (flight starts)");
   const bool need_cells = cfg->source_bias || !(cfg->cw > 0.f);
   if (need_cells && !src) return fail(CC_ERR_ARG, "the source is needed for the source cells");
-  // the cells stratify the sampler's fixed uniform slots, which the heel effect's rejection shifts
+  // the cells stratify the field point's uniforms, and the heel effect's rejection replaces a
+  // rejected point with one drawn outside its cell
   if (need_cells && src->heel_bins > 0)
     return fail(CC_ERR_ARG, "source cells (source_bias, or cw from the cells) do not combine with the heel effect");
   if (cfg->source_bias && !(cfg->bias_mix >= 0.f && cfg->bias_mix <= 1.f))

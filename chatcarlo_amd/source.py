@@ -5,8 +5,8 @@ here the host only resolves the scene's source section into a cc_source record
 (focal spot, beam basis, field, CT rotation / helical shift, spectrum CDF) and
 the source kernel samples every history from its own Philox stream.  Uniform
 slots per history: angle, helical shift, field coordinate a, field coordinate
-b, energy, tau; with the heel effect, each rejected field point draws one more
-uniform and a new (a, b).  The heel effect's spectra come from
+b, energy, tau; with the heel effect, the keep decisions and the field points
+that replace rejected ones come from the history's separate heel stream.  The heel effect's spectra come from
 spectrum.heel_spectra_for_source (SpekPy, absent here: it raises, as the
 reference does, chatcarlo/spectrum.py:100-102); the mAs calibration needs
 SpekPy as well and raises (chatcarlo/source.py:222-223).
