@@ -369,6 +369,7 @@ __device__ __forceinline__ void raise_err(int* err, int bits) { atomicOr(err, bi
 // source (chatcarlo/source.py:67-210, spectrum.py:82)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void rodrigues(const float v[3], const float ax[3], float ca, float sa, float out[3]) {
+#pragma clang fp contract(off)   // as in sample_source: the same rounding in every kernel
   float cx = ax[1] * v[2] - ax[2] * v[1];
   float cy = ax[2] * v[0] - ax[0] * v[2];
   float cz = ax[0] * v[1] - ax[1] * v[0];
@@ -382,6 +383,10 @@ __device__ __forceinline__ void rodrigues(const float v[3], const float ax[3], f
 // heel effect out (for sources without it), so that a kernel holds no registers for it.
 template <bool kHeel = true, class Rng>
 __device__ __forceinline__ void sample_source(const Src& s, Rng& rng, float p[3], float d[3], float& E) {
+  // Every operation rounded on its own (no fused multiply-adds chosen by the compiler), so that
+  // the kernels that inline the sampler (batch sources, streamed refills, the persistent kernels)
+  // draw bit-identical photons for a history whatever code surrounds them.
+#pragma clang fp contract(off)
   float u0 = rng.next(), u1 = rng.next(), u2 = rng.next(), u3 = rng.next();
   float pos[3] = {s.pos[0], s.pos[1], s.pos[2]};
   float dd[3] = {s.dir[0], s.dir[1], s.dir[2]};
