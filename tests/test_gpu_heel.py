@@ -144,3 +144,30 @@ def test_batched_heel_source_equals_a_single_scene_run(engine, heel):
     np.testing.assert_allclose(b.read(0)[0], k_ref, rtol=1e-9, atol=1e-12 * k_ref.max())
     # the dose is heavier on the cathode side (x < 0) of the slab
     assert k_ref[:10].sum() > k_ref[10:].sum()
+
+
+@pytest.mark.parametrize("form", ["rounds", "persistent"])
+def test_streamed_heel_run_equals_batch_by_batch(engine, heel, form):
+    """A heel source through the streamed refill (its heel instantiation) and the persistent
+    kernel draws the same photons as the batch-by-batch source kernel: equal counters, tallies to
+    FP64 summation order."""
+    from chatcarlo_amd import _native as N
+    from chatcarlo_amd.geometry import Geometry
+    from chatcarlo_amd.source import SourceRecord
+    raw = {"source": {**SRC, "position": [0.0, 0.0, 100.0]},
+           "geometry": [{"name": "slab", "shape": "box", "material": "water", "center": [0, 0, 0],
+                         "size_cm": [30, 30, 10]}]}
+    grid = (np.array([-20.0, -20.0, -10.0]), (20, 20, 10), 2.0)
+    engine.set_geometry(Geometry(raw["geometry"]))
+    rec = SourceRecord(raw["source"])
+    res = []
+    for extra in ({"rounds": 0, "persistent": N.RUN_PERSISTENT}[form], N.RUN_PER_BATCH):
+        engine.bind_grid(grid[0], grid[1], grid[2], True)
+        o = engine.run(rec, 200_000, 31, 5_000, 20_000, N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY | extra)
+        g = engine.read_grid(grid[1], moments=True)
+        res.append((np.array([o.n_absorbed, o.n_escaped, o.scatter_sum, o.n_fluorescence, o.n_events, o.n_flights,
+                              o.n_voxel_visits], dtype=np.int64), g))
+    (ca, ga), (cb, gb) = res
+    np.testing.assert_array_equal(ca, cb)
+    for k in ("kerma", "h10", "kerma_sum2", "h10_sum2"):
+        np.testing.assert_allclose(ga[k], gb[k], rtol=1e-9, atol=1e-13 * float(np.abs(gb[k]).max()), err_msg=k)
