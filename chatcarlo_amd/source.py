@@ -6,10 +6,11 @@ here the host only resolves the scene's source section into a cc_source record
 the source kernel samples every history from its own Philox stream.  Uniform
 slots per history: angle, helical shift, field coordinate a, field coordinate
 b, energy, tau; with the heel effect, the keep decisions and the field points
-that replace rejected ones come from the history's separate heel stream.  The heel effect's spectra come from
-spectrum.heel_spectra_for_source (SpekPy, absent here: it raises, as the
-reference does, chatcarlo/spectrum.py:100-102); the mAs calibration needs
-SpekPy as well and raises (chatcarlo/source.py:222-223).
+that replace rejected ones come from the history's separate heel stream.  The
+heel effect's spectra (spectrum.heel_spectra_for_source) and the mAs
+calibration's fluence (spectrum.fluence_per_mas) come from SpekPy, which is
+absent here: both raise, as the reference does without SpekPy
+(chatcarlo/spectrum.py:100-102, source.py:222-223).
 """
 from __future__ import annotations
 
@@ -84,6 +85,40 @@ def heel_tables(centres, spectra, rel_flu):
     cdf = np.cumsum(rows, axis=1)
     cdf /= cdf[:, -1:]
     return grid, cdf, rel, int(nz[0]), int(nz[-1])
+
+
+def photon_count_through_field(src: dict) -> float:
+    """Photons crossing the field for source.mas (chatcarlo/source.py:213-258): the fluence per
+    mAs at the SID times mAs times the field area.  Without the heel effect the fluence is the
+    central-axis value (spectrum.fluence_per_mas); with it, the mean over a 101 x 101 grid of the
+    field (inside the circle for a cone) of the heel bins' fluence per mAs interpolated at each
+    point's heel coordinate.  SpekPy supplies both and is absent here, so this raises, as the
+    reference does without SpekPy."""
+    mas = src.get("mas")
+    if mas is None:
+        raise ValueError("source.mas is not given: the photon-count calibration needs the tube current-time "
+                         "product in mAs")
+    fld = src["field"]
+    sid = float(fld["sid_cm"])
+    cone = fld.get("shape", "rect") == "cone"
+    area = math.pi * (fld["diameter_cm"] / 2.0) ** 2 if cone else float(fld["size_cm"][0]) * float(fld["size_cm"][1])
+    if not src.get("heel_effect"):
+        return _spectrum.fluence_per_mas(src, sid) * float(mas) * area
+    d = np.asarray(src["direction"], dtype=float)
+    d = d / np.linalg.norm(d)
+    u, v = beam_basis(d)
+    hu, hv = heel_axis_coeffs(src, d, u, v)
+    span = heel_span_cm(fld, hu, hv)
+    if cone:
+        g = np.linspace(-span, span, 101)
+        gu, gv = np.meshgrid(g, g)
+        s = (gu * hu + gv * hv)[gu ** 2 + gv ** 2 <= span ** 2]
+    else:
+        w, h = (float(x) for x in fld["size_cm"])
+        gu, gv = np.meshgrid(np.linspace(-w / 2, w / 2, 101), np.linspace(-h / 2, h / 2, 101))
+        s = (gu * hu + gv * hv).ravel()
+    centres, _, _, flu_abs = _spectrum.heel_spectra_for_source(src, sid, span)
+    return float(np.interp(s, centres, flu_abs).mean()) * float(mas) * area
 
 
 class SourceRecord:

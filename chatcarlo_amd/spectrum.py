@@ -61,6 +61,19 @@ def spectrum_table(src: dict):
     return e, cdf
 
 
+def fluence_per_mas(src: dict, sid: float) -> float:
+    """Photons per cm^2 per mAs on the central axis at distance sid (SpekPy's absolute fluence
+    behind the source's Al filtration, chatcarlo/source.py:255-258).  Without spekpy this raises,
+    as the reference does (the Kramers fallback has no absolute scale)."""
+    try:
+        import spekpy
+    except ImportError:
+        raise RuntimeError("photon-count calibration (mas) needs spekpy, which is not installed") from None
+    spk = spekpy.Spek(kvp=float(src["kvp"]), th=float(src.get("anode_angle_deg", 12.0)), z=float(sid), mas=1.0)
+    spk.filter("Al", float(src.get("filtration_mm_al", 2.5)))
+    return float(spk.get_flu())
+
+
 HEEL_N_BINS = 15   # bins along the heel axis (chatcarlo/spectrum.py:85)
 _heel_cache: dict = {}
 

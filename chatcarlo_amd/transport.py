@@ -364,8 +364,11 @@ def run_transport(scene, n_histories: int = 100_000, seed: int | None = None,
         flags |= N.RUN_GRID
         if track_uncertainty:
             flags |= N.RUN_TRACK_UNCERTAINTY
+    n_mas = None
     if src.get("mas") is not None and src.get("ctdi_vol_mGy") is None:
-        raise RuntimeError("photon-count calibration (mas) needs spekpy, which is not installed")
+        # before the run, so that a calibration that cannot be made (no SpekPy) fails at once
+        from .source import photon_count_through_field
+        n_mas = photon_count_through_field(src)
     rec = SourceRecord(src)
     out = eng.run(rec, n_histories, _seed_from(seed), 0, batch_size, flags)
     names = eng.pack.materials
@@ -386,7 +389,7 @@ def run_transport(scene, n_histories: int = 100_000, seed: int | None = None,
         if track_uncertainty:
             grid.n_batches = int(g["n_batches"])
             grid.n_histories = int(g["n_histories"])
-    n_real = None
+    n_real = n_mas   # chatcarlo/transport.py:448-449
     if src.get("ctdi_vol_mGy") is not None:
         # after the grid is read: the calibration binds the phantom's own scene and grid
         # (chatcarlo/transport.py:445-447, same master seed)

@@ -113,8 +113,8 @@ def test_anode_cutoff_warns_and_still_samples(monkeypatch):
     """tests/test_heel_effect.py:77-88: bins beyond the cut-off emit nothing; the sampler still
     draws, with no photons there."""
     from chatcarlo_amd import spectrum
-    monkeypatch.setattr(spectrum, "heel_spectra_for_source", make_provider(cutoff_cm=10.0))
-    src = {**SRC, "field": {"shape": "cone", "diameter_cm": 30.0, "sid_cm": 60.0}}
+    monkeypatch.setattr(spectrum, "heel_spectra_for_source", make_provider())
+    src = {**SRC, "field": {"shape": "cone", "diameter_cm": 30.0, "sid_cm": 60.0}}   # half-angle 14 > 12 degrees
     with pytest.warns(UserWarning, match="cut-off"):
         _, dirs, e = _sample(src, 5_000)
     assert np.all(np.isfinite(e)) and np.all(e >= 1.0)
@@ -171,3 +171,27 @@ def test_streamed_heel_run_equals_batch_by_batch(engine, heel, form):
     np.testing.assert_array_equal(ca, cb)
     for k in ("kerma", "h10", "kerma_sum2", "h10_sum2"):
         np.testing.assert_allclose(ga[k], gb[k], rtol=1e-9, atol=1e-13 * float(np.abs(gb[k]).max()), err_msg=k)
+
+
+def test_run_transport_with_mas_calibrates(monkeypatch):
+    """tests/test_calibration.py:58-81 with the model fluences in place of SpekPy's: without mas no
+    calibration; with mas, n_photons_real is the photon count through the field, with and without
+    the heel effect."""
+    from heel_model import make_central_fluence
+    from chatcarlo_amd import spectrum
+    from chatcarlo_amd.scene import validate_scene
+    from chatcarlo_amd.source import photon_count_through_field
+    from chatcarlo_amd.transport import run_transport
+    monkeypatch.setattr(spectrum, "fluence_per_mas", make_central_fluence())
+    monkeypatch.setattr(spectrum, "heel_spectra_for_source", make_provider())
+    geo = [{"name": "w", "shape": "box", "material": "water", "center": [0, 0, 70], "size_cm": [20, 20, 20]}]
+    src = {k: v for k, v in SRC.items() if k not in ("heel_effect", "anode_direction")}
+    plain = run_transport(validate_scene({"source": src, "geometry": geo}), n_histories=20_000, seed=2)
+    assert plain.n_photons_real is None
+    for extra in ({}, {"heel_effect": True, "anode_direction": [1.0, 0.0, 0.0]}):
+        s = {**src, **extra, "mas": 4.0}
+        sc = validate_scene({"source": s, "geometry": geo})
+        assert sc.ok, sc.errors
+        r = run_transport(sc, n_histories=20_000, seed=2)
+        assert r.n_photons_real == pytest.approx(photon_count_through_field(s), rel=1e-12)
+        assert r.n_photons_real > 0

@@ -85,8 +85,9 @@ def test_heel_tables_from_spectra():
     (drawing from it gives the bin's weights), bins beyond the cut-off take the nearest emitting
     bin's CDF and are outside [lo, hi]."""
     from chatcarlo_amd.source import heel_tables
-    with pytest.warns(UserWarning, match="cut-off"):
-        centres, spectra, rel, _ = make_provider(cutoff_cm=12.0)(SRC, 100.0, 20.0)
+    src = {**SRC, "field": {"shape": "cone", "diameter_cm": 30.0, "sid_cm": 60.0}}
+    with pytest.warns(UserWarning, match="cut-off"):   # atan(15 / 60) > 12 degrees
+        centres, spectra, rel, _ = make_provider()(src, 60.0, 15.0)
     grid, cdf, rel2, lo, hi = heel_tables(centres, spectra, rel)
     assert grid[0] >= 1.0 and np.all(np.diff(grid) > 0)
     assert lo == 0 and hi == int(np.flatnonzero(rel > 0)[-1]) and hi < len(centres) - 1
