@@ -1,8 +1,8 @@
 """Source energy spectra: explicit list or the Kramers + Al fallback, and the heel-axis spectra.
 
-Follows chatcarlo/spectrum.py:55-82.  A scene that gives `kvp` without `spectrum` gets the
-reference's own SpekPy-absent fallback, Kramers' law E (kVp - E) on 60 bins from 5 keV to kVp
-attenuated by exp(-mu_Al t).  The heel effect's off-axis spectra (chatcarlo/spectrum.py:88-152)
+Follows chatcarlo/spectrum.py:36-82.  A scene that gives `kvp` without `spectrum` gets SpekPy's
+spectrum when spekpy is installed, else the reference's own SpekPy-absent fallback, Kramers' law
+E (kVp - E) on 60 bins from 5 keV to kVp attenuated by exp(-mu_Al t), with a warning.  The heel effect's off-axis spectra (chatcarlo/spectrum.py:88-152)
 come from SpekPy when it is installed and raise otherwise, as the reference does; the GPU
 sampler takes them as data (per-bin CDFs and relative fluences, chatcarlo_amd/source.py).
 """
@@ -45,17 +45,42 @@ def kramers_spectrum_bins(kvp: float, filtration_mm_al: float, bin_keV: float = 
     return [{"energy_keV": float(e), "weight": float(x)} for e, x in zip(mid[keep], w)]
 
 
+_spectrum_cache: dict = {}
+
+
+def spekpy_spectrum(kvp: float, filtration_mm_al: float, anode_angle_deg: float):
+    """SpekPy's central-axis spectrum behind the Al filtration (chatcarlo/spectrum.py:36-52): bin
+    energies and normalised weights, negative weights clipped to 0; None without spekpy."""
+    key = (float(kvp), float(filtration_mm_al), float(anode_angle_deg))
+    if key not in _spectrum_cache:
+        try:
+            import spekpy
+        except ImportError:
+            return None
+        spk = spekpy.Spek(kvp=key[0], th=key[2])
+        spk.filter("Al", key[1])
+        e, phi = spk.get_spectrum(edges=False)
+        w = np.clip(np.asarray(phi, dtype=float), 0.0, None)
+        _spectrum_cache[key] = (np.asarray(e, dtype=float), w / w.sum())
+    return _spectrum_cache[key]
+
+
 def spectrum_table(src: dict):
-    """(energies keV, cumulative probabilities) of a source (chatcarlo/spectrum.py:73-82)."""
+    """(energies keV, cumulative probabilities) of a source (chatcarlo/spectrum.py:64-82): the
+    given list, else SpekPy's spectrum for the kVp, else the Kramers fallback with a warning."""
     spec = src.get("spectrum")
     if spec:
         e = np.array([s["energy_keV"] for s in spec], dtype=float)
         w = np.array([s["weight"] for s in spec], dtype=float)
         w = w / w.sum()
     else:
-        warnings.warn("spekpy is not available: using the Kramers-law fallback spectrum",
-                      stacklevel=2)
-        e, w = kramers_fallback_spectrum(src["kvp"], src.get("filtration_mm_al", 2.5))
+        sp = spekpy_spectrum(src["kvp"], src.get("filtration_mm_al", 2.5), src.get("anode_angle_deg", 12.0))
+        if sp is not None:
+            e, w = sp
+        else:
+            warnings.warn("spekpy is not available: using the Kramers-law fallback spectrum",
+                          stacklevel=2)
+            e, w = kramers_fallback_spectrum(src["kvp"], src.get("filtration_mm_al", 2.5))
     cdf = np.cumsum(w)
     cdf /= cdf[-1]
     return e, cdf

@@ -72,3 +72,22 @@ def test_heel_field_mean_below_the_central_axis_value(fluence):
     centres, _, _, flu = make_provider()(src, 100.0, 20.0)
     s = np.linspace(-20.0, 20.0, 101)
     assert n_heel == pytest.approx(np.interp(s, centres, flu).mean() * 10.0 * 1600.0, rel=1e-12)
+
+
+def test_kvp_spectrum_prefers_spekpy(monkeypatch):
+    """chatcarlo/spectrum.py:64-70: a kVp source takes SpekPy's spectrum when SpekPy is there, the
+    Kramers fallback with a warning otherwise; a given list wins over both."""
+    import warnings
+    from chatcarlo_amd import spectrum
+    monkeypatch.setattr(spectrum, "spekpy_spectrum", lambda kvp, f, a: (np.array([30.0, 50.0]), np.array([0.25, 0.75])))
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        e, cdf = spectrum.spectrum_table(BASE)
+    np.testing.assert_array_equal(e, [30.0, 50.0])
+    np.testing.assert_allclose(cdf, [0.25, 1.0])
+    e2, _ = spectrum.spectrum_table({**BASE, "spectrum": [{"energy_keV": 60.0, "weight": 1.0}]})
+    np.testing.assert_array_equal(e2, [60.0])
+    monkeypatch.setattr(spectrum, "spekpy_spectrum", lambda kvp, f, a: None)
+    with pytest.warns(UserWarning, match="Kramers"):
+        e3, _ = spectrum.spectrum_table(BASE)
+    assert len(e3) == 60
