@@ -2,9 +2,10 @@
 
 Follows chatcarlo/spectrum.py:36-82.  A scene that gives `kvp` without `spectrum` gets SpekPy's
 spectrum when spekpy is installed, else the reference's own SpekPy-absent fallback, Kramers' law
-E (kVp - E) on 60 bins from 5 keV to kVp attenuated by exp(-mu_Al t), with a warning.  The heel effect's off-axis spectra (chatcarlo/spectrum.py:88-152)
-come from SpekPy when it is installed and raise otherwise, as the reference does; the GPU
-sampler takes them as data (per-bin CDFs and relative fluences, chatcarlo_amd/source.py).
+E (kVp - E) on 60 bins from 5 keV to kVp attenuated by exp(-mu_Al t), with a warning.  The heel
+effect's off-axis spectra (chatcarlo/spectrum.py:88-152) and the mAs calibration's fluence come
+from SpekPy when it is installed and raise otherwise, as the reference does; the GPU sampler
+takes the heel spectra as data (per-bin CDFs and relative fluences, chatcarlo_amd/source.py).
 """
 from __future__ import annotations
 
@@ -81,6 +82,10 @@ def spectrum_table(src: dict):
             warnings.warn("spekpy is not available: using the Kramers-law fallback spectrum",
                           stacklevel=2)
             e, w = kramers_fallback_spectrum(src["kvp"], src.get("filtration_mm_al", 2.5))
+    # energies of zero weight are never drawn; dropping them keeps e.g. SpekPy's sub-keV bins, emptied
+    # by the filtration, out of the sampler's [1, 150] keV check
+    keep = w > 0
+    e, w = e[keep], w[keep]
     cdf = np.cumsum(w)
     cdf /= cdf[-1]
     return e, cdf
