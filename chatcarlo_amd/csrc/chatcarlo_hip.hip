@@ -584,6 +584,7 @@ __device__ __forceinline__ uint32_t refill_count(const MultiScene& M, uint32_t l
 // Starts the next histories of the index space in free pool slots and appends
 // them to the active list.  Scene s's history h draws from the stream keyed by
 // (seed_s, h), exactly as history h of a single-scene run with seed_s.
+template <bool kHeel>
 __global__ __launch_bounds__(kBlock) void k_refill(Launch L, int* act, uint32_t* cnt_act, uint32_t limit) {
   const MultiScene& M = L.MS;
   const unsigned long long cur = *M.cursor;
@@ -605,7 +606,7 @@ __global__ __launch_bounds__(kBlock) void k_refill(Launch L, int* act, uint32_t*
       q = M.free_stack[fc - 1 - t];
       PhiloxRng rng(M.seeds[lo], (uint64_t)h, 0u, 0u);
       float p[3], d[3], E;
-      sample_source(M.srcs[lo], rng, p, d, E);
+      sample_source<kHeel>(M.srcs[lo], rng, p, d, E);
       const float tau = -logf(rng.next());
       S.x[q] = p[0]; S.y[q] = p[1]; S.z[q] = p[2];
       S.dx[q] = d[0]; S.dy[q] = d[1]; S.dz[q] = d[2];
@@ -2971,6 +2972,7 @@ struct Ctx {
   // batched runs (many scenes in one pool)
   int nscene = 0;
   std::vector<uint64_t> scene_hist;   // histories per scene at fraction 1
+  bool b_heel = false;                // some scene's source has the heel effect
   Prim* b_prims = nullptr;
   int* b_prim_off = nullptr;
   World* b_worlds = nullptr;
@@ -3053,6 +3055,7 @@ void free_batch(Ctx* c) {
   c->b_vox_off.clear();
   c->b_grid_host.clear();
   c->scene_hist.clear();
+  c->b_heel = false;
 }
 
 // Feature frame without a scoring region: world-box centre, half the largest extent.
@@ -5113,6 +5116,7 @@ int cc_batch_upload(void* ctx, int32_t n_scenes, const cc_prim* prims, const int
   HIP_TRY(hipMemcpy(c->b_grids, c->b_grid_host.data(), sizeof(GridDev) * n_scenes, hipMemcpyHostToDevice));
   c->b_tally_cap = 2 * nvox;
   c->scene_hist.assign(n_histories, n_histories + n_scenes);
+  for (int s = 0; s < n_scenes; ++s) c->b_heel = c->b_heel || sources[s].heel_bins > 0;
   c->b_vox_off = voff;
   c->b_nvox = nvox;
   c->nscene = n_scenes;
@@ -5256,7 +5260,10 @@ static int batch_run(Ctx* c, const cc_batch_tuning* tune, double fraction, uint3
       const int cur = half == 0 ? C_ACT_A : C_ACT_B, nxt = half == 0 ? C_ACT_B : C_ACT_A;
       int* act_cur = half == 0 ? c->act0 : c->act1;
       int* act_nxt = half == 0 ? c->act1 : c->act0;
-      hipLaunchKernelGGL(k_refill, dim3(nb), dim3(bs), 0, c->stream, L, act_cur, c->cnt + cur, pool);
+      if (c->b_heel)
+        hipLaunchKernelGGL(k_refill<true>, dim3(nb), dim3(bs), 0, c->stream, L, act_cur, c->cnt + cur, pool);
+      else
+        hipLaunchKernelGGL(k_refill<false>, dim3(nb), dim3(bs), 0, c->stream, L, act_cur, c->cnt + cur, pool);
       hipLaunchKernelGGL(k_commit_and_reset, dim3(1), dim3(64), 0, c->stream, L, pool, nxt);
       hipLaunchKernelGGL((k_flight<false, true>), dim3(nb), dim3(bs), 0, c->stream, L, (const int*)act_cur,
                          (const uint32_t*)(c->cnt + cur), c->qp, c->qc, c->qr, (int*)nullptr);
