@@ -417,15 +417,14 @@ __device__ __forceinline__ void sample_source(const Src& s, Rng& rng, float p[3]
   }
   // Heel effect: field points are drawn until one is kept with the relative fluence at its heel
   // coordinate hs (chatcarlo/source.py:136-149): the first is (u2, u3), later ones and every
-  // keep decision come from the heel stream.  A photon
-  // none of whose kHeelTrialCap points is kept gets energy 0, which the flight rejects
-  // (CC_DEV_ENERGY_RANGE): the run fails instead of keeping a biased point.
+  // keep decision come from the heel stream.  A photon none of whose kHeelTrialCap points is
+  // kept gets energy 0, which the flight rejects (CC_DEV_ENERGY_RANGE): the run fails instead of
+  // keeping a biased point.
   float hs = 0.f;
   bool heel_fail = false;
   if (kHeel && s.heel_bins > 0) {
     const float bw = 2.f * s.heel_span / (float)s.heel_bins;
-    // the keep decisions and the redrawn points come from the history's own heel stream, so the
-    // sampler's other draws keep their fixed slots
+    // the history's own heel stream, so that the sampler's other draws keep their fixed slots
     PhiloxRegRng hr = rng.heel_stream();
     for (int t = 0;; ++t) {
       if (s.field == CC_FIELD_CONE) {
