@@ -148,11 +148,11 @@ def test_batched_heel_source_equals_a_single_scene_run(engine, heel):
     assert k_ref[:10].sum() > k_ref[10:].sum()
 
 
-@pytest.mark.parametrize("form", ["rounds", "persistent"])
-def test_streamed_heel_run_equals_batch_by_batch(engine, heel, form):
-    """A heel source through the streamed refill (its heel instantiation) and the persistent
-    kernel draws the same photons as the batch-by-batch source kernel: equal counters, tallies to
-    FP64 summation order."""
+@pytest.mark.parametrize("form,n", [("rounds", 200_000), ("persistent", 200_000), ("rounds", 2_500_000)])
+def test_streamed_heel_run_equals_batch_by_batch(engine, heel, form, n):
+    """A heel source through the streamed refill (its heel instantiation; 2.5e6 histories run in
+    three lanes) and the persistent kernel draws the same photons as the batch-by-batch source
+    kernel: equal counters, tallies to FP64 summation order."""
     from chatcarlo_amd import _native as N
     from chatcarlo_amd.geometry import Geometry
     from chatcarlo_amd.source import SourceRecord
@@ -165,7 +165,7 @@ def test_streamed_heel_run_equals_batch_by_batch(engine, heel, form):
     res = []
     for extra in ({"rounds": 0, "persistent": N.RUN_PERSISTENT}[form], N.RUN_PER_BATCH):
         engine.bind_grid(grid[0], grid[1], grid[2], True)
-        o = engine.run(rec, 200_000, 31, 5_000, 20_000, N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY | extra)
+        o = engine.run(rec, n, 31, 5_000, 20_000, N.RUN_FLUORESCENCE | N.RUN_GRID | N.RUN_TRACK_UNCERTAINTY | extra)
         g = engine.read_grid(grid[1], moments=True)
         res.append((np.array([o.n_absorbed, o.n_escaped, o.scatter_sum, o.n_fluorescence, o.n_events, o.n_flights,
                               o.n_voxel_visits], dtype=np.int64), g))
